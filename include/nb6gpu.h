@@ -1,0 +1,153 @@
+/*
+ * nb6gpu.h -- C ABI of libnb6gpu.so, the gfx950 (MI355X) regular-force,
+ * neighbour-list, potential and irregular-force library for Nbody6++GPU.
+ *
+ * Part 1 is the drop-in boundary: the exact symbols the Fortran integrator
+ * already calls, with gfortran/flang mangling (lower case, one trailing
+ * underscore, every argument by reference).  Part 2 is this library's own
+ * extension API (device-resident entry points, profiling, error status); the
+ * Fortran code never needs it.
+ *
+ * Array conventions (Fortran column-major seen from C):
+ *   X(3,N)           == double x[N][3]
+ *   LISTGP(LMAX,NI)  == int list[NI][LMAX]; list[i][0] = count, list[i][1..] = entries
+ */
+#ifndef NB6GPU_H
+#define NB6GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ===================================================================== *
+ * Part 1 -- reference ABI                                                *
+ * ===================================================================== */
+
+/* Select and initialise the device.  Replaces gpunb_devinit_
+ * (src/Main/gpunb.velocity.cu:906-908, GPUNB_devinit :575-627).  GPU_LIST is
+ * honoured like the reference (:583-598); the first listed device is used. */
+void gpunb_devinit_(int *irank);
+
+/* Open the regular-force library for up to *nbmax j-particles.  Replaces
+ * gpunb_open_ (gpunb.velocity.cu:909-911, GPUNB_open :629-667).  A second open
+ * prints "gpunb: it is already open" and returns, like :637-640. */
+void gpunb_open_(int *nbmax, int *irank);
+
+/* Release every device buffer.  Replaces gpunb_close_ (gpunb.velocity.cu:912-914,
+ * GPUNB_close :669-702). */
+void gpunb_close_(void);
+
+/* Upload the j-particles (mass, position, velocity, FP64).  Replaces
+ * gpunb_send_ (gpunb.velocity.cu:915-921, GPUNB_send :704-730).  The data stay
+ * FP64 on the device. */
+void gpunb_send_(int *nj, double mj[], double xj[][3], double vj[][3]);
+
+/* Regular force, jerk and potential over the non-neighbours of ni i-particles,
+ * and their neighbour lists.  Replaces gpunb_regf_ (gpunb.velocity.cu:922-936,
+ * GPUNB_regf :732-881; AVX reg.avx.cpp:367-381).
+ *   h2[i]   : RS^2 (or RS^2/BODYM when *m_flag != 0)
+ *   dtr[i]  : regular step used by the velocity criterion
+ *   j is a neighbour of i iff min(|dx|^2, |dx + dtr*dv|^2) < h2 (or < m_j*h2)
+ *   list[i][0] = count, list[i][1..count] = ascending 0-based j;  a count above
+ *   min(*nnbmax, *lmax-1) is returned as -count with no entries (reg.avx.cpp:320-322).
+ * Any ni >= 1 is accepted (the reference is limited to NIMAX = 2048). */
+void gpunb_regf_(int *ni, double h2[], double dtr[], double xi[][3], double vi[][3],
+                 double acc[][3], double jrk[][3], double pot[], int *lmax,
+                 int *nnbmax, int *list, int *m_flag);
+
+/* Print the profile line and reset the counters.  Replaces gpunb_profile_
+ * (gpunb.velocity.cu:937-939, GPUNB_profile :883-903). */
+void gpunb_profile_(int *irank);
+
+/* All-pairs potential pot[k] = sum_{j, r>0} m_j / r(istart-1+k, j) over the n
+ * particles, k < *ni, *istart 1-based.  Replaces gpupot_ (gpupot.gpu.cu:116-127,
+ * gpupot :61-114; AVX pot.avx.cpp:148-159). */
+void gpupot_(int *irank, int *istart, int *ni, int *n, double m[], double x[][3],
+             double pot[]);
+
+/* Irregular (neighbour-list) force.  Replace irr_simd_*_ (src/Main/irr.avx.cpp:568-606). */
+void irr_simd_open_(int *nmax, int *lmax, int *rank);          /* irr.avx.cpp:569-571 */
+void irr_simd_close_(int *rank);                               /* irr.avx.cpp:572-574 */
+void irr_simd_profile_(int *rank);                             /* irr.avx.cpp:575-577 */
+/* Store the predictor record of particle *addr (1-based): irr.avx.cpp:579-589. */
+void irr_simd_set_jp_(int *addr, double pos[3], double vel[3], double acc2[3],
+                      double jrk6[3], double *mass, double *time);
+/* Store the neighbour list of *addr: nblist[0] = nnb, nblist[1..] 1-based
+ * addresses (irr.avx.cpp:590-595). */
+void irr_simd_set_list_(int *addr, int *nblist);
+/* Neighbour force and jerk of the *ni particles addr[] (1-based) at time *ti,
+ * every member predicted to *ti; nnbid[k] = 1-based nearest neighbour
+ * (irr.avx.cpp:596-605). */
+void irr_simd_firr_vec_(double *ti, int *ni, int addr[], double acc[][3],
+                        double jrk[][3], int nnbid[]);
+
+/* gpuirr_* names used by the commented-out GPUIRR calls (fpoly0.F:42,70,202,
+ * intgrt.F:1329).  The reference has no implementation; these are aliases of the
+ * irr_simd_* entry points with the argument lists of those calls. */
+void gpuirr_open_(int *nmax, int *lmax);
+void gpuirr_close_(void);
+void gpuirr_set_jp_(int *addr, double pos[3], double vel[3], double acc2[3],
+                    double jrk6[3], double *mass, double *time);
+void gpuirr_set_list_(int *addr, int *nblist);
+void gpuirr_firr_vec_(double *ti, int *ni, int addr[], double acc[][3],
+                      double jrk[][3], int nnbid[]);
+
+/* ===================================================================== *
+ * Part 2 -- extension API (no counterpart in the reference)              *
+ * ===================================================================== */
+
+/* Last error code (0 = none) and message; cleared by nb6_clear_error(). */
+int         nb6_last_error(void);
+const char *nb6_last_error_message(void);
+void        nb6_clear_error(void);
+
+/* 1 (default): argument errors abort the process with a message, like the
+ * reference's assert()s.  0: they are reported through nb6_last_error() and the
+ * call returns without touching its outputs. */
+void nb6_set_abort_on_error(int on);
+
+/* Number of HIP devices visible (0 when none); never initialises a context
+ * beyond hipGetDeviceCount. */
+int nb6_device_count(void);
+
+/* Device-resident regular force (no host copies).  All pointers are device
+ * pointers on the library's device:
+ *   d_i   : SoA double[8][ni_stride]  (x, y, z, vx, vy, vz, h2, dtr)
+ *   d_out : SoA double[7][ni_stride]  (ax, ay, az, jx, jy, jz, pot)
+ *   d_list: int[ni][lmax] in the gpunb_regf_ row format
+ * The j-set is the one uploaded by gpunb_send_ / nb6_send_device.
+ * stream: a hipStream_t (NULL = the library's stream).  Asynchronous.
+ * Returns 0 or an error code. */
+int nb6_regf_device(int ni, int ni_stride, const double *d_i, int lmax, int nnbmax,
+                    int m_flag, double *d_out, int *d_list, void *stream);
+
+/* Device-resident j upload: d_j is SoA double[7][nj_stride] (x,y,z,vx,vy,vz,m). */
+int nb6_send_device(int nj, int nj_stride, const double *d_j, void *stream);
+
+/* Device-resident potential: d_x SoA double[4][n_stride] (x,y,z,m) for all n
+ * particles; d_pot[k] for particles [i0, i0+ni) (0-based). */
+int nb6_pot_device(int n, int n_stride, const double *d_x, int i0, int ni,
+                   double *d_pot, void *stream);
+
+/* Tile configuration of the regular-force kernel: i-particles per lane (1, 2 or
+ * 4) and the minimum number of work-groups a call is split into. */
+int nb6_set_regf_config(int i_per_lane, int min_groups);
+
+/* Average device time (ms) of the last launch of the named kernel family
+ * ("regf_tile", "regf_finalize", "pot", "irr") measured with HIP events, and
+ * the number of j-chunks the last regf call used. */
+double nb6_kernel_ms(const char *name);
+int    nb6_last_regf_chunks(void);
+
+/* Device allocation / copies for callers without another GPU runtime
+ * (tests, benchmarks). */
+void *nb6_dev_alloc(unsigned long long bytes);
+void  nb6_dev_free(void *p);
+int   nb6_memcpy_htod(void *dst, const void *src, unsigned long long bytes);
+int   nb6_memcpy_dtoh(void *dst, const void *src, unsigned long long bytes);
+int   nb6_sync(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NB6GPU_H */

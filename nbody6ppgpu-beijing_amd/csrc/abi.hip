@@ -1,0 +1,511 @@
+// Host side of libnb6gpu.so: device context, buffers, error state, and the
+// Fortran-callable C ABI (gpunb_*_, gpupot_, irr_simd_*_, gpuirr_*_) plus the
+// nb6_* extension entry points.  See include/nb6gpu.h for the contract of each
+// symbol and the reference lines it replaces.
+#include "nb6_internal.h"
+#include "../../include/nb6gpu.h"
+
+#include <sys/time.h>
+#include <unistd.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace nb6 {
+
+// irr.hip
+int irr_open(int nmax, int lmax, int rank);
+int irr_close(int rank);
+int irr_set_jp(int addr1, const double *pos, const double *vel, const double *acc2,
+               const double *jrk6, double mass, double time);
+int irr_set_list(int addr1, const int *nblist);
+int irr_firr_vec(double ti, int ni, const int *addr, double (*acc)[3], double (*jrk)[3],
+                 int *nnbid);
+int irr_profile(int rank);
+
+// ------------------------------------------------------------------ errors
+static int g_err = 0;
+static std::string g_msg;
+static int g_abort = -1;   // -1: not yet read from NB6_ABORT_ON_ERROR
+
+static bool abort_enabled() {
+    if (g_abort < 0) {
+        const char *e = getenv("NB6_ABORT_ON_ERROR");
+        g_abort = (e && e[0] == '0') ? 0 : 1;
+    }
+    return g_abort == 1;
+}
+
+void set_error(int code, const std::string &msg) {
+    g_err = code;
+    g_msg = msg;
+    fprintf(stderr, "nb6gpu error %d: %s\n", code, msg.c_str());
+    if (abort_enabled()) abort();
+}
+
+void hip_fail(hipError_t e, const char *what, const char *file, int line) {
+    fprintf(stderr, "nb6gpu: HIP error '%s' in %s at %s:%d\n", hipGetErrorString(e), what, file,
+            line);
+    abort();
+}
+
+static double wtime() {
+    struct timeval tv;
+    gettimeofday(&tv, nullptr);
+    return tv.tv_sec + 1e-6 * tv.tv_usec;
+}
+
+// ------------------------------------------------------------------ device
+Device &device(int irank) {
+    static Device d;
+    if (d.ready) return d;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        set_error(E_NODEV, "no HIP device available");
+        return d;
+    }
+    std::vector<int> ids;
+    if (const char *gl = getenv("GPU_LIST")) {   // same variable as gpunb.velocity.cu:583
+        std::string s(gl);
+        size_t p = 0;
+        while (p < s.size()) {
+            while (p < s.size() && s[p] == ' ') p++;
+            size_t q = p;
+            while (q < s.size() && s[q] != ' ') q++;
+            if (q > p) ids.push_back(atoi(s.substr(p, q - p).c_str()));
+            p = q;
+        }
+    }
+    if (ids.empty())
+        d.id = irank >= 0 ? irank % n : 0;
+    else
+        d.id = ids[(irank >= 0 ? irank : 0) % ids.size()];
+    if (d.id < 0 || d.id >= n) {
+        set_error(E_NODEV, "GPU_LIST names a device that does not exist");
+        return d;
+    }
+    NB6_HIP(hipSetDevice(d.id));
+    NB6_HIP(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    NB6_HIP(hipGetDeviceProperties(&prop, d.id));
+    d.num_cu = prop.multiProcessorCount;
+    char host[150] = {0};
+    gethostname(host, sizeof(host) - 1);
+    fprintf(stderr, "# GPU initialization - rank: %d; HOST %s; NGPU 1; device: %d %s (%s, %d CU)\n",
+            irank, host, d.id, prop.name, prop.gcnArchName, d.num_cu);
+    d.ready = true;
+    return d;
+}
+
+void *DBuf::get(size_t need) {
+    if (need == 0) need = 1;
+    if (need > bytes) {
+        if (p) NB6_HIP(hipFree(p));
+        size_t nb = need + need / 8;
+        NB6_HIP(hipMalloc(&p, nb));
+        bytes = nb;
+    }
+    return p;
+}
+void DBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+void *HBuf::get(size_t need) {
+    if (need == 0) need = 1;
+    if (need > bytes) {
+        if (p) NB6_HIP(hipHostFree(p));
+        size_t nb = need + need / 8;
+        NB6_HIP(hipHostMalloc(&p, nb, hipHostMallocDefault));
+        bytes = nb;
+    }
+    return p;
+}
+void HBuf::release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+void KTimer::start(hipStream_t s) {
+    if (!a) {
+        NB6_HIP(hipEventCreate(&a));
+        NB6_HIP(hipEventCreate(&b));
+    }
+    NB6_HIP(hipEventRecord(a, s));
+}
+void KTimer::stop(hipStream_t s) {
+    NB6_HIP(hipEventRecord(b, s));
+    armed = true;
+}
+double KTimer::ms() {
+    if (!armed) return -1.0;
+    NB6_HIP(hipEventSynchronize(b));
+    float t = 0.f;
+    NB6_HIP(hipEventElapsedTime(&t, a, b));
+    return t;
+}
+KTimer &ktimer(const char *name) {
+    static std::map<std::string, KTimer> m;
+    return m[name];
+}
+
+// ------------------------------------------------------------------ gpunb state
+struct Gpunb {
+    bool open = false;
+    int nbmax = 0;
+    JSet js;
+    HBuf hj, hi, hout, hlist;
+    DBuf di, dout, dlist;
+    double time_send = 0, time_grav = 0, time_reduce = 0, time_out = 0;
+    long long num_inter = 0;
+    int icall = 0, ini = 0, isend = 0;
+} g_nb;
+
+static bool check_finite(const double *a, size_t n) {
+    for (size_t k = 0; k < n; k++)
+        if (!(a[k] == a[k])) return false;
+    return true;
+}
+
+// spread a host loop over a few threads (list assembly and packing)
+template <class F>
+static void par_for(int n, F f) {
+    const int nt = n > 4096 ? std::min(8, (int)std::thread::hardware_concurrency()) : 1;
+    if (nt <= 1) { f(0, n); return; }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++) {
+        const int a = (int)((long long)n * t / nt), b = (int)((long long)n * (t + 1) / nt);
+        th.emplace_back([=] { f(a, b); });
+    }
+    for (auto &x : th) x.join();
+}
+
+}  // namespace nb6
+
+using namespace nb6;
+
+extern "C" {
+
+// ================================================================== errors
+int nb6_last_error(void) { return g_err; }
+const char *nb6_last_error_message(void) { return g_msg.c_str(); }
+void nb6_clear_error(void) { g_err = 0; g_msg.clear(); }
+void nb6_set_abort_on_error(int on) { g_abort = on ? 1 : 0; }
+int nb6_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+// ================================================================== gpunb
+void gpunb_devinit_(int *irank) { (void)device(irank ? *irank : 0); }
+
+void gpunb_open_(int *nbmax, int *irank) {
+    Gpunb &g = g_nb;
+    g.time_send = g.time_grav = g.time_reduce = g.time_out = 0.0;
+    g.num_inter = 0;
+    g.icall = g.ini = g.isend = 0;
+    Device &d = device(*irank);
+    if (!d.ready) return;
+    if (g.open) {
+        fprintf(stderr, "gpunb: it is already open\n");
+        return;
+    }
+    if (*nbmax <= 0) {
+        set_error(E_ARG, "gpunb_open: nbmax must be positive");
+        return;
+    }
+    g.nbmax = *nbmax;
+    jset_reserve(g.js, g.nbmax);
+    g.js.n = 0;
+    g.open = true;
+    fprintf(stderr, "# Open GPU regular force - rank: %d; nbmax: %d\n", *irank, *nbmax);
+}
+
+void gpunb_close_(void) {
+    Gpunb &g = g_nb;
+    if (!g.open) {
+        fprintf(stderr, "gpunb: it is already close\n");
+        return;
+    }
+    g.open = false;
+    jset_release(g.js);
+    g.hj.release(); g.hi.release(); g.hout.release(); g.hlist.release();
+    g.di.release(); g.dout.release(); g.dlist.release();
+    regf_release();
+    g.nbmax = 0;
+}
+
+void gpunb_send_(int *nj, double mj[], double xj[][3], double vj[][3]) {
+    Gpunb &g = g_nb;
+    if (!g.open) { set_error(E_STATE, "gpunb_send: library not open"); return; }
+    const int n = *nj;
+    if (n < 0 || n > g.nbmax) { set_error(E_ARG, "gpunb_send: nj outside [0, nbmax]"); return; }
+    if (!check_finite(mj, n) || !check_finite(&xj[0][0], 3 * (size_t)n) ||
+        !check_finite(&vj[0][0], 3 * (size_t)n)) {
+        set_error(E_ARG, "gpunb_send: NaN in j-particle data");
+        return;
+    }
+    g.time_send -= wtime();
+    g.isend++;
+    Device &d = device();
+    // AoS records (x y z m vx vy vz 0) in the caller's order, plus box and max mass
+    double *h = static_cast<double *>(g.hj.get((size_t)8 * std::max(n, 1) * sizeof(double)));
+    double box[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+    double mmax = 0.0;
+    for (int j = 0; j < n; j++) {
+        double *r = h + (size_t)8 * j;
+        r[0] = xj[j][0]; r[1] = xj[j][1]; r[2] = xj[j][2]; r[3] = mj[j];
+        r[4] = vj[j][0]; r[5] = vj[j][1]; r[6] = vj[j][2]; r[7] = 0.0;
+        for (int k = 0; k < 3; k++) {
+            box[k] = r[k] < box[k] ? r[k] : box[k];
+            box[3 + k] = r[k] > box[3 + k] ? r[k] : box[3 + k];
+        }
+        mmax = mj[j] > mmax ? mj[j] : mmax;
+    }
+    if (n > 0)
+        NB6_HIP(hipMemcpyAsync(g.js.in64.p, h, (size_t)8 * n * sizeof(double),
+                               hipMemcpyHostToDevice, d.stream));
+    jset_build(g.js, n, box, mmax, d.stream);
+    NB6_HIP(hipStreamSynchronize(d.stream));
+    g.time_send += wtime();
+}
+
+void gpunb_regf_(int *ni_, double h2[], double dtr[], double xi[][3], double vi[][3],
+                 double acc[][3], double jrk[][3], double pot[], int *lmax_, int *nnbmax_,
+                 int *list, int *m_flag) {
+    Gpunb &g = g_nb;
+    if (!g.open) { set_error(E_STATE, "gpunb_regf: library not open"); return; }
+    const int ni = *ni_, lmax = *lmax_, nnbmax = *nnbmax_;
+    if (ni <= 0) { set_error(E_ARG, "gpunb_regf: ni must be positive"); return; }
+    if (lmax < 1) { set_error(E_ARG, "gpunb_regf: lmax must be >= 1"); return; }
+    if (!check_finite(h2, ni) || !check_finite(dtr, ni) || !check_finite(&xi[0][0], 3 * (size_t)ni) ||
+        !check_finite(&vi[0][0], 3 * (size_t)ni)) {
+        set_error(E_ARG, "gpunb_regf: NaN in i-particle data");
+        return;
+    }
+    const double t0 = wtime();
+    g.num_inter += (long long)ni * g.js.n;
+    g.ini += ni;
+    g.icall++;
+    Device &d = device();
+    hipStream_t s = d.stream;
+    double *hi = static_cast<double *>(g.hi.get((size_t)8 * ni * sizeof(double)));
+    par_for(ni, [&](int a, int b) {
+        for (int i = a; i < b; i++) {
+            hi[0 * (size_t)ni + i] = xi[i][0];
+            hi[1 * (size_t)ni + i] = xi[i][1];
+            hi[2 * (size_t)ni + i] = xi[i][2];
+            hi[3 * (size_t)ni + i] = vi[i][0];
+            hi[4 * (size_t)ni + i] = vi[i][1];
+            hi[5 * (size_t)ni + i] = vi[i][2];
+            hi[6 * (size_t)ni + i] = h2[i];
+            hi[7 * (size_t)ni + i] = dtr[i];
+        }
+    });
+    double *di = static_cast<double *>(g.di.get((size_t)8 * ni * sizeof(double)));
+    double *dout = static_cast<double *>(g.dout.get((size_t)7 * ni * sizeof(double)));
+    int *dlist = static_cast<int *>(g.dlist.get((size_t)ni * lmax * sizeof(int)));
+    NB6_HIP(hipMemcpyAsync(di, hi, (size_t)8 * ni * sizeof(double), hipMemcpyHostToDevice, s));
+    if (regf_launch(g.js, ni, ni, di, lmax, nnbmax, *m_flag, dout, dlist, s) != OK) return;
+    double *ho = static_cast<double *>(g.hout.get((size_t)7 * ni * sizeof(double)));
+    int *hl = static_cast<int *>(g.hlist.get((size_t)ni * lmax * sizeof(int)));
+    NB6_HIP(hipMemcpyAsync(ho, dout, (size_t)7 * ni * sizeof(double), hipMemcpyDeviceToHost, s));
+    NB6_HIP(hipMemcpyAsync(hl, dlist, (size_t)ni * lmax * sizeof(int), hipMemcpyDeviceToHost, s));
+    NB6_HIP(hipStreamSynchronize(s));
+    const double t1 = wtime();
+    const int nnbeff = std::max(0, std::min(nnbmax, lmax - 1));
+    par_for(ni, [&](int a, int b) {
+        for (int i = a; i < b; i++) {
+            acc[i][0] = ho[0 * (size_t)ni + i];
+            acc[i][1] = ho[1 * (size_t)ni + i];
+            acc[i][2] = ho[2 * (size_t)ni + i];
+            jrk[i][0] = ho[3 * (size_t)ni + i];
+            jrk[i][1] = ho[4 * (size_t)ni + i];
+            jrk[i][2] = ho[5 * (size_t)ni + i];
+            pot[i] = ho[6 * (size_t)ni + i];
+            const int *src = hl + (size_t)i * lmax;
+            int *dst = list + (size_t)i * lmax;
+            const int c = src[0];
+            dst[0] = c;
+            if (c > 0 && c <= nnbeff) std::memcpy(dst + 1, src + 1, (size_t)c * sizeof(int));
+        }
+    });
+    const double t2 = wtime();
+    g.time_grav += t1 - t0;
+    g.time_reduce += t2 - t1;
+}
+
+void gpunb_profile_(int *irank) {
+    Gpunb &g = g_nb;
+    if (g.icall) {
+        fprintf(stderr,
+                "[R.%d GPU Reg.F ] Nsend %d  Ngrav %d  <Ni> %d   send(s) %f grav(s) %f  nb(s) %f  "
+                "out(s) %f  Perf.(Gflops) %f\n",
+                *irank, g.isend, g.icall, g.isend ? g.ini / g.isend : 0, g.time_send, g.time_grav,
+                g.time_reduce, g.time_out,
+                g.time_grav > 0 ? 60.e-9 * (double)g.num_inter / g.time_grav : 0.0);
+    }
+    g.time_send = g.time_grav = g.time_reduce = g.time_out = 0.0;
+    g.num_inter = 0;
+    g.icall = g.ini = g.isend = 0;
+}
+
+// ================================================================== gpupot
+static HBuf g_pot_h, g_pot_ho;
+static DBuf g_pot_d, g_pot_do;
+
+void gpupot_(int *irank, int *istart, int *ni_, int *n_, double m[], double x[][3], double pot[]) {
+    Device &d = device(*irank);
+    if (!d.ready) return;
+    const int n = *n_, ni = *ni_, i0 = *istart - 1;
+    if (n <= 0 || ni < 0 || i0 < 0 || i0 + ni > n) {
+        set_error(E_ARG, "gpupot: istart/ni outside the particle range");
+        return;
+    }
+    if (!check_finite(m, n) || !check_finite(&x[0][0], 3 * (size_t)n)) {
+        set_error(E_ARG, "gpupot: NaN in particle data");
+        return;
+    }
+    const double t0 = wtime();
+    double *h = static_cast<double *>(g_pot_h.get((size_t)4 * n * sizeof(double)));
+    par_for(n, [&](int a, int b) {
+        for (int j = a; j < b; j++) {
+            h[0 * (size_t)n + j] = x[j][0];
+            h[1 * (size_t)n + j] = x[j][1];
+            h[2 * (size_t)n + j] = x[j][2];
+            h[3 * (size_t)n + j] = m[j];
+        }
+    });
+    double *dx = static_cast<double *>(g_pot_d.get((size_t)4 * n * sizeof(double)));
+    double *dp = static_cast<double *>(g_pot_do.get((size_t)std::max(ni, 1) * sizeof(double)));
+    double *hp = static_cast<double *>(g_pot_ho.get((size_t)std::max(ni, 1) * sizeof(double)));
+    NB6_HIP(hipMemcpyAsync(dx, h, (size_t)4 * n * sizeof(double), hipMemcpyHostToDevice, d.stream));
+    if (pot_launch(n, n, dx, i0, ni, dp, d.stream) != OK) return;
+    NB6_HIP(hipMemcpyAsync(hp, dp, (size_t)ni * sizeof(double), hipMemcpyDeviceToHost, d.stream));
+    NB6_HIP(hipStreamSynchronize(d.stream));
+    std::memcpy(pot, hp, (size_t)ni * sizeof(double));
+    const double t1 = wtime();
+    fprintf(stderr, "[R.%d GPU Pot.A] Ni %d  NTOT %d  pot(s) %f\n", *irank, ni, n, t1 - t0);
+}
+
+// ================================================================== irr_simd
+void irr_simd_open_(int *nmax, int *lmax, int *rank) { irr_open(*nmax, *lmax, *rank); }
+void irr_simd_close_(int *rank) { irr_close(*rank); }
+void irr_simd_profile_(int *rank) { irr_profile(*rank); }
+void irr_simd_set_jp_(int *addr, double pos[3], double vel[3], double acc2[3], double jrk6[3],
+                      double *mass, double *time) {
+    irr_set_jp(*addr, pos, vel, acc2, jrk6, *mass, *time);
+}
+void irr_simd_set_list_(int *addr, int *nblist) { irr_set_list(*addr, nblist); }
+void irr_simd_firr_vec_(double *ti, int *ni, int addr[], double acc[][3], double jrk[][3],
+                        int nnbid[]) {
+    irr_firr_vec(*ti, *ni, addr, acc, jrk, nnbid);
+}
+
+// gpuirr_* aliases (argument lists of the commented GPUIRR calls, fpoly0.F:42,70,202)
+void gpuirr_open_(int *nmax, int *lmax) { irr_open(*nmax, *lmax, 0); }
+void gpuirr_close_(void) { irr_close(0); }
+void gpuirr_set_jp_(int *addr, double pos[3], double vel[3], double acc2[3], double jrk6[3],
+                    double *mass, double *time) {
+    irr_set_jp(*addr, pos, vel, acc2, jrk6, *mass, *time);
+}
+void gpuirr_set_list_(int *addr, int *nblist) { irr_set_list(*addr, nblist); }
+void gpuirr_firr_vec_(double *ti, int *ni, int addr[], double acc[][3], double jrk[][3],
+                      int nnbid[]) {
+    irr_firr_vec(*ti, *ni, addr, acc, jrk, nnbid);
+}
+
+// ================================================================== extensions
+int nb6_regf_device(int ni, int ni_stride, const double *d_i, int lmax, int nnbmax, int m_flag,
+                    double *d_out, int *d_list, void *stream) {
+    Gpunb &g = g_nb;
+    if (!g.open) { set_error(E_STATE, "nb6_regf_device: library not open"); return E_STATE; }
+    if (ni <= 0 || ni_stride < ni) { set_error(E_ARG, "nb6_regf_device: bad ni / stride"); return E_ARG; }
+    Device &d = device();
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : d.stream;
+    g.num_inter += (long long)ni * g.js.n;
+    g.ini += ni;
+    g.icall++;
+    return regf_launch(g.js, ni, ni_stride, d_i, lmax, nnbmax, m_flag, d_out, d_list, s);
+}
+
+int nb6_send_device(int nj, int nj_stride, const double *d_j, void *stream) {
+    Gpunb &g = g_nb;
+    if (!g.open) { set_error(E_STATE, "nb6_send_device: library not open"); return E_STATE; }
+    if (nj < 0 || nj > g.nbmax || nj_stride < nj) {
+        set_error(E_ARG, "nb6_send_device: nj outside [0, nbmax] or bad stride");
+        return E_ARG;
+    }
+    Device &d = device();
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : d.stream;
+    jset_from_soa(g.js, nj, nj_stride, d_j, s);
+    g.isend++;
+    return OK;
+}
+
+int nb6_pot_device(int n, int n_stride, const double *d_x, int i0, int ni, double *d_pot,
+                   void *stream) {
+    Device &d = device();
+    if (!d.ready) return E_NODEV;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : d.stream;
+    return pot_launch(n, n_stride, d_x, i0, ni, d_pot, s);
+}
+
+int nb6_set_regf_config(int i_per_lane, int min_groups) {
+    if (i_per_lane != 1 && i_per_lane != 2 && i_per_lane != 4) {
+        set_error(E_ARG, "nb6_set_regf_config: i_per_lane must be 1, 2 or 4");
+        return E_ARG;
+    }
+    if (min_groups < 1) {
+        set_error(E_ARG, "nb6_set_regf_config: min_groups must be positive");
+        return E_ARG;
+    }
+    regf_config().ipl = i_per_lane;
+    regf_config().min_groups = min_groups;
+    return OK;
+}
+
+double nb6_kernel_ms(const char *name) { return ktimer(name).ms(); }
+int nb6_last_regf_chunks(void) { return regf_last_chunks(); }
+
+void *nb6_dev_alloc(unsigned long long bytes) {
+    if (!device().ready) return nullptr;
+    void *p = nullptr;
+    NB6_HIP(hipMalloc(&p, bytes ? bytes : 1));
+    return p;
+}
+void nb6_dev_free(void *p) {
+    if (p) NB6_HIP(hipFree(p));
+}
+int nb6_memcpy_htod(void *dst, const void *src, unsigned long long bytes) {
+    Device &d = device();
+    if (!d.ready) return E_NODEV;
+    NB6_HIP(hipStreamSynchronize(d.stream));
+    NB6_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return OK;
+}
+int nb6_memcpy_dtoh(void *dst, const void *src, unsigned long long bytes) {
+    Device &d = device();
+    if (!d.ready) return E_NODEV;
+    NB6_HIP(hipStreamSynchronize(d.stream));
+    NB6_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return OK;
+}
+int nb6_sync(void) {
+    Device &d = device();
+    if (!d.ready) return E_NODEV;
+    NB6_HIP(hipStreamSynchronize(d.stream));
+    NB6_HIP(hipDeviceSynchronize());
+    return OK;
+}
+
+}  // extern "C"

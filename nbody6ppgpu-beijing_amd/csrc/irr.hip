@@ -1,0 +1,403 @@
+// Irregular (neighbour-list) force for gfx950, behind irr_simd_*_ / gpuirr_*_.
+//
+// Semantics follow irr.avx.cpp:
+//   predictor  s = ti - t0;  x += s(v + s(a/2 + s jdot/6));  v += 2s(a/2 + 1.5 s jdot/6)
+//                                                   irr.avx.cpp:142-156, xbpredall.F:55-59
+//   pair terms acc += m r^-3 dx;  jrk += m r^-3 (dv - 3 (dx.dv) r^-2 dx)
+//                                                   irr.avx.cpp:303-332, nbint.F:36-57
+//   nnbid      1-based index of the nearest list member (ties: lower index)
+//                                                   irr.avx.cpp:271-283,293
+//
+// Method (own): device-resident particle records, 64 B each
+//   { double x, y, z;  float m, vx, vy, vz, ax2, ay2, az2, jx6, jy6, jz6 }
+// plus a separate FP64 time array, and device-resident lists [nmax][lmax]
+// (row[0] = nnb, 0-based entries).  set_jp / set_list only append to host logs;
+// the logs are uploaded and scattered by one kernel each at the next firr call.
+// firr: one wavefront per active particle, lanes over its list members; the
+// separation is formed in FP64 and the predictor increments in FP32; FP32 lane
+// sums are reduced across the wavefront in FP64; nearest neighbour by a 64-bit
+// (r^2 bits, index) minimum.
+#include "nb6_internal.h"
+
+#include <cstring>
+#include <vector>
+
+namespace nb6 {
+
+namespace {
+
+struct alignas(16) IrrRec {
+    double x, y, z;
+    float m, vx, vy, vz;
+    float ax, ay, az, jx;
+    float jy, jz;
+};
+static_assert(sizeof(IrrRec) == 64, "record must be 64 bytes");
+
+struct IrrState {
+    bool open = false;
+    int nmax = 0, lmax = 0;
+    DBuf rec, tim, lists;          // device
+    // host logs
+    std::vector<int> jp_pos;       // addr -> slot in jp log (or -1)
+    std::vector<int> ls_pos;       // addr -> slot in list log (or -1)
+    HBuf jp_log;                   // records: IrrRec + double t0 (72 B) + int addr
+    int jp_n = 0;
+    HBuf ls_log;                   // [addr, nnb, entries...] variable length
+    std::vector<size_t> ls_off;    // offsets (ints) of live entries in ls_log
+    size_t ls_used = 0;            // ints used
+    DBuf d_jp, d_ls, d_lsoff;
+    DBuf d_addr, d_out;
+    HBuf h_addr, h_out;
+    // profile
+    double time_grav = 0.0;
+    unsigned long long num_inter = 0, num_fcall = 0, num_steps = 0;
+} g_irr;
+
+struct JpEntry {
+    IrrRec r;
+    double t0;
+    int addr;
+    int pad;
+};
+static_assert(sizeof(JpEntry) == 80, "jp entry layout");
+
+__global__ void irr_scatter_jp(const JpEntry *__restrict__ e, int n, IrrRec *__restrict__ rec,
+                               double *__restrict__ tim) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int a = e[k].addr;
+    rec[a] = e[k].r;
+    tim[a] = e[k].t0;
+}
+
+// one wavefront per list entry: entry = [addr, nnb, e_1 .. e_nnb] (e 1-based)
+__global__ void irr_scatter_list(const int *__restrict__ log, const long long *__restrict__ off,
+                                 int n, int *__restrict__ lists, int lmax) {
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (w >= n) return;
+    const int *e = log + off[w];
+    const int a = e[0], nnb = e[1];
+    int *row = lists + (size_t)a * lmax;
+    if (lane == 0) row[0] = nnb;
+    for (int k = lane; k < nnb; k += 64) row[1 + k] = e[2 + k] - 1;
+}
+
+struct Pred {
+    double x, y, z;
+    float dx, dy, dz;     // FP32 predictor increments
+    float vx, vy, vz;
+    float m;
+};
+
+__device__ __forceinline__ Pred predict(const IrrRec *__restrict__ rec,
+                                        const double *__restrict__ tim, int a, double ti) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(rec + a);
+    const uint4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+    IrrRec r;
+    uint4 *rr = reinterpret_cast<uint4 *>(&r);
+    rr[0] = q0; rr[1] = q1; rr[2] = q2; rr[3] = q3;
+    const float s = (float)(ti - tim[a]);
+    const float s1 = s + s, s2 = 1.5f * s;
+    Pred o;
+    o.x = r.x; o.y = r.y; o.z = r.z;
+    o.dx = s * (r.vx + s * (r.ax + s * r.jx));
+    o.dy = s * (r.vy + s * (r.ay + s * r.jy));
+    o.dz = s * (r.vz + s * (r.az + s * r.jz));
+    o.vx = r.vx + s1 * (r.ax + s2 * r.jx);
+    o.vy = r.vy + s1 * (r.ay + s2 * r.jy);
+    o.vz = r.vz + s1 * (r.az + s2 * r.jz);
+    o.m = r.m;
+    return o;
+}
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void irr_force_kernel(
+    double ti, int ni, const int *__restrict__ addr, const IrrRec *__restrict__ rec,
+    const double *__restrict__ tim, const int *__restrict__ lists, int lmax,
+    double *__restrict__ out /* [ni][8]: acc3, jrk3, nnbid, nnb */) {
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (w >= ni) return;
+    const int ia = addr[w] - 1;
+    const Pred pi = predict(rec, tim, ia, ti);
+    const int *row = lists + (size_t)ia * lmax;
+    const int nnb = row[0];
+    float ax = 0.f, ay = 0.f, az = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
+    unsigned long long best = ~0ull;
+    for (int k = lane; k < nnb; k += 64) {
+        const int j = row[1 + k];
+        const Pred pj = predict(rec, tim, j, ti);
+        const float dx = (float)(pj.x - pi.x) + (pj.dx - pi.dx);
+        const float dy = (float)(pj.y - pi.y) + (pj.dy - pi.dy);
+        const float dz = (float)(pj.z - pi.z) + (pj.dz - pi.dz);
+        const float du = pj.vx - pi.vx, dv = pj.vy - pi.vy, dw = pj.vz - pi.vz;
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        const float rv = fmaf(dz, dw, fmaf(dy, dv, dx * du));
+        const float rinv = __builtin_amdgcn_rsqf(r2);
+        const float rinv2 = rinv * rinv;
+        const float mr3 = pj.m * rinv * rinv2;
+        const float al = -3.0f * rv * rinv2;
+        ax = fmaf(mr3, dx, ax);
+        ay = fmaf(mr3, dy, ay);
+        az = fmaf(mr3, dz, az);
+        jx = fmaf(mr3, fmaf(al, dx, du), jx);
+        jy = fmaf(mr3, fmaf(al, dy, dv), jy);
+        jz = fmaf(mr3, fmaf(al, dz, dw), jz);
+        const unsigned long long key =
+            ((unsigned long long)__float_as_uint(r2) << 32) | (unsigned)j;
+        best = key < best ? key : best;
+    }
+    const double sax = wsum((double)ax), say = wsum((double)ay), saz = wsum((double)az);
+    const double sjx = wsum((double)jx), sjy = wsum((double)jy), sjz = wsum((double)jz);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long other = __shfl_xor(best, o, 64);
+        best = other < best ? other : best;
+    }
+    if (lane == 0) {
+        double *o = out + (size_t)w * 8;
+        o[0] = sax; o[1] = say; o[2] = saz;
+        o[3] = sjx; o[4] = sjy; o[5] = sjz;
+        o[6] = nnb > 0 ? (double)((unsigned)(best & 0xffffffffu) + 1u) : 0.0;
+        o[7] = (double)nnb;
+    }
+}
+
+void flush_logs(hipStream_t s) {
+    IrrState &g = g_irr;
+    if (g.jp_n > 0) {
+        const size_t bytes = (size_t)g.jp_n * sizeof(JpEntry);
+        void *d = g.d_jp.get(bytes);
+        NB6_HIP(hipMemcpyAsync(d, g.jp_log.p, bytes, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(irr_scatter_jp, dim3((g.jp_n + 255) / 256), dim3(256), 0, s,
+                           static_cast<const JpEntry *>(d), g.jp_n, g.rec.as<IrrRec>(),
+                           g.tim.as<double>());
+        NB6_HIP(hipGetLastError());
+        const JpEntry *e = g.jp_log.as<JpEntry>();
+        for (int k = 0; k < g.jp_n; k++) g.jp_pos[e[k].addr] = -1;
+        g.jp_n = 0;
+    }
+    if (!g.ls_off.empty()) {
+        const size_t bytes = g.ls_used * sizeof(int);
+        void *d = g.d_ls.get(bytes);
+        NB6_HIP(hipMemcpyAsync(d, g.ls_log.p, bytes, hipMemcpyHostToDevice, s));
+        // live entries only (a repeated set_list of one particle keeps the last)
+        std::vector<long long> live;
+        live.reserve(g.ls_off.size());
+        const int *lg = g.ls_log.as<int>();
+        for (size_t k = 0; k < g.ls_off.size(); k++) {
+            const int a = lg[g.ls_off[k]];
+            if (a >= 0 && g.ls_pos[a] == (int)k) live.push_back((long long)g.ls_off[k]);
+        }
+        const int n = (int)live.size();
+        HBuf &hb = g.h_addr;   // reuse as staging for offsets
+        long long *ho = static_cast<long long *>(hb.get(live.size() * sizeof(long long)));
+        std::memcpy(ho, live.data(), live.size() * sizeof(long long));
+        void *doff = g.d_lsoff.get(live.size() * sizeof(long long));
+        NB6_HIP(hipMemcpyAsync(doff, ho, live.size() * sizeof(long long), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(irr_scatter_list, dim3((n + 3) / 4), dim3(256), 0, s,
+                           static_cast<const int *>(d), static_cast<const long long *>(doff), n,
+                           g.lists.as<int>(), g.lmax);
+        NB6_HIP(hipGetLastError());
+        NB6_HIP(hipStreamSynchronize(s));   // staging buffers are reused next
+        for (size_t k = 0; k < g.ls_off.size(); k++) {
+            const int a = lg[g.ls_off[k]];
+            if (a >= 0) g.ls_pos[a] = -1;
+        }
+        g.ls_off.clear();
+        g.ls_used = 0;
+    }
+    NB6_HIP(hipStreamSynchronize(s));   // the host logs are rewritten after this
+}
+
+}  // namespace
+
+void irr_release() {
+    IrrState &g = g_irr;
+    g.rec.release(); g.tim.release(); g.lists.release();
+    g.d_jp.release(); g.d_ls.release(); g.d_lsoff.release();
+    g.d_addr.release(); g.d_out.release();
+    g.jp_log.release(); g.ls_log.release(); g.h_addr.release(); g.h_out.release();
+    g.jp_pos.clear(); g.ls_pos.clear(); g.ls_off.clear();
+    g.jp_n = 0; g.ls_used = 0;
+    g.open = false;
+}
+
+// ------------------------------------------------------------------ host API
+int irr_open(int nmax, int lmax, int rank) {
+    IrrState &g = g_irr;
+    if (g.open) {
+        fprintf(stderr, "irr_simd: it is already open\n");
+        return OK;
+    }
+    if (nmax <= 0 || lmax < 1) {
+        set_error(E_ARG, "irr_simd_open: nmax and lmax must be positive");
+        return E_ARG;
+    }
+    Device &dev = device(rank);
+    if (!dev.ready) return E_NODEV;
+    g.nmax = nmax; g.lmax = lmax;
+    g.rec.get((size_t)(nmax + 1) * sizeof(IrrRec));
+    g.tim.get((size_t)(nmax + 1) * sizeof(double));
+    g.lists.get((size_t)(nmax + 1) * lmax * sizeof(int));
+    NB6_HIP(hipMemsetAsync(g.rec.p, 0, (size_t)(nmax + 1) * sizeof(IrrRec), dev.stream));
+    NB6_HIP(hipMemsetAsync(g.tim.p, 0, (size_t)(nmax + 1) * sizeof(double), dev.stream));
+    NB6_HIP(hipMemsetAsync(g.lists.p, 0, (size_t)(nmax + 1) * lmax * sizeof(int), dev.stream));
+    NB6_HIP(hipStreamSynchronize(dev.stream));
+    g.jp_pos.assign(nmax + 1, -1);
+    g.ls_pos.assign(nmax + 1, -1);
+    g.jp_n = 0; g.ls_used = 0; g.ls_off.clear();
+    g.time_grav = 0.0; g.num_inter = g.num_fcall = g.num_steps = 0;
+    g.open = true;
+    fprintf(stderr, "# Opening IRR lib. gfx950 ver. - rank: %d; nmax: %d, lmax: %d\n", rank, nmax,
+            lmax);
+    return OK;
+}
+
+int irr_close(int rank) {
+    IrrState &g = g_irr;
+    if (!g.open) {
+        fprintf(stderr, "irr_simd: it is already close\n");
+        return OK;
+    }
+    irr_release();
+    fprintf(stderr, "Closing IRR lib. gfx950 ver. - rank: %d\n", rank);
+    return OK;
+}
+
+int irr_set_jp(int addr1, const double *pos, const double *vel, const double *acc2,
+               const double *jrk6, double mass, double time) {
+    IrrState &g = g_irr;
+    if (!g.open) { set_error(E_STATE, "irr_simd_set_jp: library not open"); return E_STATE; }
+    const int a = addr1 - 1;
+    if (a < 0 || a >= g.nmax) { set_error(E_ARG, "irr_simd_set_jp: address out of range"); return E_ARG; }
+    int slot = g.jp_pos[a];
+    if (slot < 0) {
+        slot = g.jp_n++;
+        JpEntry *base = static_cast<JpEntry *>(g.jp_log.p);
+        if ((size_t)g.jp_n * sizeof(JpEntry) > g.jp_log.bytes) {
+            // grow, keeping the entries already logged
+            const size_t nb = std::max<size_t>((size_t)g.jp_n * sizeof(JpEntry) * 2, 4096);
+            HBuf nbuf;
+            nbuf.get(nb);
+            if (base) std::memcpy(nbuf.p, base, (size_t)(g.jp_n - 1) * sizeof(JpEntry));
+            g.jp_log.release();
+            g.jp_log = nbuf;
+            nbuf.p = nullptr;
+        }
+        g.jp_pos[a] = slot;
+    }
+    JpEntry &e = g.jp_log.as<JpEntry>()[slot];
+    e.r.x = pos[0]; e.r.y = pos[1]; e.r.z = pos[2];
+    e.r.m = (float)mass;
+    e.r.vx = (float)vel[0]; e.r.vy = (float)vel[1]; e.r.vz = (float)vel[2];
+    e.r.ax = (float)acc2[0]; e.r.ay = (float)acc2[1]; e.r.az = (float)acc2[2];
+    e.r.jx = (float)jrk6[0]; e.r.jy = (float)jrk6[1]; e.r.jz = (float)jrk6[2];
+    e.t0 = time;
+    e.addr = a;
+    e.pad = 0;
+    return OK;
+}
+
+int irr_set_list(int addr1, const int *nblist) {
+    IrrState &g = g_irr;
+    if (!g.open) { set_error(E_STATE, "irr_simd_set_list: library not open"); return E_STATE; }
+    const int a = addr1 - 1;
+    const int nnb = nblist[0];
+    if (a < 0 || a >= g.nmax) { set_error(E_ARG, "irr_simd_set_list: address out of range"); return E_ARG; }
+    if (nnb < 0 || nnb > g.lmax - 1) {
+        set_error(E_ARG, "irr_simd_set_list: neighbour count outside [0, lmax-1]");
+        return E_ARG;
+    }
+    for (int k = 1; k <= nnb; k++) {
+        if (nblist[k] < 1 || nblist[k] > g.nmax) {
+            set_error(E_ARG, "irr_simd_set_list: neighbour address out of range");
+            return E_ARG;
+        }
+    }
+    const size_t need = g.ls_used + 2 + (size_t)nnb;
+    if (need * sizeof(int) > g.ls_log.bytes) {
+        HBuf nbuf;
+        nbuf.get(std::max<size_t>(need * sizeof(int) * 2, 1 << 16));
+        if (g.ls_log.p) std::memcpy(nbuf.p, g.ls_log.p, g.ls_used * sizeof(int));
+        g.ls_log.release();
+        g.ls_log = nbuf;
+        nbuf.p = nullptr;
+    }
+    int *e = g.ls_log.as<int>() + g.ls_used;
+    e[0] = a;
+    e[1] = nnb;
+    std::memcpy(e + 2, nblist + 1, (size_t)nnb * sizeof(int));
+    g.ls_pos[a] = (int)g.ls_off.size();
+    g.ls_off.push_back(g.ls_used);
+    g.ls_used = need;
+    return OK;
+}
+
+int irr_firr_vec(double ti, int ni, const int *addr, double (*acc)[3], double (*jrk)[3],
+                 int *nnbid) {
+    IrrState &g = g_irr;
+    if (!g.open) { set_error(E_STATE, "irr_simd_firr_vec: library not open"); return E_STATE; }
+    if (ni <= 0) return OK;
+    for (int k = 0; k < ni; k++) {
+        if (addr[k] < 1 || addr[k] > g.nmax) {
+            set_error(E_ARG, "irr_simd_firr_vec: address out of range");
+            return E_ARG;
+        }
+    }
+    Device &dev = device();
+    hipStream_t s = dev.stream;
+    flush_logs(s);
+    int *ha = static_cast<int *>(g.h_addr.get((size_t)ni * sizeof(int)));
+    std::memcpy(ha, addr, (size_t)ni * sizeof(int));
+    int *da = static_cast<int *>(g.d_addr.get((size_t)ni * sizeof(int)));
+    double *dout = static_cast<double *>(g.d_out.get((size_t)ni * 8 * sizeof(double)));
+    double *hout = static_cast<double *>(g.h_out.get((size_t)ni * 8 * sizeof(double)));
+    NB6_HIP(hipMemcpyAsync(da, ha, (size_t)ni * sizeof(int), hipMemcpyHostToDevice, s));
+    KTimer &t = ktimer("irr");
+    t.start(s);
+    hipLaunchKernelGGL(irr_force_kernel, dim3((ni + 3) / 4), dim3(256), 0, s, ti, ni, da,
+                       g.rec.as<IrrRec>(), g.tim.as<double>(), g.lists.as<int>(), g.lmax, dout);
+    NB6_HIP(hipGetLastError());
+    t.stop(s);
+    NB6_HIP(hipMemcpyAsync(hout, dout, (size_t)ni * 8 * sizeof(double), hipMemcpyDeviceToHost, s));
+    NB6_HIP(hipStreamSynchronize(s));
+    unsigned long long ninter = 0;
+    for (int k = 0; k < ni; k++) {
+        const double *o = hout + (size_t)k * 8;
+        acc[k][0] = o[0]; acc[k][1] = o[1]; acc[k][2] = o[2];
+        jrk[k][0] = o[3]; jrk[k][1] = o[4]; jrk[k][2] = o[5];
+        nnbid[k] = (int)o[6];
+        ninter += (unsigned long long)o[7];
+    }
+    g.num_inter += ninter;
+    g.time_grav += t.ms() * 1e-3;
+    g.num_fcall++;
+    g.num_steps += ni;
+    return OK;
+}
+
+int irr_profile(int rank) {
+    IrrState &g = g_irr;
+    if (g.open && g.num_fcall) {
+        const double gf = 60.0 * double(g.num_inter) * 1e-9 / (g.time_grav > 0 ? g.time_grav : 1e-30);
+        const double us = 1e6 * g.time_grav / double(g.num_fcall);
+        const double nb = g.num_steps ? double(g.num_inter) / double(g.num_steps) : 0.0;
+        const int niavr = (int)(g.num_steps / g.num_fcall);
+        fprintf(stderr,
+                "[R.%d GPU Irr.F ] Ncall: %llu <NI>: %d <NB>: %f grav: %f s, %f Gflops, %f usec\n",
+                rank, g.num_fcall, niavr, nb, g.time_grav, gf, us);
+    }
+    g.time_grav = 0.0;
+    g.num_inter = g.num_fcall = g.num_steps = 0;
+    return OK;
+}
+
+}  // namespace nb6

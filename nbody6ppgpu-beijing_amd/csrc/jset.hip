@@ -1,0 +1,186 @@
+// Device-side layout of the j-particles (what gpunb_send_ uploads).
+//
+// The reference packs j into FP32 absolute coordinates (Jparticle,
+// gpunb.velocity.cu:58-93).  Here the j-set is kept in FP64 and re-ordered
+// along a Morton (Z-order) curve on the device at every send, so that each tile
+// of 256 consecutive j occupies a small region of space.  For every tile t the
+// first particle is the tile origin (P_t, V_t); the regular-force kernel streams
+//   rec32[s] = { x - Px, y - Py, z - Pz, m,  vx - Vx, vy - Vy, vz - Vz, 0 }  (FP32)
+// and forms the i side relative to the same origin in FP64, so the FP32
+// separation carries the rounding error of the tile's extent, not of |x|.
+// rec64 (sorted, AoS, 64 B) feeds the FP64 near-field / neighbour re-test, and
+// perm[s] maps a sorted position back to the caller's 0-based j index.
+#include "nb6_internal.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cfloat>
+
+namespace nb6 {
+
+namespace {
+
+__device__ __forceinline__ unsigned spread10(unsigned a) {
+    a &= 0x3ffu;
+    a = (a | (a << 16)) & 0x030000FFu;
+    a = (a | (a << 8)) & 0x0300F00Fu;
+    a = (a | (a << 4)) & 0x030C30C3u;
+    a = (a | (a << 2)) & 0x09249249u;
+    return a;
+}
+
+__global__ __launch_bounds__(256) void morton_kernel(const double *__restrict__ in, int n,
+                                                     double lx, double ly, double lz, double sx,
+                                                     double sy, double sz,
+                                                     unsigned *__restrict__ key,
+                                                     int *__restrict__ val) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const double *r = in + (size_t)j * 8;
+    const int qx = min(1023, max(0, (int)((r[0] - lx) * sx)));
+    const int qy = min(1023, max(0, (int)((r[1] - ly) * sy)));
+    const int qz = min(1023, max(0, (int)((r[2] - lz) * sz)));
+    key[j] = spread10(qx) | (spread10(qy) << 1) | (spread10(qz) << 2);
+    val[j] = j;
+}
+
+// sorted position s <- original j = perm[s]; tile origin = sorted particle t*TJ
+__global__ __launch_bounds__(256) void build_kernel(const double *__restrict__ in, int n,
+                                                    const int *__restrict__ perm,
+                                                    double *__restrict__ rec64,
+                                                    float4 *__restrict__ rec32,
+                                                    double *__restrict__ torg) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= n) return;
+    const int t = s / JTILE;
+    const double *p = in + (size_t)perm[t * JTILE] * 8;
+    const double *r = in + (size_t)perm[s] * 8;
+    double *o = rec64 + (size_t)s * 8;
+#pragma unroll
+    for (int f = 0; f < 8; f++) o[f] = r[f];
+    rec32[2 * (size_t)s] = make_float4((float)(r[0] - p[0]), (float)(r[1] - p[1]),
+                                       (float)(r[2] - p[2]), (float)r[3]);
+    rec32[2 * (size_t)s + 1] = make_float4((float)(r[4] - p[4]), (float)(r[5] - p[5]),
+                                           (float)(r[6] - p[6]), 0.f);
+    if (s == t * JTILE) {
+        double *q = torg + (size_t)t * 8;
+        q[0] = p[0]; q[1] = p[1]; q[2] = p[2];
+        q[3] = p[4]; q[4] = p[5]; q[5] = p[6];
+        q[6] = 0.0; q[7] = 0.0;
+    }
+}
+
+// SoA [7][stride] (x y z vx vy vz m) -> AoS [n][8] (x y z m vx vy vz 0)
+__global__ __launch_bounds__(256) void soa_to_aos_kernel(const double *__restrict__ soa,
+                                                         int n, int stride,
+                                                         double *__restrict__ aos) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    double *o = aos + (size_t)j * 8;
+    o[0] = soa[0 * (size_t)stride + j];
+    o[1] = soa[1 * (size_t)stride + j];
+    o[2] = soa[2 * (size_t)stride + j];
+    o[3] = soa[6 * (size_t)stride + j];
+    o[4] = soa[3 * (size_t)stride + j];
+    o[5] = soa[4 * (size_t)stride + j];
+    o[6] = soa[5 * (size_t)stride + j];
+    o[7] = 0.0;
+}
+
+// bounding box of positions and maximum mass: one work-group, 256 threads
+__global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ aos, int n,
+                                                    double *__restrict__ out /* 7 */) {
+    __shared__ double sh[7][256];
+    double v[7] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX, 0.0};
+    for (int j = threadIdx.x; j < n; j += 256) {
+        const double *r = aos + (size_t)j * 8;
+        for (int d = 0; d < 3; d++) {
+            v[d] = fmin(v[d], r[d]);
+            v[3 + d] = fmax(v[3 + d], r[d]);
+        }
+        v[6] = fmax(v[6], r[3]);
+    }
+    for (int q = 0; q < 7; q++) sh[q][threadIdx.x] = v[q];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            for (int d = 0; d < 3; d++) sh[d][threadIdx.x] = fmin(sh[d][threadIdx.x], sh[d][threadIdx.x + o]);
+            for (int d = 3; d < 7; d++) sh[d][threadIdx.x] = fmax(sh[d][threadIdx.x], sh[d][threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 7) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+
+}  // namespace
+
+void jset_reserve(JSet &js, int cap) {
+    if (cap <= js.cap) return;
+    js.cap = cap;
+    const int ntile = (cap + JTILE - 1) / JTILE;
+    js.in64.get((size_t)cap * 8 * sizeof(double));
+    js.rec64.get((size_t)cap * 8 * sizeof(double));
+    js.rec32.get((size_t)cap * 2 * sizeof(float4));
+    js.torg.get((size_t)ntile * 8 * sizeof(double));
+    js.perm.get((size_t)cap * sizeof(int));
+    js.keys.get((size_t)cap * 2 * sizeof(unsigned));
+    js.vals.get((size_t)cap * sizeof(int));
+    js.stat.get(8 * sizeof(double));
+}
+
+void jset_release(JSet &js) {
+    js.in64.release(); js.rec64.release(); js.rec32.release(); js.torg.release();
+    js.perm.release(); js.keys.release(); js.vals.release(); js.stat.release();
+    js.sort_tmp.release();
+    js.cap = 0; js.n = 0; js.ntile = 0;
+}
+
+// in64 holds n AoS records in the caller's order; box = {lo[3], hi[3]} or nullptr
+void jset_build(JSet &js, int n, const double *box, double mmax, hipStream_t s) {
+    js.n = n;
+    js.ntile = (n + JTILE - 1) / JTILE;
+    if (n <= 0) return;
+    double st[7];
+    if (!box) {
+        hipLaunchKernelGGL(stats_kernel, dim3(1), dim3(256), 0, s, js.in64.as<double>(), n,
+                           js.stat.as<double>());
+        NB6_HIP(hipGetLastError());
+        NB6_HIP(hipMemcpyAsync(st, js.stat.p, 7 * sizeof(double), hipMemcpyDeviceToHost, s));
+        NB6_HIP(hipStreamSynchronize(s));
+        box = st;
+        mmax = st[6];
+    }
+    js.mmax = mmax;
+    double sc[3];
+    for (int d = 0; d < 3; d++) {
+        const double w = box[3 + d] - box[d];
+        sc[d] = w > 0.0 ? 1024.0 / (w * (1.0 + 1e-12)) : 0.0;
+    }
+    unsigned *k0 = js.keys.as<unsigned>();
+    unsigned *k1 = k0 + js.cap;
+    int *v0 = js.vals.as<int>();
+    int *v1 = js.perm.as<int>();
+    hipLaunchKernelGGL(morton_kernel, dim3((n + 255) / 256), dim3(256), 0, s, js.in64.as<double>(),
+                       n, box[0], box[1], box[2], sc[0], sc[1], sc[2], k0, v0);
+    NB6_HIP(hipGetLastError());
+    size_t tb = 0;
+    NB6_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, n, 0, 30, s));
+    void *tmp = js.sort_tmp.get(tb);
+    NB6_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, v1, n, 0, 30, s));
+    hipLaunchKernelGGL(build_kernel, dim3((n + 255) / 256), dim3(256), 0, s, js.in64.as<double>(), n,
+                       js.perm.as<int>(), js.rec64.as<double>(), js.rec32.as<float4>(),
+                       js.torg.as<double>());
+    NB6_HIP(hipGetLastError());
+}
+
+void jset_from_soa(JSet &js, int n, int stride, const double *d_soa, hipStream_t s) {
+    if (n > 0) {
+        hipLaunchKernelGGL(soa_to_aos_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_soa, n,
+                           stride, js.in64.as<double>());
+        NB6_HIP(hipGetLastError());
+    }
+    jset_build(js, n, nullptr, 0.0, s);
+}
+
+}  // namespace nb6

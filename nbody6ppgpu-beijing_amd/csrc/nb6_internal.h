@@ -1,0 +1,110 @@
+// Internal declarations shared by the translation units of libnb6gpu.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+namespace nb6 {
+
+// ---------------------------------------------------------------- errors
+enum Err : int {
+    OK = 0,
+    E_ARG = 1,        // invalid argument (sizes, NaN input, ...)
+    E_STATE = 2,      // call in the wrong library state (not open, ...)
+    E_HIP = 3,        // HIP runtime failure
+    E_NODEV = 4,      // no usable device
+};
+
+void set_error(int code, const std::string &msg);   // records; aborts if enabled
+void hip_fail(hipError_t e, const char *what, const char *file, int line);  // aborts
+
+#define NB6_HIP(call)                                                        \
+    do {                                                                     \
+        hipError_t e_ = (call);                                              \
+        if (e_ != hipSuccess) ::nb6::hip_fail(e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+// ---------------------------------------------------------------- device
+struct Device {
+    bool ready = false;
+    int id = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 0;
+};
+Device &device(int irank = 0);          // initialises on first use
+
+// Grow-only device buffer.
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    void *get(size_t need);
+    void release();
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Grow-only pinned host buffer.
+struct HBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    void *get(size_t need);
+    void release();
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Event timing of the last launch of a kernel family.
+struct KTimer {
+    hipEvent_t a = nullptr, b = nullptr;
+    bool armed = false;
+    void start(hipStream_t s);
+    void stop(hipStream_t s);
+    double ms();          // synchronises on the stop event
+};
+KTimer &ktimer(const char *name);
+
+// ---------------------------------------------------------------- regf
+constexpr int JTILE = 256;   // j-particles per tile (one tile origin each)
+
+// j-set on the device, Morton-ordered at every send (jset.hip):
+//   in64  AoS [cap][8] caller order    (x y z m vx vy vz 0), upload staging
+//   rec64 AoS [cap][8] sorted order    (x y z m vx vy vz 0)
+//   rec32 [cap][2] float4 sorted       (x-P, y-P, z-P, m), (vx-V, vy-V, vz-V, 0)
+//   torg  [ntile][8] FP64 tile origins (Px Py Pz Vx Vy Vz 0 0)
+//   perm  [cap] int: sorted position -> caller's 0-based index
+struct JSet {
+    int n = 0;
+    int cap = 0;
+    int ntile = 0;
+    double mmax = 0.0;
+    DBuf in64, rec64, rec32, torg, perm, keys, vals, stat, sort_tmp;
+};
+void jset_reserve(JSet &js, int cap);
+void jset_release(JSet &js);
+// in64 already holds n records; box = {lo x,y,z, hi x,y,z} or nullptr (computed)
+void jset_build(JSet &js, int n, const double *box, double mmax, hipStream_t s);
+void jset_from_soa(JSet &js, int n, int stride, const double *d_soa, hipStream_t s);
+
+struct RegfConfig {
+    int ipl = 1;          // i-particles per lane
+    int min_groups = 2048;
+};
+RegfConfig &regf_config();
+
+// Launch the regular-force pipeline on device data.  d_i SoA[8][ni_stride],
+// d_out SoA[7][ni_stride], d_list [ni][lmax].
+int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lmax,
+                int nnbmax, int m_flag, double *d_out, int *d_list, hipStream_t s);
+int regf_last_chunks();
+void regf_release();
+
+// ---------------------------------------------------------------- pot
+int pot_launch(int n, int n_stride, const double *d_x, int i0, int ni, double *d_pot,
+               hipStream_t s);
+void pot_release();
+
+// ---------------------------------------------------------------- irr
+void irr_release();
+
+}  // namespace nb6

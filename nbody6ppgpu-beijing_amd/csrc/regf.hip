@@ -1,0 +1,582 @@
+// Regular force + neighbour list for gfx950.
+//
+// Semantics (what is computed) follow the reference:
+//   pair terms      gpunb.velocity.cu:163-209 / :212-258, reg.avx.cpp:250-305
+//   neighbour test  min(r2, |dx + dtr dv|^2) < h2 (or < m_j h2)   gpunb.velocity.cu:188,236
+//   neighbours add nothing to acc / jerk / pot                     gpunb.velocity.cu:192
+//   list rows       [0] = count, ascending 0-based j, -count on overflow  reg.avx.cpp:316-327
+//
+// Method (how it is computed) is this library's own:
+//
+//  * The j-set is Morton-ordered at send time (jset.hip) and streamed as FP32
+//    records relative to the origin of their 256-particle tile.  regf_tile_kernel
+//    (grid: i-blocks x j-chunks, 256 threads, IPL i-particles per lane) stages one
+//    tile in LDS per step, forms each lane's i-particle relative to the same tile
+//    origin in FP64, and evaluates every pair in FP32: v_rsq_f32 refined by one
+//    Newton-Raphson step, FP32 partial sums flushed into FP64 every 64 j.
+//
+//  * Candidates.  A pair whose FP32 velocity criterion passes the neighbour
+//    threshold widened by its rounding-error band is a "hard" candidate; a pair
+//    inside twice the neighbour radius is a "soft" (near-field) candidate.  Both
+//    are left out of the FP32 sums and their sorted j index goes to per-(i, chunk)
+//    slot arrays: hard candidates always (they contain every FP64 neighbour),
+//    soft ones while slots last (a soft pair without a slot simply stays in the
+//    FP32 sum).  The near field is where the 1/r^3-weighted jerk terms amplify
+//    FP32 rounding.  The test is two-stage: a cheap, conservative FP32 bound
+//    (r2p >= r2 + 2 dtr min(rv, 0)) guards a rarely-taken branch with the full test.
+//
+//  * regf_finalize_kernel: one wavefront per i.  Sums the FP64 chunk partials,
+//    re-tests every hard candidate in FP64 with the reference formula (no FMA
+//    contraction), adds the FP64 force of every non-neighbour candidate, maps
+//    neighbours back to the caller's indices and writes each row in ascending
+//    order (rank sort in LDS).
+#include "nb6_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace nb6 {
+
+namespace {
+
+constexpr int RF_THREADS = 256;
+constexpr int RF_TJ = JTILE;          // j-particles per LDS tile (one per thread)
+constexpr int RF_FLUSH = 64;          // j per FP32 partial sum
+constexpr int CAP_SOFT = 1024;        // soft (near-field) slots per (i, chunk)
+constexpr int NB_SORT_MAX = 2048;     // largest neighbour row the finalize sorts
+
+// rounding-error margins (u = 2^-24)
+constexpr double U24 = 5.9604644775390625e-08;
+constexpr float KAPPA_P = 1.9073486328125e-06f;   // 2^-19 = 32u : full test
+constexpr float KAPPA_Q = 3.814697265625e-06f;    // 2^-18 = 64u : cheap test
+constexpr double H_REL = 64.0 * U24;              // relative slack on h2
+constexpr float D_ABS = (float)(64.0 * U24);      // absolute slack factor (x2 spare)
+constexpr float Q_REL = (float)(192.0 * U24);     // cheap threshold over the full one
+                                                  // (includes 1/(1-KAPPA_Q))
+constexpr float F_NEAR = 2.0f;                    // near field: r < 2 RS
+constexpr float TINY = 1.1754944e-38f;            // keeps r2 = 0 a candidate
+
+struct TileParams {
+    const double *__restrict__ ib;   // SoA [8][ib_stride]: x y z vx vy vz h2 dtr
+    int ni, ib_stride;
+    const float4 *__restrict__ rec32;
+    const double *__restrict__ torg;
+    int nj;
+    int chunk_len;                   // multiple of RF_TJ
+    double mmax;
+    double *__restrict__ part;       // [nchunk][7][ni]
+    int *__restrict__ cnt;           // [nchunk][2][ni]: hard, soft
+    int *__restrict__ hslot;         // [(chunk*ni + i) * caph + k]
+    int caph;
+    int *__restrict__ sslot;         // [(chunk*ni + i) * caps + k]
+    int caps;
+};
+
+// per-lane constants that do not depend on the tile
+struct ILane {
+    double x, y, z, u, v, w;         // FP64 position / velocity
+    float dt1, dt2;                  // dtr, 2 dtr
+    float h2H, h2S;                  // (1+slack) h2 and F_NEAR^2 (1+slack) h2 (MFLAG: slopes)
+    float rsH, rsS;                  // radii used in the absolute error bands
+    bool ok;
+};
+
+// per-lane constants relative to one tile origin
+struct IC {
+    float x, y, z, u, v, w;          // i relative to the tile origin
+    float dt1, dt2;
+    float hH, dH, hS, dS, hQ, dQ;    // thresholds (MFLAG: slopes h*, offsets d*)
+};
+
+template <bool MFLAG>
+__device__ __forceinline__ ILane make_lane(const double *__restrict__ ib, int is, int i, bool ok,
+                                           double mmax) {
+    ILane L;
+    const int ii = ok ? i : 0;
+    L.x = ib[0 * is + ii]; L.y = ib[1 * is + ii]; L.z = ib[2 * is + ii];
+    L.u = ib[3 * is + ii]; L.v = ib[4 * is + ii]; L.w = ib[5 * is + ii];
+    const double h2 = ib[6 * is + ii], dtr = ib[7 * is + ii];
+    L.dt1 = (float)dtr;
+    // r2 + 2 dtr min(rv,0) <= (r2 - kq r2 + 2 dtr rv)/(1 - kq): the cheap test uses
+    // 2 dtr/(1-kq) here and thresholds divided by (1-kq) (folded into Q_REL)
+    L.dt2 = (float)(2.0 * dtr / (1.0 - (double)KAPPA_Q));
+    const double hh = h2 > 0.0 ? h2 : 0.0;
+    const double hk = hh * (1.0 + H_REL) * (1.0 + 4.0 * U24);
+    L.h2H = (float)hk;
+    L.h2S = (float)(hk * (double)(F_NEAR * F_NEAR));
+    const double rs = MFLAG ? sqrt(hh * mmax) : sqrt(hh);
+    L.rsH = (float)(rs * (1.0 + 4.0 * U24));
+    L.rsS = F_NEAR * L.rsH;
+    L.ok = ok;
+    return L;
+}
+
+template <bool MFLAG>
+__device__ __forceinline__ IC tile_ic(const ILane &L, const double *__restrict__ T) {
+    IC c;
+    c.x = (float)(L.x - T[0]); c.y = (float)(L.y - T[1]); c.z = (float)(L.z - T[2]);
+    c.u = (float)(L.u - T[3]); c.v = (float)(L.v - T[4]); c.w = (float)(L.w - T[5]);
+    c.dt1 = L.dt1;
+    c.dt2 = L.dt2;   // 2 dtr / (1 - kappa_q), see make_lane
+    // |x_i - P|, |v_i - V| bound the rounding error of the FP32 separations
+    const float A = sqrtf(fmaf(c.z, c.z, fmaf(c.y, c.y, c.x * c.x))) * 1.0001f;
+    const float B = sqrtf(fmaf(c.w, c.w, fmaf(c.v, c.v, c.u * c.u))) * 1.0001f;
+    const float g = A + fabsf(c.dt1) * B;
+    const float dH = D_ABS * L.rsH * (g + L.rsH) + TINY;
+    const float dS = D_ABS * L.rsS * (g + L.rsS) + TINY;
+    if (MFLAG) {
+        c.hH = L.h2H; c.dH = dH;
+        c.hS = L.h2S; c.dS = dS;
+        c.hQ = L.h2S * (1.0f + Q_REL); c.dQ = dS * (1.0f + Q_REL);
+    } else {
+        c.hH = L.h2H + dH; c.dH = 0.f;
+        c.hS = L.h2S + dS; c.dS = 0.f;
+        c.hQ = c.hS * (1.0f + Q_REL); c.dQ = 0.f;
+    }
+    if (!L.ok) {
+        c.hH = c.hS = c.hQ = -INFINITY;
+        c.dH = c.dS = c.dQ = -INFINITY;
+    }
+    return c;
+}
+
+struct PairBase {
+    float dx, dy, dz, du, dv, dw, r2, rv;
+};
+
+__device__ __forceinline__ PairBase pair_base(const IC &c, const float4 &a, const float4 &b) {
+    PairBase q;
+    q.dx = a.x - c.x; q.dy = a.y - c.y; q.dz = a.z - c.z;
+    q.du = b.x - c.u; q.dv = b.y - c.v; q.dw = b.z - c.w;
+    q.r2 = fmaf(q.dz, q.dz, fmaf(q.dy, q.dy, q.dx * q.dx));
+    q.rv = fmaf(q.dz, q.dw, fmaf(q.dy, q.dv, q.dx * q.du));
+    return q;
+}
+
+template <int IPL, bool MFLAG>
+__global__ __launch_bounds__(RF_THREADS) void regf_tile_kernel(TileParams p) {
+    __shared__ float4 sA[2][RF_TJ];   // x', y', z', m
+    __shared__ float4 sB[2][RF_TJ];   // vx', vy', vz', 0
+
+    const int tid = threadIdx.x;
+    const int chunk = blockIdx.y;
+    const int ibase = blockIdx.x * (RF_THREADS * IPL);
+
+    ILane L[IPL];
+    int nh[IPL], ns[IPL];
+    int *hp[IPL], *sp[IPL];
+    float a32[IPL][10];
+    double a64[IPL][7];
+
+#pragma unroll
+    for (int k = 0; k < IPL; k++) {
+        const int i = ibase + k * RF_THREADS + tid;
+        const bool ok = i < p.ni;
+        L[k] = make_lane<MFLAG>(p.ib, p.ib_stride, i, ok, p.mmax);
+        nh[k] = 0;
+        ns[k] = 0;
+        const size_t row = (size_t)chunk * p.ni + (ok ? i : 0);
+        hp[k] = p.hslot + row * p.caph;
+        sp[k] = p.sslot + row * p.caps;
+#pragma unroll
+        for (int f = 0; f < 10; f++) a32[k][f] = 0.f;
+#pragma unroll
+        for (int f = 0; f < 7; f++) a64[k][f] = 0.0;
+    }
+
+    const int j0 = chunk * p.chunk_len;
+    const int j1 = min(j0 + p.chunk_len, p.nj);
+    const int ntile = (j1 - j0 + RF_TJ - 1) / RF_TJ;
+    const int tile0 = j0 / RF_TJ;
+
+    float4 ga, gb;
+    auto gload = [&](int t) {
+        const int j = j0 + t * RF_TJ + tid;
+        if (j < j1) {
+            ga = p.rec32[2 * (size_t)j];
+            gb = p.rec32[2 * (size_t)j + 1];
+        } else {   // far, massless dummy: never a candidate, contributes zero
+            ga = make_float4(1.0e17f, 0.f, 0.f, 0.f);
+            gb = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    };
+
+    gload(0);
+    sA[0][tid] = ga;
+    sB[0][tid] = gb;
+    __syncthreads();
+
+    for (int t = 0; t < ntile; t++) {
+        const int buf = t & 1;
+        if (t + 1 < ntile) gload(t + 1);
+        const int jt = j0 + t * RF_TJ;
+        const double *T = p.torg + (size_t)(tile0 + t) * 8;
+        IC c[IPL];
+#pragma unroll
+        for (int k = 0; k < IPL; k++) c[k] = tile_ic<MFLAG>(L[k], T);
+
+        for (int js0 = 0; js0 < RF_TJ; js0 += RF_FLUSH) {
+#pragma unroll 2
+            for (int jj = js0; jj < js0 + RF_FLUSH; jj++) {
+                const float4 a = sA[buf][jj];
+                const float4 b = sB[buf][jj];
+#pragma unroll
+                for (int k = 0; k < IPL; k++) {
+                    const PairBase q = pair_base(c[k], a, b);
+                    // v_rsq_f32 plus one Newton-Raphson step
+                    float rinv = __builtin_amdgcn_rsqf(q.r2);
+                    {
+                        const float h = rinv * rinv;
+                        const float e = fmaf(-q.r2, h, 1.0f);
+                        rinv = fmaf(rinv, 0.5f * e, rinv);
+                    }
+                    // cheap conservative bound on the soft test:
+                    // r2p >= r2 + 2 dtr min(rv, 0)  (dt2 carries the 1/(1-kappa) slack)
+                    const float t1 = fmaf(c[k].dt2, fminf(q.rv, 0.f), q.r2);
+                    const float thq = MFLAG ? fmaf(a.w, c[k].hQ, c[k].dQ) : c[k].hQ;
+                    if (t1 < thq) {
+                        // keep the full test inside the rare branch: the empty asm
+                        // makes dtr opaque here, so the compiler cannot hoist it
+                        float dtl = c[k].dt1;
+                        asm volatile("" : "+v"(dtl));
+                        const float ex = fmaf(dtl, q.du, q.dx);
+                        const float ey = fmaf(dtl, q.dv, q.dy);
+                        const float ez = fmaf(dtl, q.dw, q.dz);
+                        const float r2p = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+                        const float mn = fminf(q.r2, fmaf(-KAPPA_P, q.r2, r2p));
+                        const float thh = MFLAG ? fmaf(a.w, c[k].hH, c[k].dH) : c[k].hH;
+                        const float ths = MFLAG ? fmaf(a.w, c[k].hS, c[k].dS) : c[k].hS;
+                        const int jg = jt + jj;
+                        if (mn < thh && jg < j1) {
+                            if (nh[k] < p.caph) hp[k][nh[k]] = jg;
+                            nh[k]++;
+                            rinv = 0.f;
+                        } else if (mn < ths && jg < j1 && ns[k] < p.caps) {
+                            sp[k][ns[k]] = jg;
+                            ns[k]++;
+                            rinv = 0.f;
+                        }
+                    }
+                    const float rinv2 = rinv * rinv;
+                    const float mr1 = a.w * rinv;
+                    const float mr3 = mr1 * rinv2;
+                    const float be = (q.rv * rinv2) * mr3;
+                    a32[k][0] = fmaf(mr3, q.dx, a32[k][0]);
+                    a32[k][1] = fmaf(mr3, q.dy, a32[k][1]);
+                    a32[k][2] = fmaf(mr3, q.dz, a32[k][2]);
+                    a32[k][3] += mr1;
+                    a32[k][4] = fmaf(mr3, q.du, a32[k][4]);
+                    a32[k][5] = fmaf(mr3, q.dv, a32[k][5]);
+                    a32[k][6] = fmaf(mr3, q.dw, a32[k][6]);
+                    a32[k][7] = fmaf(be, q.dx, a32[k][7]);
+                    a32[k][8] = fmaf(be, q.dy, a32[k][8]);
+                    a32[k][9] = fmaf(be, q.dz, a32[k][9]);
+                }
+            }
+            // flush FP32 partials into FP64 (jerk = J1 - 3 J2)
+#pragma unroll
+            for (int k = 0; k < IPL; k++) {
+                a64[k][0] += (double)a32[k][0];
+                a64[k][1] += (double)a32[k][1];
+                a64[k][2] += (double)a32[k][2];
+                a64[k][3] += (double)a32[k][4] - 3.0 * (double)a32[k][7];
+                a64[k][4] += (double)a32[k][5] - 3.0 * (double)a32[k][8];
+                a64[k][5] += (double)a32[k][6] - 3.0 * (double)a32[k][9];
+                a64[k][6] += (double)a32[k][3];
+#pragma unroll
+                for (int f = 0; f < 10; f++) a32[k][f] = 0.f;
+            }
+        }
+
+        if (t + 1 < ntile) {
+            sA[buf ^ 1][tid] = ga;
+            sB[buf ^ 1][tid] = gb;
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int k = 0; k < IPL; k++) {
+        const int i = ibase + k * RF_THREADS + tid;
+        if (i < p.ni) {
+            double *pp = p.part + (size_t)chunk * 7 * p.ni + i;
+#pragma unroll
+            for (int f = 0; f < 7; f++) pp[(size_t)f * p.ni] = a64[k][f];
+            p.cnt[((size_t)chunk * 2 + 0) * p.ni + i] = nh[k];
+            p.cnt[((size_t)chunk * 2 + 1) * p.ni + i] = ns[k];
+        }
+    }
+}
+
+struct FinParams {
+    const double *__restrict__ ib;   // SoA [8][ib_stride]
+    int ni, ib_stride;
+    const double *__restrict__ rec64;   // sorted AoS [nj][8]
+    const int *__restrict__ perm;       // sorted -> caller index
+    int nj;
+    int nchunk;
+    const double *__restrict__ part;
+    const int *__restrict__ cnt;
+    const int *__restrict__ hslot;
+    int caph;
+    const int *__restrict__ sslot;
+    int caps;
+    int lmax, nnbmax;                // nnbmax already min(nnbmax, lmax-1)
+    double *__restrict__ out;        // SoA [7][out_stride]
+    int out_stride;
+    int *__restrict__ list;          // [ni][lmax]
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ int lane_prefix(unsigned long long b) {
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((unsigned)b, 0u));
+}
+
+struct IF64 {
+    double x, y, z, u, v, w, h2, dtr;
+};
+
+// FP64 neighbour test of sorted j against i (formula order = the oracle's) and,
+// for a non-neighbour, its exact force added to s[].  test=false: force only.
+template <bool MFLAG>
+__device__ __forceinline__ bool classify(const double *__restrict__ rec, const IF64 &I, bool test,
+                                         double *s) {
+    const double4 r0 = reinterpret_cast<const double4 *>(rec)[0];   // x y z m
+    const double4 r1 = reinterpret_cast<const double4 *>(rec)[1];   // vx vy vz 0
+    double dx, dy, dz, du, dv, dw, r2, rmin;
+    {
+#pragma clang fp contract(off)
+        dx = r0.x - I.x; dy = r0.y - I.y; dz = r0.z - I.z;
+        du = r1.x - I.u; dv = r1.y - I.v; dw = r1.z - I.w;
+        r2 = dx * dx + dy * dy + dz * dz;
+        const double ex = dx + I.dtr * du;
+        const double ey = dy + I.dtr * dv;
+        const double ez = dz + I.dtr * dw;
+        const double r2p = ex * ex + ey * ey + ez * ez;
+        rmin = r2 < r2p ? r2 : r2p;
+    }
+    const double thr = MFLAG ? r0.w * I.h2 : I.h2;
+    const bool isnb = test && rmin < thr;
+    if (!isnb && r2 > 0.0) {
+        const double rinv2 = 1.0 / r2;
+        const double rinv = sqrt(rinv2);
+        const double mr1 = r0.w * rinv;
+        const double mr3 = mr1 * rinv2;
+        const double rv = dx * du + dy * dv + dz * dw;
+        const double al = -3.0 * rv * rinv2;
+        s[0] += mr3 * dx;
+        s[1] += mr3 * dy;
+        s[2] += mr3 * dz;
+        s[3] += mr3 * (du + al * dx);
+        s[4] += mr3 * (dv + al * dy);
+        s[5] += mr3 * (dw + al * dz);
+        s[6] += mr1;
+    }
+    return isnb;
+}
+
+template <bool MFLAG>
+__global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
+    __shared__ int nbuf[4][NB_SORT_MAX];
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    const int i = blockIdx.x * 4 + wid;
+    if (i >= f.ni) return;   // whole wavefront leaves together
+    int *buf = nbuf[wid];
+
+    // 1. FP64 sum of the chunk partials
+    double s[7];
+#pragma unroll
+    for (int q = 0; q < 7; q++) s[q] = 0.0;
+    for (int c = lane; c < f.nchunk; c += 64) {
+        const double *pp = f.part + (size_t)c * 7 * f.ni + i;
+#pragma unroll
+        for (int q = 0; q < 7; q++) s[q] += pp[(size_t)q * f.ni];
+    }
+
+    // 2. FP64 work on the candidates
+    const int is = f.ib_stride;
+    IF64 I;
+    I.x = f.ib[0 * is + i]; I.y = f.ib[1 * is + i]; I.z = f.ib[2 * is + i];
+    I.u = f.ib[3 * is + i]; I.v = f.ib[4 * is + i]; I.w = f.ib[5 * is + i];
+    I.h2 = f.ib[6 * is + i]; I.dtr = f.ib[7 * is + i];
+
+    int nnb = 0;          // exact neighbours found (wave-uniform)
+    int dropped = 0;      // hard candidates beyond the slot capacity
+    for (int cb = 0; cb < f.nchunk; cb += 64) {
+        const int c = cb + lane;
+        const int nh = c < f.nchunk ? f.cnt[((size_t)c * 2 + 0) * f.ni + i] : 0;
+        const int ns = c < f.nchunk ? f.cnt[((size_t)c * 2 + 1) * f.ni + i] : 0;
+        dropped += wave_sum_i(nh > f.caph ? nh - f.caph : 0);
+        unsigned long long mask = __ballot(nh > 0 || ns > 0);
+        while (mask) {
+            const int cl = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            const size_t rowc = (size_t)(cb + cl) * f.ni + i;
+            const int nhc = min(__shfl(nh, cl, 64), f.caph);
+            const int nsc = __shfl(ns, cl, 64);
+            const int *hs = f.hslot + rowc * f.caph;
+            for (int kb = 0; kb < nhc; kb += 64) {
+                const int k = kb + lane;
+                bool isnb = false;
+                int jo = 0;
+                if (k < nhc) {
+                    const int sj = hs[k];
+                    if ((unsigned)sj < (unsigned)f.nj) {
+                        isnb = classify<MFLAG>(f.rec64 + (size_t)sj * 8, I, true, s);
+                        jo = f.perm[sj];
+                    }
+                }
+                const unsigned long long b = __ballot(isnb);
+                const int pos = nnb + lane_prefix(b);
+                if (isnb && pos < NB_SORT_MAX) buf[pos] = jo;
+                nnb += __popcll(b);
+            }
+            const int *ss = f.sslot + rowc * f.caps;
+            for (int k = lane; k < nsc; k += 64) {
+                const int sj = ss[k];
+                if ((unsigned)sj < (unsigned)f.nj)
+                    classify<MFLAG>(f.rec64 + (size_t)sj * 8, I, false, s);
+            }
+        }
+    }
+
+#pragma unroll
+    for (int q = 0; q < 7; q++) s[q] = wave_sum(s[q]);
+    int *row = f.list + (size_t)i * f.lmax;
+    const int tot = nnb + dropped;
+    if (lane == 0) {
+        double *o = f.out + i;
+        const int os = f.out_stride;
+        o[0 * os] = s[0]; o[1 * os] = s[1]; o[2 * os] = s[2];
+        o[3 * os] = s[3]; o[4 * os] = s[4]; o[5 * os] = s[5];
+        o[6 * os] = s[6];
+        row[0] = tot > f.nnbmax ? -tot : tot;
+    }
+    if (tot <= f.nnbmax && nnb > 0) {
+        // ascending row: the rank of each (distinct) index among the row
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int k = lane; k < nnb; k += 64) {
+            const int e = buf[k];
+            int rank = 0;
+            for (int m = 0; m < nnb; m++) rank += buf[m] < e;
+            row[1 + rank] = e;
+        }
+    }
+}
+
+struct Work {
+    DBuf part, cnt, hslot, sslot;
+    int last_chunks = 0;
+} g_work;
+
+template <int IPL>
+void launch_tile(bool mflag, dim3 grid, const TileParams &tp, hipStream_t s) {
+    if (mflag)
+        hipLaunchKernelGGL((regf_tile_kernel<IPL, true>), grid, dim3(RF_THREADS), 0, s, tp);
+    else
+        hipLaunchKernelGGL((regf_tile_kernel<IPL, false>), grid, dim3(RF_THREADS), 0, s, tp);
+}
+
+}  // namespace
+
+RegfConfig &regf_config() {
+    static RegfConfig c;
+    return c;
+}
+
+int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lmax,
+                int nnbmax, int m_flag, double *d_out, int *d_list, hipStream_t s) {
+    if (ni <= 0) return OK;
+    if (js.n <= 0) {
+        set_error(E_STATE, "gpunb_regf: no j-particles sent");
+        return E_STATE;
+    }
+    if (lmax < 1 || lmax > NB_SORT_MAX + 1) {
+        set_error(E_ARG, "gpunb_regf: lmax outside [1, 2049]");
+        return E_ARG;
+    }
+    const RegfConfig &cfg = regf_config();
+    const int ipl = cfg.ipl;
+    const int nbi = RF_THREADS * ipl;
+    const int nib = (ni + nbi - 1) / nbi;
+    const int nj = js.n;
+    const int ntile = (nj + RF_TJ - 1) / RF_TJ;
+    int want = (cfg.min_groups + nib - 1) / nib;
+    int maxc = std::max(1, ntile / 4);
+    int nchunk = std::max(1, std::min(want, maxc));
+    const int tpc = (ntile + nchunk - 1) / nchunk;   // tiles per chunk
+    const int clen = tpc * RF_TJ;
+    nchunk = (ntile + tpc - 1) / tpc;
+    const int nnbeff = std::max(0, std::min(nnbmax, lmax - 1));
+    const int caph = nnbeff + 8;
+    const int caps = CAP_SOFT;
+
+    const size_t rows = (size_t)nchunk * ni;
+    double *part = static_cast<double *>(g_work.part.get(rows * 7 * sizeof(double)));
+    int *cnt = static_cast<int *>(g_work.cnt.get(rows * 2 * sizeof(int)));
+    int *hslot = static_cast<int *>(g_work.hslot.get(rows * caph * sizeof(int)));
+    int *sslot = static_cast<int *>(g_work.sslot.get(rows * caps * sizeof(int)));
+    g_work.last_chunks = nchunk;
+
+    TileParams tp;
+    tp.ib = d_i; tp.ni = ni; tp.ib_stride = ni_stride;
+    tp.rec32 = js.rec32.as<float4>(); tp.torg = js.torg.as<double>(); tp.nj = nj;
+    tp.chunk_len = clen;
+    tp.mmax = js.mmax;
+    tp.part = part; tp.cnt = cnt;
+    tp.hslot = hslot; tp.caph = caph; tp.sslot = sslot; tp.caps = caps;
+    dim3 grid(nib, nchunk);
+    KTimer &t1 = ktimer("regf_tile");
+    t1.start(s);
+    switch (ipl) {
+        case 2: launch_tile<2>(m_flag != 0, grid, tp, s); break;
+        case 4: launch_tile<4>(m_flag != 0, grid, tp, s); break;
+        default: launch_tile<1>(m_flag != 0, grid, tp, s); break;
+    }
+    NB6_HIP(hipGetLastError());
+    t1.stop(s);
+
+    FinParams fp;
+    fp.ib = d_i; fp.ni = ni; fp.ib_stride = ni_stride;
+    fp.rec64 = js.rec64.as<double>(); fp.perm = js.perm.as<int>(); fp.nj = nj;
+    fp.nchunk = nchunk;
+    fp.part = part; fp.cnt = cnt;
+    fp.hslot = hslot; fp.caph = caph; fp.sslot = sslot; fp.caps = caps;
+    fp.lmax = lmax; fp.nnbmax = nnbeff;
+    fp.out = d_out; fp.out_stride = ni_stride; fp.list = d_list;
+    KTimer &t2 = ktimer("regf_finalize");
+    t2.start(s);
+    const dim3 fgrid((ni + 3) / 4);
+    if (m_flag)
+        hipLaunchKernelGGL(regf_finalize_kernel<true>, fgrid, dim3(256), 0, s, fp);
+    else
+        hipLaunchKernelGGL(regf_finalize_kernel<false>, fgrid, dim3(256), 0, s, fp);
+    NB6_HIP(hipGetLastError());
+    t2.stop(s);
+    return OK;
+}
+
+int regf_last_chunks() { return g_work.last_chunks; }
+
+void regf_release() {
+    g_work.part.release();
+    g_work.cnt.release();
+    g_work.hslot.release();
+    g_work.sslot.release();
+}
+
+}  // namespace nb6

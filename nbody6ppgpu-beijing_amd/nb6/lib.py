@@ -1,0 +1,208 @@
+"""ctypes binding of libnb6gpu.so (the product library) and a thin Python
+mirror of the reference's host interface for this path.
+
+The classes keep the reference's names, argument meaning and error behaviour:
+
+* ``Gpunb``  -- gpunb_open_/send_/regf_/profile_/close_ (gpunb.velocity.cu:905-940)
+* ``gpupot`` -- gpupot_ (gpupot.gpu.cu:116-127)
+* ``IrrSimd`` -- irr_simd_open_/set_jp_/set_list_/firr_vec_/close_ (irr.avx.cpp:568-606)
+
+There is no CPU fallback: if the library is missing, ``load()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent.parent
+LIBPATH = PKG / "libnb6gpu.so"
+HEADER = PKG.parent / "include" / "nb6gpu.h"
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+_L = None
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _ri(v):
+    return ctypes.byref(ctypes.c_int(int(v)))
+
+
+def _rd(v):
+    return ctypes.byref(ctypes.c_double(float(v)))
+
+
+def load():
+    """Load libnb6gpu.so (raises if it has not been built)."""
+    global _L
+    if _L is not None:
+        return _L
+    if not LIBPATH.exists():
+        raise RuntimeError(f"{LIBPATH} is not built; run __graft_entry__.build() or make -C "
+                           f"{PKG}")
+    L = ctypes.CDLL(str(LIBPATH), mode=os.RTLD_LOCAL)
+    L.nb6_last_error.restype = ctypes.c_int
+    L.nb6_last_error_message.restype = ctypes.c_char_p
+    L.nb6_device_count.restype = ctypes.c_int
+    L.nb6_kernel_ms.restype = ctypes.c_double
+    L.nb6_kernel_ms.argtypes = [ctypes.c_char_p]
+    L.nb6_last_regf_chunks.restype = ctypes.c_int
+    L.nb6_dev_alloc.restype = _vp
+    L.nb6_dev_alloc.argtypes = [ctypes.c_ulonglong]
+    L.nb6_dev_free.argtypes = [_vp]
+    L.nb6_memcpy_htod.argtypes = [_vp, _vp, ctypes.c_ulonglong]
+    L.nb6_memcpy_dtoh.argtypes = [_vp, _vp, ctypes.c_ulonglong]
+    L.nb6_regf_device.argtypes = [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, _vp, _vp, _vp]
+    L.nb6_send_device.argtypes = [ctypes.c_int, ctypes.c_int, _vp, _vp]
+    L.nb6_pot_device.argtypes = [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int,
+                                 _vp, _vp]
+    L.nb6_set_regf_config.argtypes = [ctypes.c_int, ctypes.c_int]
+    _L = L
+    return L
+
+
+def header_symbols() -> list[str]:
+    """Function names declared in include/nb6gpu.h."""
+    import re
+    txt = HEADER.read_text()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt.split("extern \"C\" {", 1)[1])
+
+
+def check_error():
+    L = load()
+    code = L.nb6_last_error()
+    if code:
+        msg = L.nb6_last_error_message().decode()
+        L.nb6_clear_error()
+        raise RuntimeError(f"nb6gpu error {code}: {msg}")
+
+
+def c64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Gpunb:
+    """Mirror of the reference regular-force library (one instance at a time)."""
+
+    def __init__(self, nbmax: int, rank: int = 0):
+        self.L = load()
+        self.rank = rank
+        self.L.gpunb_open_(_ri(nbmax), _ri(rank))
+        check_error()
+
+    def send(self, m, x, v):
+        m, x, v = c64(m), c64(x), c64(v)
+        self.L.gpunb_send_(_ri(m.size), _d(m), _d(x), _d(v))
+        check_error()
+
+    def regf(self, h2, dtr, xi, vi, lmax, nnbmax, m_flag=0):
+        h2, dtr, xi, vi = c64(h2), c64(dtr), c64(xi), c64(vi)
+        ni = h2.size
+        acc = np.zeros((ni, 3)); jrk = np.zeros((ni, 3)); pot = np.zeros(ni)
+        lst = np.zeros((ni, lmax), dtype=np.int32)
+        self.L.gpunb_regf_(_ri(ni), _d(h2), _d(dtr), _d(xi), _d(vi), _d(acc), _d(jrk), _d(pot),
+                           _ri(lmax), _ri(nnbmax), _i(lst), _ri(m_flag))
+        check_error()
+        return acc, jrk, pot, lst
+
+    def profile(self):
+        self.L.gpunb_profile_(_ri(self.rank))
+
+    def close(self):
+        self.L.gpunb_close_()
+
+
+def gpupot(m, x, istart=1, ni=None, rank=0):
+    L = load()
+    m, x = c64(m), c64(x)
+    n = m.size
+    ni = n - (istart - 1) if ni is None else ni
+    out = np.zeros(max(ni, 1))
+    L.gpupot_(_ri(rank), _ri(istart), _ri(ni), _ri(n), _d(m), _d(x), _d(out))
+    check_error()
+    return out[:ni]
+
+
+class IrrSimd:
+    """Mirror of the reference irregular-force library (irr_simd_* / gpuirr_*)."""
+
+    def __init__(self, nmax, lmax, rank=0, alias=False):
+        self.L = load()
+        self.rank = rank
+        self.alias = alias
+        if alias:
+            self.L.gpuirr_open_(_ri(nmax), _ri(lmax))
+        else:
+            self.L.irr_simd_open_(_ri(nmax), _ri(lmax), _ri(rank))
+        check_error()
+
+    def set_jp(self, addr, pos, vel, acc2, jrk6, mass, time):
+        f = self.L.gpuirr_set_jp_ if self.alias else self.L.irr_simd_set_jp_
+        f(_ri(addr), _d(c64(pos)), _d(c64(vel)), _d(c64(acc2)), _d(c64(jrk6)), _rd(mass), _rd(time))
+        check_error()
+
+    def set_list(self, addr, row):
+        row = np.ascontiguousarray(row, dtype=np.int32)
+        f = self.L.gpuirr_set_list_ if self.alias else self.L.irr_simd_set_list_
+        f(_ri(addr), _i(row))
+        check_error()
+
+    def firr_vec(self, ti, addr):
+        addr = np.ascontiguousarray(addr, dtype=np.int32)
+        ni = addr.size
+        acc = np.zeros((ni, 3)); jrk = np.zeros((ni, 3)); nn = np.zeros(ni, np.int32)
+        f = self.L.gpuirr_firr_vec_ if self.alias else self.L.irr_simd_firr_vec_
+        f(_rd(ti), _ri(ni), _i(addr), _d(acc), _d(jrk), _i(nn))
+        check_error()
+        return acc, jrk, nn
+
+    def profile(self):
+        self.L.irr_simd_profile_(_ri(self.rank))
+
+    def close(self):
+        if self.alias:
+            self.L.gpuirr_close_()
+        else:
+            self.L.irr_simd_close_(_ri(self.rank))
+
+
+class DeviceArray:
+    """Minimal owned device buffer for the device-resident entry points."""
+
+    def __init__(self, nbytes: int):
+        self.L = load()
+        self.nbytes = int(nbytes)
+        self.ptr = self.L.nb6_dev_alloc(self.nbytes)
+        if not self.ptr:
+            check_error()
+            raise RuntimeError("device allocation failed")
+
+    @classmethod
+    def from_host(cls, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes)
+        d.L.nb6_memcpy_htod(d.ptr, a.ctypes.data, a.nbytes)
+        return d
+
+    def to_host(self, dtype, shape):
+        out = np.empty(shape, dtype=dtype)
+        self.L.nb6_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes)
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.L.nb6_dev_free(self.ptr)
+            self.ptr = None

@@ -1,0 +1,79 @@
+"""GPU parity of irr_simd_firr_vec_ (and the gpuirr_* aliases) against the FP64
+oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import c1_inputs
+from nb6 import lib
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 2.0e-6
+
+
+def irr_state(n=1024, seed=5):
+    """Particles with predictor records, per-particle times on a block grid,
+    neighbour lists from the FP64 regular-force oracle (self removed, 1-based)."""
+    m, x, v, h2, dtr = c1_inputs(n=n)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    a2 = 0.5 * rng.normal(0, 0.3, (n, 3))
+    j6 = rng.normal(0, 0.2, (n, 3)) / 6.0
+    t0 = np.floor(rng.uniform(0, 64, n)) / 1024.0
+    _, _, _, nb = O.regf(m, x, v, h2, dtr, x, v, 400, 350)
+    lists = np.zeros((n, 400), np.int32)
+    for i in range(n):
+        c = nb[i, 0]
+        row = [j + 1 for j in nb[i, 1:1 + c] if j != i]
+        lists[i, 0] = len(row)
+        lists[i, 1:1 + len(row)] = row
+    return m, x, v, a2, j6, t0, lists
+
+
+@pytest.mark.parametrize("alias", [False, True])
+def test_firr_matches_fp64(alias):
+    m, x, v, a2, j6, t0, lists = irr_state()
+    n = m.size
+    ti = 64.0 / 1024.0
+    irr = lib.IrrSimd(n, 400, alias=alias)
+    for i in range(n):
+        irr.set_jp(i + 1, x[i], v[i], a2[i], j6[i], m[i], t0[i])
+        irr.set_list(i + 1, lists[i])
+    addr = np.arange(1, n + 1, dtype=np.int32)[::3]
+    acc, jrk, nn = irr.firr_vec(ti, addr)
+    racc, rjrk, rnn = O.firr(ti, addr, x, v, a2, j6, m, t0, lists, 400)
+    irr.profile()
+    irr.close()
+    ea = np.linalg.norm(acc - racc, axis=1) / np.linalg.norm(racc, axis=1)
+    ej = np.linalg.norm(jrk - rjrk, axis=1) / np.linalg.norm(rjrk, axis=1)
+    print(f"irr: acc max {ea.max():.3e} jrk max {ej.max():.3e} nnbid agree "
+          f"{(nn == rnn).mean():.4f}", flush=True)
+    assert ea.max() <= REL_TOL and ej.max() <= REL_TOL
+    assert (nn == rnn).all()
+
+
+def test_firr_update_one_particle_and_list():
+    """set_jp / set_list after a firr call take effect at the next call; a
+    repeated set_list of one particle keeps the last list."""
+    m, x, v, a2, j6, t0, lists = irr_state(n=512, seed=9)
+    n = m.size
+    irr = lib.IrrSimd(n, 400)
+    for i in range(n):
+        irr.set_jp(i + 1, x[i], v[i], a2[i], j6[i], m[i], t0[i])
+        irr.set_list(i + 1, lists[i])
+    ti = 0.0625
+    addr = np.array([1, 2, 3, 100], np.int32)
+    irr.firr_vec(ti, addr)
+    x2 = x.copy(); x2[5] += 1e-3
+    irr.set_jp(6, x2[5], v[5], a2[5], j6[5], m[5], t0[5])
+    lists2 = lists.copy()
+    short = lists[2, 1:1 + lists[2, 0]][: 3]
+    irr.set_list(3, np.r_[7, lists[2, 1:8]].astype(np.int32))
+    lists2[2, 0] = len(short); lists2[2, 1:1 + len(short)] = short
+    irr.set_list(3, np.r_[len(short), short].astype(np.int32))
+    acc, jrk, nn = irr.firr_vec(ti, addr)
+    irr.close()
+    racc, rjrk, rnn = O.firr(ti, addr, x2, v, a2, j6, m, t0, lists2, 400)
+    ea = np.linalg.norm(acc - racc, axis=1) / np.linalg.norm(racc, axis=1)
+    assert ea.max() <= REL_TOL
+    assert (nn == rnn).all()

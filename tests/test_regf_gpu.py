@@ -1,0 +1,152 @@
+"""GPU parity of gpunb_regf_ against the FP64 oracle (oracle/nb6_oracle.c).
+
+Bar: neighbour lists bit-exact (same sorted sets, same counts, same overflow
+sign and magnitude); acc and pot within REL_TOL of FP64 for every particle
+(per-particle vector relative error).  Jerk: 99th percentile within REL_TOL
+and every particle within JRK_MAX.  The regular jerk excludes the neighbours,
+so |jerk| is a random-walk sum that is occasionally very small for a given
+particle; its relative error therefore has a tail that grows with the number of
+particles sampled, for any FP32 far field.  The reference's own FP32 jerk
+error at the 1k Plummer case is 2.2e-6 (max), see test_golden_cpu.py.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import (LMAX_LARGE, LMAX_SMALL, c1_inputs, c2_inputs, nnbmax_for,
+                     rows_as_sets, vec_rel_err)
+from nb6 import lib
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1.0e-6          # acc / pot vs FP64, every particle; jerk at the 99th percentile
+JRK_MAX = 4.0e-6          # jerk vs FP64, every particle
+
+
+def _compare(got, ref, label):
+    acc, jrk, pot, lst = got
+    racc, rjrk, rpot, rlst = ref
+    ea = vec_rel_err(acc, racc)
+    ej = vec_rel_err(jrk, rjrk)
+    ep = vec_rel_err(pot[:, None], rpot[:, None])
+    print(f"{label}: acc max {ea.max():.3e} med {np.median(ea):.3e} | jrk max {ej.max():.3e} "
+          f"p99 {np.percentile(ej, 99):.3e} med {np.median(ej):.3e} | pot max {ep.max():.3e}",
+          flush=True)
+    assert (lst[:, 0] == rlst[:, 0]).all(), "neighbour counts differ"
+    assert rows_as_sets(lst) == rows_as_sets(rlst), "neighbour lists differ"
+    assert ea.max() <= REL_TOL, f"acc rel err {ea.max()}"
+    assert ep.max() <= REL_TOL, f"pot rel err {ep.max()}"
+    assert np.percentile(ej, 99) <= REL_TOL, f"jrk p99 rel err {np.percentile(ej, 99)}"
+    assert ej.max() <= JRK_MAX, f"jrk rel err {ej.max()}"
+
+
+@pytest.fixture
+def gpunb():
+    g = None
+
+    def make(nbmax):
+        nonlocal g
+        g = lib.Gpunb(nbmax)
+        return g
+    yield make
+    if g is not None:
+        g.close()
+
+
+@pytest.mark.parametrize("ipl", [1, 2, 4])
+def test_regf_plummer_1k_all_i(gpunb, ipl):
+    m, x, v, h2, dtr = c1_inputs()
+    n = m.size
+    lmax, nnbmax = LMAX_SMALL, nnbmax_for(n, LMAX_SMALL)
+    lib.load().nb6_set_regf_config(ipl, 2048)
+    g = gpunb(n)
+    g.send(m, x, v)
+    got = g.regf(h2, dtr, x, v, lmax, nnbmax)
+    ref = O.regf(m, x, v, h2, dtr, x, v, lmax, nnbmax)
+    lib.load().nb6_set_regf_config(1, 2048)
+    _compare(got, ref, f"C1 ipl={ipl}")
+    # the self pair is always a neighbour (r2 = 0 < h2)
+    for i in range(n):
+        c = got[3][i, 0]
+        assert i in set(got[3][i, 1:1 + c])
+
+
+@pytest.mark.parametrize("ni", [1, 256, 4096])
+@pytest.mark.parametrize("m_flag", [0, 1])
+def test_regf_kroupa_64k_subsets(gpunb, ni, m_flag):
+    m, x, v, idx, h2, dtr = c2_inputs(ni=ni, m_flag=m_flag)
+    n = m.size
+    lmax, nnbmax = LMAX_SMALL + 100, nnbmax_for(n, LMAX_SMALL + 100)
+    g = gpunb(n)
+    g.send(m, x, v)
+    got = g.regf(h2, dtr, x[idx], v[idx], lmax, nnbmax, m_flag)
+    ref = O.regf(m, x, v, h2, dtr, x[idx], v[idx], lmax, nnbmax, m_flag)
+    _compare(got, ref, f"C2 ni={ni} m_flag={m_flag}")
+
+
+def test_regf_overflow_exact_negative_counts(gpunb):
+    """Counts above nnbmax come back as -count with the exact count
+    (reg.avx.cpp:320-322); the caller's RS rescale (util_gpu.F:82-86) needs it."""
+    m, x, v, idx, h2, dtr = c2_inputs(n=16384, ni=512, counts_median=60.0)
+    n = m.size
+    lmax, nnbmax = 120, 64
+    g = gpunb(n)
+    g.send(m, x, v)
+    got = g.regf(h2, dtr, x[idx], v[idx], lmax, nnbmax)
+    ref = O.regf(m, x, v, h2, dtr, x[idx], v[idx], lmax, nnbmax)
+    assert (got[3][:, 0] < 0).any() and (got[3][:, 0] > 0).any()
+    assert (got[3][:, 0] == ref[3][:, 0]).all()
+    ok = got[3][:, 0] >= 0
+    assert rows_as_sets(got[3][ok]) == rows_as_sets(ref[3][ok])
+
+
+def test_regf_split_j_many_chunks(gpunb):
+    """A single i against all j split over many chunks keeps lists ascending."""
+    m, x, v, idx, h2, dtr = c2_inputs(ni=3)
+    n = m.size
+    lmax, nnbmax = 600, 550
+    g = gpunb(n)
+    g.send(m, x, v)
+    got = g.regf(h2, dtr, x[idx], v[idx], lmax, nnbmax)
+    assert lib.load().nb6_last_regf_chunks() > 8
+    ref = O.regf(m, x, v, h2, dtr, x[idx], v[idx], lmax, nnbmax)
+    _compare(got, ref, "split-j")
+    for row in got[3]:
+        c = row[0]
+        assert (np.diff(row[1:1 + c]) > 0).all()
+
+
+def test_regf_ragged_j_and_resend(gpunb):
+    """nj not a multiple of the tile; a second send with fewer particles."""
+    m, x, v, h2, dtr = c1_inputs(n=1000)
+    g = gpunb(1000)
+    for nj in (1000, 777):
+        g.send(m[:nj], x[:nj], v[:nj])
+        got = g.regf(h2[:50], dtr[:50], x[:50], v[:50], 400, 350)
+        ref = O.regf(m[:nj], x[:nj], v[:nj], h2[:50], dtr[:50], x[:50], v[:50], 400, 350)
+        _compare(got, ref, f"ragged nj={nj}")
+
+
+def test_regf_device_entry_point_matches_host_abi(gpunb):
+    """nb6_regf_device (inputs resident in HBM) gives the same bits as gpunb_regf_."""
+    m, x, v, idx, h2, dtr = c2_inputs(n=16384, ni=1000)
+    n = m.size
+    lmax, nnbmax = 500, 450
+    g = gpunb(n)
+    g.send(m, x, v)
+    host = g.regf(h2, dtr, x[idx], v[idx], lmax, nnbmax)
+    ni = idx.size
+    ib = np.concatenate([x[idx].T, v[idx].T, h2[None], dtr[None]]).astype(np.float64)
+    di = lib.DeviceArray.from_host(ib)
+    do = lib.DeviceArray(7 * ni * 8)
+    dl = lib.DeviceArray(ni * lmax * 4)
+    rc = lib.load().nb6_regf_device(ni, ni, di.ptr, lmax, nnbmax, 0, do.ptr, dl.ptr, None)
+    assert rc == 0
+    out = do.to_host(np.float64, (7, ni))
+    lst = dl.to_host(np.int32, (ni, lmax))
+    for d in (di, do, dl):
+        d.free()
+    assert np.array_equal(out[0:3].T, host[0])
+    assert np.array_equal(out[3:6].T, host[1])
+    assert np.array_equal(out[6], host[2])
+    assert rows_as_sets(lst) == rows_as_sets(host[3])
