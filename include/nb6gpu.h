@@ -139,6 +139,10 @@ int nb6_set_regf_config(int i_per_lane, int min_groups);
 double nb6_kernel_ms(const char *name);
 int    nb6_last_regf_chunks(void);
 
+/* Streaming device-to-device copy of bytes (a multiple of 16) by a plain copy
+ * kernel; used to calibrate HBM byte counters against a known size. */
+int nb6_device_copy(void *dst, const void *src, unsigned long long bytes, void *stream);
+
 /* Device allocation / copies for callers without another GPU runtime
  * (tests, benchmarks). */
 void *nb6_dev_alloc(unsigned long long bytes);

@@ -475,6 +475,15 @@ int nb6_set_regf_config(int i_per_lane, int min_groups) {
 }
 
 double nb6_kernel_ms(const char *name) { return ktimer(name).ms(); }
+
+int nb6_device_copy(void *dst, const void *src, unsigned long long bytes, void *stream) {
+    Device &d = device();
+    if (!d.ready) return E_NODEV;
+    if (bytes % 16) { set_error(E_ARG, "nb6_device_copy: size must be a multiple of 16"); return E_ARG; }
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : d.stream;
+    device_copy(dst, src, bytes, s);
+    return OK;
+}
 int nb6_last_regf_chunks(void) { return regf_last_chunks(); }
 
 void *nb6_dev_alloc(unsigned long long bytes) {

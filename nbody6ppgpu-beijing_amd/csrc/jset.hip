@@ -113,7 +113,23 @@ __global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ a
     if (threadIdx.x < 7) out[threadIdx.x] = sh[threadIdx.x][0];
 }
 
+// plain streaming copy (used to calibrate HBM byte counters against a known size)
+__global__ __launch_bounds__(256) void copy_kernel(const uint4 *__restrict__ src,
+                                                   uint4 *__restrict__ dst, size_t n16) {
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n16; k += (size_t)gridDim.x * 256)
+        dst[k] = src[k];
+}
+
 }  // namespace
+
+void device_copy(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    const size_t n16 = bytes / 16;
+    if (n16 == 0) return;
+    const size_t blocks = std::min<size_t>((n16 + 255) / 256, 8192);
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, s,
+                       static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n16);
+    NB6_HIP(hipGetLastError());
+}
 
 void jset_reserve(JSet &js, int cap) {
     if (cap <= js.cap) return;

@@ -99,6 +99,9 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
 int regf_last_chunks();
 void regf_release();
 
+// streaming copy of bytes (multiple of 16) on stream s
+void device_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
+
 // ---------------------------------------------------------------- pot
 int pot_launch(int n, int n_stride, const double *d_x, int i0, int ni, double *d_pot,
                hipStream_t s);

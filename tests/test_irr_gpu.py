@@ -77,3 +77,23 @@ def test_firr_update_one_particle_and_list():
     ea = np.linalg.norm(acc - racc, axis=1) / np.linalg.norm(racc, axis=1)
     assert ea.max() <= REL_TOL
     assert (nn == rnn).all()
+
+
+def test_firr_golden_inputs():
+    """Reference AVX irregular force on its golden inputs: our error against the
+    FP64 oracle is within REL_TOL and nearest neighbours agree with FP64."""
+    from pathlib import Path
+    g = dict(np.load(Path(__file__).resolve().parent / "golden" / "irr_plummer1k.npz"))
+    n = g["m"].size
+    irr = lib.IrrSimd(n, 400)
+    for i in range(n):
+        irr.set_jp(i + 1, g["x"][i], g["v"][i], g["a2"][i], g["j6"][i], g["m"][i], g["t0"][i])
+        irr.set_list(i + 1, g["lists"][i])
+    acc, jrk, nn = irr.firr_vec(float(g["ti"]), g["addr"])
+    irr.close()
+    racc, rjrk, rnn = O.firr(float(g["ti"]), g["addr"], g["x"], g["v"], g["a2"], g["j6"], g["m"],
+                             g["t0"], g["lists"], 400)
+    ea = np.linalg.norm(acc - racc, axis=1) / np.linalg.norm(racc, axis=1)
+    e_avx = np.linalg.norm(g["acc"] - racc, axis=1) / np.linalg.norm(racc, axis=1)
+    assert ea.max() <= REL_TOL and ea.max() <= e_avx.max()
+    assert (nn == rnn).all()

@@ -37,3 +37,15 @@ def test_pot_coincident_particles_skipped():
     x = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 0.0], [1.0, 0.0, 0.0]])
     got = lib.gpupot(m, x)
     assert np.allclose(got, [0.5, 0.5, 0.5], rtol=1e-7)
+
+
+def test_pot_golden_binaries_beats_reference():
+    """Reference AVX potential on a binary-rich cluster (tests/golden): our error
+    against FP64 is below the reference's own."""
+    from pathlib import Path
+    g = dict(np.load(Path(__file__).resolve().parent / "golden" / "pot_binaries4k.npz"))
+    got = lib.gpupot(g["m"], g["x"])
+    ref = O.pot(g["m"], g["x"])
+    e_ours = np.abs(got - ref) / ref
+    e_avx = np.abs(g["pot"] - ref) / ref
+    assert e_ours.max() <= REL_TOL and e_ours.max() < e_avx.max()
