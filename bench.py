@@ -113,12 +113,119 @@ def accuracy(m, x, v, idx, h2, dtr, out, lst, k=256):
             "pot_max": float(ep.max()), "lists_identical": bool(same)}
 
 
+def bench_ensemble(steps, warmup):
+    """Batched ensemble (256 Plummer clusters, N log-uniform in 1024-32768, Kroupa
+    masses): one regular step of every cluster in one persistent hipGraph-replayed
+    launch, against looping the plain gpunb path (send + regf) over the clusters."""
+    import ctypes
+    rng = np.random.Generator(np.random.PCG64(SEED + 40))
+    nc = 256
+    sizes = np.exp(rng.uniform(np.log(1024), np.log(32768), nc)).astype(int)
+    t = time.time()
+    ms, xs, vs, h2s, dts = [], [], [], [], []
+    for c, n in enumerate(sizes):
+        m, x, v = ics.plummer(int(n), SEED + 1000 + c, masses=ics.kroupa_masses(int(n), SEED + 2000 + c))
+        rs = ics.rs_for_counts(x, np.arange(n), np.full(n, 50))
+        r2 = (x * x).sum(1)
+        ms.append(m); xs.append(x); vs.append(v); h2s.append(rs * rs)
+        dts.append(np.minimum(0.125 * np.sqrt(1 + r2), 0.125) * 0.05)
+    log(f"ensemble built in {time.time() - t:.1f} s, {sizes.sum()} particles")
+    off = np.r_[0, np.cumsum(sizes)].astype(np.int32)
+    ntot = int(off[-1])
+    lmax, nnbmax = 400, 350
+    m, x, v = np.concatenate(ms), np.concatenate(xs), np.concatenate(vs)
+    h2, dtr = np.concatenate(h2s), np.concatenate(dts)
+    L = lib.load()
+    dj = lib.DeviceArray.from_host(np.ascontiguousarray(np.concatenate([x.T, v.T, m[None]])))
+    di = lib.DeviceArray.from_host(np.ascontiguousarray(np.concatenate([x.T, v.T, h2[None], dtr[None]])))
+    do = lib.DeviceArray(7 * ntot * 8)
+    dl = lib.DeviceArray(ntot * lmax * 4)
+    pairs = float((sizes.astype(float) ** 2).sum())
+
+    # batched: send + one graph-replayed regular step per ensemble step
+    L.nb6_ens_open(nc, ntot)
+    L.nb6_ens_items.restype = ctypes.c_int
+    offp = off.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+    def ens_step():
+        if L.nb6_ens_send_device(nc, offp, dj.ptr, ntot, None) != 0:
+            lib.check_error()
+        if L.nb6_ens_regf_device(di.ptr, ntot, lmax, nnbmax, 0, do.ptr, dl.ptr, 1, None) != 0:
+            lib.check_error()
+    for _ in range(warmup):
+        ens_step()
+    L.nb6_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ens_step()
+    L.nb6_sync()
+    t_ens = (time.perf_counter() - t0) / steps
+    # regular step alone (graph replays, send excluded)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        L.nb6_ens_regf_device(di.ptr, ntot, lmax, nnbmax, 0, do.ptr, dl.ptr, 1, None)
+    L.nb6_sync()
+    t_ens_regf = (time.perf_counter() - t0) / steps
+    out_b = do.to_host(np.float64, (7, ntot))
+    lst_b = dl.to_host(np.int32, (ntot, lmax))
+    L.nb6_ens_close()
+
+    # looped: the plain path, one cluster after the other
+    g = lib.Gpunb(int(sizes.max()))
+
+    def loop_step():
+        for c in range(nc):
+            a, n = int(off[c]), int(sizes[c])
+            if L.nb6_send_device(n, ntot, dj.ptr + 8 * a, None) != 0:
+                lib.check_error()
+            if L.nb6_regf_device(n, ntot, di.ptr + 8 * a, lmax, nnbmax, 0, do.ptr + 8 * a,
+                                 dl.ptr + 4 * lmax * a, None) != 0:
+                lib.check_error()
+    for _ in range(warmup):
+        loop_step()
+    L.nb6_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loop_step()
+    L.nb6_sync()
+    t_loop = (time.perf_counter() - t0) / steps
+    out_l = do.to_host(np.float64, (7, ntot))
+    lst_l = dl.to_host(np.int32, (ntot, lmax))
+    g.close()
+    for d in (dj, di, do, dl):
+        d.free()
+    # the two paths sum the FP32 partials in different orders: compare per particle
+    da = np.linalg.norm(out_b[0:3] - out_l[0:3], axis=0) / np.linalg.norm(out_l[0:3], axis=0)
+    dp = np.abs(out_b[6] - out_l[6]) / np.abs(out_l[6])
+    same_lists = bool(np.array_equal(lst_b[:, 0], lst_l[:, 0]))
+    for i in range(0, ntot, 997):
+        c = lst_l[i, 0]
+        if c > 0 and not np.array_equal(lst_b[i, 1:1 + c], lst_l[i, 1:1 + c]):
+            same_lists = False
+    return {"metric": "ensemble clusters/s (256 Plummer clusters, one regular step each)",
+            "value": nc / t_ens, "unit": "clusters/s", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": 1e3 * t_ens, "higher_is_better": True,
+            "dtype": "fp32", "data": "synthetic (seeded Plummer clusters, Kroupa masses)",
+            "config": {"workload": "ensemble", "clusters": nc, "particles": ntot,
+                       "pairs_per_step": pairs, "lmax": lmax, "nnbmax": nnbmax},
+            "batched": {"clusters_per_s": nc / t_ens, "ms_per_step": 1e3 * t_ens,
+                        "regf_only_ms": 1e3 * t_ens_regf, "pairs_per_s": pairs / t_ens,
+                        "work_items": L.nb6_ens_items()},
+            "looped": {"clusters_per_s": nc / t_loop, "ms_per_step": 1e3 * t_loop,
+                       "pairs_per_s": pairs / t_loop},
+            "speedup": t_loop / t_ens,
+            "batched_vs_looped": {"acc_max_rel": float(da.max()), "pot_max_rel": float(dp.max()),
+                                  "lists_identical": same_lists}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--workload", default="regf", choices=["regf", "ensemble"],
+                    help="regf: the headline N=1M step; ensemble: 256-cluster batch vs loop")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one plain copy kernel of MIB MiB (byte-counter calibration)")
     args = ap.parse_args()
@@ -135,6 +242,10 @@ def main():
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
         return 2
+
+    if args.workload == "ensemble":
+        print(json.dumps(bench_ensemble(args.steps, args.warmup)), flush=True)
+        return 0
 
     m, x, v, idx, h2, dtr = make_workload()
     L = lib.load()

@@ -129,15 +129,36 @@ int nb6_send_device(int nj, int nj_stride, const double *d_j, void *stream);
 int nb6_pot_device(int n, int n_stride, const double *d_x, int i0, int ni,
                    double *d_pot, void *stream);
 
-/* Tile configuration of the regular-force kernel: i-particles per lane (1, 2 or
- * 4) and the minimum number of work-groups a call is split into. */
-int nb6_set_regf_config(int i_per_lane, int min_groups);
+/* Tile configuration of the regular-force kernel: work-group size = LDS j-tile
+ * (128, 256 or 512), i-particles per lane (1 or 2), and the minimum number of
+ * work-groups a call is split into (j-chunks are added until it is reached). */
+int nb6_set_regf_config(int threads, int i_per_lane, int min_groups);
 
 /* Average device time (ms) of the last launch of the named kernel family
  * ("regf_tile", "regf_finalize", "pot", "irr") measured with HIP events, and
  * the number of j-chunks the last regf call used. */
 double nb6_kernel_ms(const char *name);
 int    nb6_last_regf_chunks(void);
+
+/* Ensemble of independent clusters in one device j-set (no counterpart in the
+ * reference, which holds one j-set per library instance).
+ *   nb6_ens_send_device: offsets[nclusters+1] (host) index the concatenated SoA
+ *     double[7][stride] (x,y,z,vx,vy,vz,m) on the device; cluster c owns
+ *     [offsets[c], offsets[c+1]).
+ *   nb6_ens_regf_device: one regular step of every particle of every cluster
+ *     against its own cluster: d_i SoA double[8][stride] (x,y,z,vx,vy,vz,h2,dtr)
+ *     for all particles, d_out SoA double[7][stride], d_list int[ntot][lmax] with
+ *     0-based indices relative to the particle's own cluster.  One persistent
+ *     launch over (cluster, i-block) work items; use_graph=1 captures the step
+ *     into a hipGraph on the first call and replays it while the pointers and
+ *     sizes stay the same.  Results equal gpunb_send_ + gpunb_regf_ per cluster. */
+int nb6_ens_open(int max_clusters, int max_particles);
+int nb6_ens_close(void);
+int nb6_ens_send_device(int nclusters, const int *offsets, const double *d_soa, int stride,
+                        void *stream);
+int nb6_ens_regf_device(const double *d_i, int stride, int lmax, int nnbmax, int m_flag,
+                        double *d_out, int *d_list, int use_graph, void *stream);
+int nb6_ens_items(void);
 
 /* Streaming device-to-device copy of bytes (a multiple of 16) by a plain copy
  * kernel; used to calibrate HBM byte counters against a known size. */

@@ -29,6 +29,14 @@ int irr_firr_vec(double ti, int ni, const int *addr, double (*acc)[3], double (*
                  int *nnbid);
 int irr_profile(int rank);
 
+// ensemble.hip
+int ens_open(int max_clusters, int max_particles);
+int ens_close();
+int ens_send_device(int nc, const int *offsets, const double *d_soa, int stride, hipStream_t s);
+int ens_regf_device(const double *d_i, int stride, int lmax, int nnbmax, int m_flag, double *d_out,
+                    int *d_list, int use_graph, hipStream_t s);
+int ens_items();
+
 // ------------------------------------------------------------------ errors
 static int g_err = 0;
 static std::string g_msg;
@@ -460,21 +468,44 @@ int nb6_pot_device(int n, int n_stride, const double *d_x, int i0, int ni, doubl
     return pot_launch(n, n_stride, d_x, i0, ni, d_pot, s);
 }
 
-int nb6_set_regf_config(int i_per_lane, int min_groups) {
-    if (i_per_lane != 1 && i_per_lane != 2 && i_per_lane != 4) {
-        set_error(E_ARG, "nb6_set_regf_config: i_per_lane must be 1, 2 or 4");
+int nb6_set_regf_config(int threads, int i_per_lane, int min_groups) {
+    if (threads != 128 && threads != 256 && threads != 512) {
+        set_error(E_ARG, "nb6_set_regf_config: threads must be 128, 256 or 512");
+        return E_ARG;
+    }
+    if (i_per_lane != 1 && i_per_lane != 2) {
+        set_error(E_ARG, "nb6_set_regf_config: i_per_lane must be 1 or 2");
         return E_ARG;
     }
     if (min_groups < 1) {
         set_error(E_ARG, "nb6_set_regf_config: min_groups must be positive");
         return E_ARG;
     }
+    regf_config().threads = threads;
     regf_config().ipl = i_per_lane;
     regf_config().min_groups = min_groups;
     return OK;
 }
 
 double nb6_kernel_ms(const char *name) { return ktimer(name).ms(); }
+
+int nb6_ens_open(int max_clusters, int max_particles) { return ens_open(max_clusters, max_particles); }
+int nb6_ens_close(void) { return ens_close(); }
+int nb6_ens_send_device(int nclusters, const int *offsets, const double *d_soa, int stride,
+                        void *stream) {
+    Device &d = device();
+    if (!d.ready) return E_NODEV;
+    return ens_send_device(nclusters, offsets, d_soa, stride,
+                           stream ? static_cast<hipStream_t>(stream) : d.stream);
+}
+int nb6_ens_regf_device(const double *d_i, int stride, int lmax, int nnbmax, int m_flag,
+                        double *d_out, int *d_list, int use_graph, void *stream) {
+    Device &d = device();
+    if (!d.ready) return E_NODEV;
+    return ens_regf_device(d_i, stride, lmax, nnbmax, m_flag, d_out, d_list, use_graph,
+                           stream ? static_cast<hipStream_t>(stream) : d.stream);
+}
+int nb6_ens_items(void) { return ens_items(); }
 
 int nb6_device_copy(void *dst, const void *src, unsigned long long bytes, void *stream) {
     Device &d = device();

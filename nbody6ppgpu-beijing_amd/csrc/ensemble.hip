@@ -1,0 +1,377 @@
+// Batched regular force for an ensemble of independent clusters.
+//
+// The reference has one j-set per library instance (gpunb_open_/gpunb_send_,
+// gpunb.velocity.cu:629-730), so many small clusters mean many small launches.
+// Here all clusters live in one device j-set:
+//   * send: one segmented Morton sort of all particles (key = cluster << 32 |
+//     Morton code inside the cluster's own bounding box); each cluster's sorted
+//     range is padded to whole 256-j tiles with far, massless dummies so no tile
+//     spans two clusters, and every tile gets its FP64 origin (as in jset.hip);
+//   * regular step: one persistent launch of regf_items_kernel walks a queue of
+//     (cluster, i-block) work items ordered longest first, then one finalize over
+//     all particles (per-i cluster base for the output indices).  The step (queue
+//     reset, persistent kernel, finalize) is captured once into a hipGraph and
+//     replayed while the buffers stay the same.
+// Each cluster's results are exactly those of a gpunb_send_ + gpunb_regf_ of
+// that cluster alone (same lists, same FP64 re-test).
+#include "nb6_internal.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cfloat>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+namespace nb6 {
+
+namespace {
+
+__device__ __forceinline__ unsigned spread10e(unsigned a) {
+    a &= 0x3ffu;
+    a = (a | (a << 16)) & 0x030000FFu;
+    a = (a | (a << 8)) & 0x0300F00Fu;
+    a = (a | (a << 4)) & 0x030C30C3u;
+    a = (a | (a << 2)) & 0x09249249u;
+    return a;
+}
+
+// per-cluster bounding box and maximum mass: one work-group per cluster
+__global__ __launch_bounds__(256) void ens_bbox_kernel(const double *__restrict__ aos,
+                                                       const int *__restrict__ off,
+                                                       double *__restrict__ box) {
+    __shared__ double sh[7][256];
+    const int c = blockIdx.x;
+    double v[7] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX, 0.0};
+    for (int j = off[c] + threadIdx.x; j < off[c + 1]; j += 256) {
+        const double *r = aos + (size_t)j * 8;
+        for (int d = 0; d < 3; d++) {
+            v[d] = fmin(v[d], r[d]);
+            v[3 + d] = fmax(v[3 + d], r[d]);
+        }
+        v[6] = fmax(v[6], r[3]);
+    }
+    for (int q = 0; q < 7; q++) sh[q][threadIdx.x] = v[q];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            for (int d = 0; d < 3; d++) sh[d][threadIdx.x] = fmin(sh[d][threadIdx.x], sh[d][threadIdx.x + o]);
+            for (int d = 3; d < 7; d++) sh[d][threadIdx.x] = fmax(sh[d][threadIdx.x], sh[d][threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 7) box[c * 7 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void ens_key_kernel(const double *__restrict__ aos, int n,
+                                                      const int *__restrict__ cid,
+                                                      const double *__restrict__ box,
+                                                      unsigned long long *__restrict__ key,
+                                                      int *__restrict__ val) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const int c = cid[j];
+    const double *r = aos + (size_t)j * 8;
+    const double *b = box + c * 7;
+    unsigned q[3];
+    for (int d = 0; d < 3; d++) {
+        const double w = b[3 + d] - b[d];
+        const double sc = w > 0.0 ? 1024.0 / (w * (1.0 + 1e-12)) : 0.0;
+        q[d] = (unsigned)min(1023, max(0, (int)((r[d] - b[d]) * sc)));
+    }
+    key[j] = ((unsigned long long)c << 32) |
+             (spread10e(q[0]) | (spread10e(q[1]) << 1) | (spread10e(q[2]) << 2));
+    val[j] = j;
+}
+
+// fill every padded slot with a far massless dummy (overwritten for real particles)
+__global__ __launch_bounds__(256) void ens_dummy_kernel(double *__restrict__ rec64,
+                                                        int *__restrict__ perm, int npad) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= npad) return;
+    double *o = rec64 + (size_t)s * 8;
+    o[0] = 1.0e17; o[1] = 0.0; o[2] = 0.0; o[3] = 0.0;
+    o[4] = 0.0; o[5] = 0.0; o[6] = 0.0; o[7] = 0.0;
+    perm[s] = -1;
+}
+
+// compact sorted position s -> padded position (cluster start + rank in cluster)
+__global__ __launch_bounds__(256) void ens_scatter_kernel(const double *__restrict__ in, int n,
+                                                          const int *__restrict__ sorted,
+                                                          const int *__restrict__ cid,
+                                                          const int *__restrict__ off,
+                                                          const int *__restrict__ padoff,
+                                                          double *__restrict__ rec64,
+                                                          int *__restrict__ perm) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= n) return;
+    const int j = sorted[s];
+    const int c = cid[j];
+    const int ps = padoff[c] + (s - off[c]);   // clusters keep their order in the sort
+    const double *r = in + (size_t)j * 8;
+    double *o = rec64 + (size_t)ps * 8;
+#pragma unroll
+    for (int f = 0; f < 8; f++) o[f] = r[f];
+    perm[ps] = j;
+}
+
+// FP32 tile-relative records and tile origins on the padded layout
+__global__ __launch_bounds__(256) void ens_build_kernel(const double *__restrict__ rec64,
+                                                        int npad, float4 *__restrict__ rec32,
+                                                        double *__restrict__ torg) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= npad) return;
+    const int t = s / JTILE;
+    const double *p = rec64 + (size_t)t * JTILE * 8;
+    const double *r = rec64 + (size_t)s * 8;
+    rec32[2 * (size_t)s] = make_float4((float)(r[0] - p[0]), (float)(r[1] - p[1]),
+                                       (float)(r[2] - p[2]), (float)r[3]);
+    rec32[2 * (size_t)s + 1] = make_float4((float)(r[4] - p[4]), (float)(r[5] - p[5]),
+                                           (float)(r[6] - p[6]), 0.f);
+    if (s == t * JTILE) {
+        double *q = torg + (size_t)t * 8;
+        q[0] = p[0]; q[1] = p[1]; q[2] = p[2];
+        q[3] = p[4]; q[4] = p[5]; q[5] = p[6];
+        q[6] = 0.0; q[7] = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void ens_soa_to_aos(const double *__restrict__ soa, int n,
+                                                      int stride, double *__restrict__ aos) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    double *o = aos + (size_t)j * 8;
+    o[0] = soa[0 * (size_t)stride + j];
+    o[1] = soa[1 * (size_t)stride + j];
+    o[2] = soa[2 * (size_t)stride + j];
+    o[3] = soa[6 * (size_t)stride + j];
+    o[4] = soa[3 * (size_t)stride + j];
+    o[5] = soa[4 * (size_t)stride + j];
+    o[6] = soa[5 * (size_t)stride + j];
+    o[7] = 0.0;
+}
+
+__global__ void ens_zero_kernel(int *q) { *q = 0; }
+
+struct GraphKey {
+    const void *ptr[4] = {nullptr, nullptr, nullptr, nullptr};   // d_i, d_out, d_list, items
+    long long val[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    bool operator==(const GraphKey &o) const {
+        return std::equal(ptr, ptr + 4, o.ptr) && std::equal(val, val + 11, o.val);
+    }
+};
+
+struct Ens {
+    bool open = false;
+    int max_clusters = 0, max_particles = 0;
+    int nc = 0, ntot = 0, npad = 0;
+    std::vector<int> off, padoff;
+    JSet js;                         // padded, sorted j-set of all clusters
+    DBuf d_off, d_padoff, d_cid, d_box, d_keys, d_vals, d_skeys, d_svals, d_tmp, d_jbase;
+    DBuf d_items, d_queue;
+    RegfWork work;                   // own scratch: a captured graph holds its addresses
+    int nitems = 0;
+    // graph of the last regular step and everything it captured by address or value
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    GraphKey gkey;
+} g_ens;
+
+void drop_graph() {
+    if (g_ens.exec) (void)hipGraphExecDestroy(g_ens.exec);
+    if (g_ens.graph) (void)hipGraphDestroy(g_ens.graph);
+    g_ens.exec = nullptr;
+    g_ens.graph = nullptr;
+}
+
+}  // namespace
+
+int ens_open(int max_clusters, int max_particles) {
+    Ens &e = g_ens;
+    if (e.open) {
+        fprintf(stderr, "nb6 ensemble: it is already open\n");
+        return OK;
+    }
+    if (max_clusters <= 0 || max_particles <= 0) {
+        set_error(E_ARG, "nb6_ens_open: sizes must be positive");
+        return E_ARG;
+    }
+    Device &d = device();
+    if (!d.ready) return E_NODEV;
+    e.max_clusters = max_clusters;
+    e.max_particles = max_particles;
+    const int padmax = max_particles + max_clusters * JTILE;
+    jset_reserve(e.js, padmax);
+    e.d_off.get((size_t)(max_clusters + 1) * sizeof(int));
+    e.d_padoff.get((size_t)(max_clusters + 1) * sizeof(int));
+    e.d_cid.get((size_t)max_particles * sizeof(int));
+    e.d_box.get((size_t)max_clusters * 7 * sizeof(double));
+    e.d_keys.get((size_t)max_particles * sizeof(unsigned long long));
+    e.d_skeys.get((size_t)max_particles * sizeof(unsigned long long));
+    e.d_vals.get((size_t)max_particles * sizeof(int));
+    e.d_svals.get((size_t)max_particles * sizeof(int));
+    e.d_jbase.get((size_t)max_particles * sizeof(int));
+    e.d_queue.get(sizeof(int));
+    e.open = true;
+    return OK;
+}
+
+int ens_close() {
+    Ens &e = g_ens;
+    if (!e.open) return OK;
+    drop_graph();
+    jset_release(e.js);
+    e.d_off.release(); e.d_padoff.release(); e.d_cid.release(); e.d_box.release();
+    e.d_keys.release(); e.d_skeys.release(); e.d_vals.release(); e.d_svals.release();
+    e.d_tmp.release(); e.d_jbase.release(); e.d_items.release(); e.d_queue.release();
+    e.work.release();
+    e.nc = 0;
+    e.open = false;
+    return OK;
+}
+
+// offsets[nc+1] (host) into the concatenated SoA [7][stride] device array
+int ens_send_device(int nc, const int *offsets, const double *d_soa, int stride, hipStream_t s) {
+    Ens &e = g_ens;
+    if (!e.open) { set_error(E_STATE, "nb6_ens_send_device: ensemble not open"); return E_STATE; }
+    if (nc <= 0 || nc > e.max_clusters || offsets[0] != 0 || offsets[nc] > e.max_particles ||
+        stride < offsets[nc]) {
+        set_error(E_ARG, "nb6_ens_send_device: bad cluster count, offsets or stride");
+        return E_ARG;
+    }
+    for (int c = 0; c < nc; c++)
+        if (offsets[c + 1] <= offsets[c]) {
+            set_error(E_ARG, "nb6_ens_send_device: every cluster needs at least one particle");
+            return E_ARG;
+        }
+    e.nc = nc;
+    e.ntot = offsets[nc];
+    e.off.assign(offsets, offsets + nc + 1);
+    e.padoff.assign(nc + 1, 0);
+    for (int c = 0; c < nc; c++) {
+        const int n = offsets[c + 1] - offsets[c];
+        e.padoff[c + 1] = e.padoff[c] + (n + JTILE - 1) / JTILE * JTILE;
+    }
+    e.npad = e.padoff[nc];
+    std::vector<int> cid(e.ntot), jb(e.ntot);
+    for (int c = 0; c < nc; c++)
+        for (int j = offsets[c]; j < offsets[c + 1]; j++) {
+            cid[j] = c;
+            jb[j] = offsets[c];
+        }
+    NB6_HIP(hipMemcpyAsync(e.d_off.p, e.off.data(), (nc + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+    NB6_HIP(hipMemcpyAsync(e.d_padoff.p, e.padoff.data(), (nc + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+    NB6_HIP(hipMemcpyAsync(e.d_cid.p, cid.data(), e.ntot * sizeof(int), hipMemcpyHostToDevice, s));
+    NB6_HIP(hipMemcpyAsync(e.d_jbase.p, jb.data(), e.ntot * sizeof(int), hipMemcpyHostToDevice, s));
+    const int n = e.ntot;
+    double *in64 = e.js.in64.as<double>();
+    hipLaunchKernelGGL(ens_soa_to_aos, dim3((n + 255) / 256), dim3(256), 0, s, d_soa, n, stride, in64);
+    hipLaunchKernelGGL(ens_bbox_kernel, dim3(nc), dim3(256), 0, s, in64, e.d_off.as<int>(),
+                       e.d_box.as<double>());
+    hipLaunchKernelGGL(ens_key_kernel, dim3((n + 255) / 256), dim3(256), 0, s, in64, n,
+                       e.d_cid.as<int>(), e.d_box.as<double>(),
+                       e.d_keys.as<unsigned long long>(), e.d_vals.as<int>());
+    NB6_HIP(hipGetLastError());
+    int hi = 32;
+    while ((1 << (hi - 32)) <= nc && hi < 63) hi++;
+    size_t tb = 0;
+    NB6_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, e.d_keys.as<unsigned long long>(),
+                                               e.d_skeys.as<unsigned long long>(), e.d_vals.as<int>(),
+                                               e.d_svals.as<int>(), n, 0, hi, s));
+    void *tmp = e.d_tmp.get(tb);
+    NB6_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, e.d_keys.as<unsigned long long>(),
+                                               e.d_skeys.as<unsigned long long>(), e.d_vals.as<int>(),
+                                               e.d_svals.as<int>(), n, 0, hi, s));
+    const int np = e.npad;
+    hipLaunchKernelGGL(ens_dummy_kernel, dim3((np + 255) / 256), dim3(256), 0, s,
+                       e.js.rec64.as<double>(), e.js.perm.as<int>(), np);
+    hipLaunchKernelGGL(ens_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, s, in64, n,
+                       e.d_svals.as<int>(), e.d_cid.as<int>(), e.d_off.as<int>(),
+                       e.d_padoff.as<int>(), e.js.rec64.as<double>(), e.js.perm.as<int>());
+    hipLaunchKernelGGL(ens_build_kernel, dim3((np + 255) / 256), dim3(256), 0, s,
+                       e.js.rec64.as<double>(), np, e.js.rec32.as<float4>(), e.js.torg.as<double>());
+    NB6_HIP(hipGetLastError());
+    e.js.n = np;
+    e.js.ntile = np / JTILE;
+    // maximum mass, for the m_flag error band
+    std::vector<double> box((size_t)nc * 7);
+    NB6_HIP(hipMemcpyAsync(box.data(), e.d_box.p, box.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    NB6_HIP(hipStreamSynchronize(s));
+    double mmax = 0.0;
+    for (int c = 0; c < nc; c++) mmax = std::max(mmax, box[(size_t)c * 7 + 6]);
+    e.js.mmax = mmax;
+    // work items: (cluster, i-block), longest first
+    const RegfConfig &cfg = regf_config();
+    const int bi = cfg.threads * cfg.ipl;
+    std::vector<int4> items;
+    std::vector<long long> cost;
+    for (int c = 0; c < nc; c++) {
+        const int nci = offsets[c + 1] - offsets[c];
+        const int plen = e.padoff[c + 1] - e.padoff[c];
+        for (int ib = offsets[c]; ib < offsets[c + 1]; ib += bi) {
+            items.push_back(make_int4(ib, offsets[c + 1], e.padoff[c], e.padoff[c + 1]));
+            cost.push_back((long long)std::min(bi, offsets[c + 1] - ib) * plen);
+        }
+        (void)nci;
+    }
+    std::vector<int> order(items.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<int4> sorted(items.size());
+    for (size_t k = 0; k < items.size(); k++) sorted[k] = items[order[k]];
+    e.nitems = (int)sorted.size();
+    e.d_items.get(sorted.size() * sizeof(int4));
+    NB6_HIP(hipMemcpyAsync(e.d_items.p, sorted.data(), sorted.size() * sizeof(int4),
+                           hipMemcpyHostToDevice, s));
+    NB6_HIP(hipStreamSynchronize(s));
+    return OK;
+}
+
+int ens_regf_device(const double *d_i, int stride, int lmax, int nnbmax, int m_flag, double *d_out,
+                    int *d_list, int use_graph, hipStream_t s) {
+    Ens &e = g_ens;
+    if (!e.open || e.nc == 0) { set_error(E_STATE, "nb6_ens_regf_device: nothing sent"); return E_STATE; }
+    if (stride < e.ntot) { set_error(E_ARG, "nb6_ens_regf_device: stride below the particle count"); return E_ARG; }
+    Device &d = device();
+    const RegfConfig &cfg = regf_config();
+    const int per_cu = std::max(1, (8 * 256) / cfg.threads);
+    const int grid = std::min(e.nitems, d.num_cu * per_cu);
+    auto step = [&](hipStream_t st) {
+        hipLaunchKernelGGL(ens_zero_kernel, dim3(1), dim3(1), 0, st, e.d_queue.as<int>());
+        return regf_items_launch(e.js, e.ntot, stride, d_i, e.d_items.as<int4>(), e.nitems,
+                                 e.d_queue.as<int>(), e.d_jbase.as<int>(), lmax, nnbmax, m_flag,
+                                 d_out, d_list, grid, e.work, st);
+    };
+    if (!use_graph) return step(s);
+    // The graph holds kernel arguments by value: pointers, sizes, the launch shape and
+    // (for m_flag) the maximum mass of the j-set.  Any change there needs a new capture.
+    GraphKey key;
+    key.ptr[0] = d_i; key.ptr[1] = d_out; key.ptr[2] = d_list; key.ptr[3] = e.d_items.p;
+    long long mbits = 0;
+    if (m_flag) std::memcpy(&mbits, &e.js.mmax, sizeof(mbits));
+    const long long v[11] = {stride, lmax, nnbmax, m_flag, cfg.threads, cfg.ipl, e.nitems, grid,
+                             e.ntot, e.npad, mbits};
+    std::copy(v, v + 11, key.val);
+    const bool same = e.exec && key == e.gkey;
+    if (!same) {
+        drop_graph();
+        // work buffers are sized (and allocated) by one eager step first
+        int rc = step(s);
+        if (rc != OK) return rc;
+        NB6_HIP(hipStreamSynchronize(s));
+        NB6_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        rc = step(s);
+        NB6_HIP(hipStreamEndCapture(s, &e.graph));
+        if (rc != OK) return rc;
+        NB6_HIP(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
+        e.gkey = key;
+        return OK;   // the eager step already produced this call's results
+    }
+    NB6_HIP(hipGraphLaunch(e.exec, s));
+    return OK;
+}
+
+int ens_items() { return g_ens.nitems; }
+
+}  // namespace nb6

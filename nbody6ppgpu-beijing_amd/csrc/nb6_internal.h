@@ -87,7 +87,8 @@ void jset_build(JSet &js, int n, const double *box, double mmax, hipStream_t s);
 void jset_from_soa(JSet &js, int n, int stride, const double *d_soa, hipStream_t s);
 
 struct RegfConfig {
-    int ipl = 1;          // i-particles per lane
+    int threads = 256;    // work-group size = LDS j-tile (128, 256 or 512)
+    int ipl = 1;          // i-particles per lane (1 or 2)
     int min_groups = 2048;
 };
 RegfConfig &regf_config();
@@ -98,6 +99,19 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
                 int nnbmax, int m_flag, double *d_out, int *d_list, hipStream_t s);
 int regf_last_chunks();
 void regf_release();
+
+// Scratch of one regular-force pipeline: FP32 partials, candidate counts and slots.
+struct RegfWork {
+    DBuf part, cnt, hslot, sslot;
+    void release() { part.release(); cnt.release(); hslot.release(); sslot.release(); }
+};
+
+// One persistent launch over work items (ibase, iend, jbeg, jend) of a padded j-set;
+// queue is a device counter the caller zeroes first; jbase[i] maps j back to the
+// caller's indexing.  Safe under stream capture (no timers, no allocation once w is sized).
+int regf_items_launch(const JSet &js, int ntot, int stride, const double *d_i, const int4 *items,
+                      int nitems, int *queue, const int *jbase, int lmax, int nnbmax, int m_flag,
+                      double *d_out, int *d_list, int grid, RegfWork &w, hipStream_t s);
 
 // streaming copy of bytes (multiple of 16) on stream s
 void device_copy(void *dst, const void *src, size_t bytes, hipStream_t s);

@@ -39,8 +39,6 @@ namespace nb6 {
 
 namespace {
 
-constexpr int RF_THREADS = 256;
-constexpr int RF_TJ = JTILE;          // j-particles per LDS tile (one per thread)
 constexpr int RF_FLUSH = 64;          // j per FP32 partial sum
 constexpr int CAP_SOFT = 1024;        // soft (near-field) slots per (i, chunk)
 constexpr int NB_SORT_MAX = 2048;     // largest neighbour row the finalize sorts
@@ -62,7 +60,7 @@ struct TileParams {
     const float4 *__restrict__ rec32;
     const double *__restrict__ torg;
     int nj;
-    int chunk_len;                   // multiple of RF_TJ
+    int chunk_len;                   // multiple of the block size T and of JTILE
     double mmax;
     double *__restrict__ part;       // [nchunk][7][ni]
     int *__restrict__ cnt;           // [nchunk][2][ni]: hard, soft
@@ -153,14 +151,12 @@ __device__ __forceinline__ PairBase pair_base(const IC &c, const float4 &a, cons
     return q;
 }
 
-template <int IPL, bool MFLAG>
-__global__ __launch_bounds__(RF_THREADS) void regf_tile_kernel(TileParams p) {
-    __shared__ float4 sA[2][RF_TJ];   // x', y', z', m
-    __shared__ float4 sB[2][RF_TJ];   // vx', vy', vz', 0
-
+// One work item: i in [ibase, min(ibase + T*IPL, iend)) against sorted j in
+// [j0, j1) (a whole number of tiles), partial-sum row = chunk * ni + i.
+template <int T, int IPL, bool MFLAG>
+__device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int iend, int j0,
+                                          int j1, int chunk, float4 (*sA)[T], float4 (*sB)[T]) {
     const int tid = threadIdx.x;
-    const int chunk = blockIdx.y;
-    const int ibase = blockIdx.x * (RF_THREADS * IPL);
 
     ILane L[IPL];
     int nh[IPL], ns[IPL];
@@ -170,8 +166,8 @@ __global__ __launch_bounds__(RF_THREADS) void regf_tile_kernel(TileParams p) {
 
 #pragma unroll
     for (int k = 0; k < IPL; k++) {
-        const int i = ibase + k * RF_THREADS + tid;
-        const bool ok = i < p.ni;
+        const int i = ibase + k * T + tid;
+        const bool ok = i < iend;
         L[k] = make_lane<MFLAG>(p.ib, p.ib_stride, i, ok, p.mmax);
         nh[k] = 0;
         ns[k] = 0;
@@ -184,14 +180,11 @@ __global__ __launch_bounds__(RF_THREADS) void regf_tile_kernel(TileParams p) {
         for (int f = 0; f < 7; f++) a64[k][f] = 0.0;
     }
 
-    const int j0 = chunk * p.chunk_len;
-    const int j1 = min(j0 + p.chunk_len, p.nj);
-    const int ntile = (j1 - j0 + RF_TJ - 1) / RF_TJ;
-    const int tile0 = j0 / RF_TJ;
+    const int ntile = (j1 - j0 + T - 1) / T;
 
     float4 ga, gb;
     auto gload = [&](int t) {
-        const int j = j0 + t * RF_TJ + tid;
+        const int j = j0 + t * T + tid;
         if (j < j1) {
             ga = p.rec32[2 * (size_t)j];
             gb = p.rec32[2 * (size_t)j + 1];
@@ -209,13 +202,17 @@ __global__ __launch_bounds__(RF_THREADS) void regf_tile_kernel(TileParams p) {
     for (int t = 0; t < ntile; t++) {
         const int buf = t & 1;
         if (t + 1 < ntile) gload(t + 1);
-        const int jt = j0 + t * RF_TJ;
-        const double *T = p.torg + (size_t)(tile0 + t) * 8;
+        const int jt = j0 + t * T;
+        // i relative to the origin of the 256-j tile the LDS tile starts in
         IC c[IPL];
 #pragma unroll
-        for (int k = 0; k < IPL; k++) c[k] = tile_ic<MFLAG>(L[k], T);
-
-        for (int js0 = 0; js0 < RF_TJ; js0 += RF_FLUSH) {
+        for (int k = 0; k < IPL; k++) c[k] = tile_ic<MFLAG>(L[k], p.torg + (size_t)(jt / JTILE) * 8);
+        for (int js0 = 0; js0 < T; js0 += RF_FLUSH) {
+            if (T > JTILE && js0 > 0 && js0 % JTILE == 0) {
+#pragma unroll
+                for (int k = 0; k < IPL; k++)
+                    c[k] = tile_ic<MFLAG>(L[k], p.torg + (size_t)((jt + js0) / JTILE) * 8);
+            }
 #pragma unroll 2
             for (int jj = js0; jj < js0 + RF_FLUSH; jj++) {
                 const float4 a = sA[buf][jj];
@@ -297,8 +294,8 @@ __global__ __launch_bounds__(RF_THREADS) void regf_tile_kernel(TileParams p) {
 
 #pragma unroll
     for (int k = 0; k < IPL; k++) {
-        const int i = ibase + k * RF_THREADS + tid;
-        if (i < p.ni) {
+        const int i = ibase + k * T + tid;
+        if (i < iend) {
             double *pp = p.part + (size_t)chunk * 7 * p.ni + i;
 #pragma unroll
             for (int f = 0; f < 7; f++) pp[(size_t)f * p.ni] = a64[k][f];
@@ -308,11 +305,42 @@ __global__ __launch_bounds__(RF_THREADS) void regf_tile_kernel(TileParams p) {
     }
 }
 
+template <int T, int IPL, bool MFLAG>
+__global__ __launch_bounds__(T) void regf_tile_kernel(TileParams p) {
+    __shared__ float4 sA[2][T];   // x', y', z', m
+    __shared__ float4 sB[2][T];   // vx', vy', vz', 0
+    const int chunk = blockIdx.y;
+    const int j0 = chunk * p.chunk_len;
+    tile_work<T, IPL, MFLAG>(p, blockIdx.x * (T * IPL), p.ni, j0, min(j0 + p.chunk_len, p.nj),
+                             chunk, sA, sB);
+}
+
+// Persistent form for a batch of independent clusters: each work-group takes the
+// next work item from a queue (items are ordered longest first) until none is left.
+template <int T, int IPL, bool MFLAG>
+__global__ __launch_bounds__(T) void regf_items_kernel(TileParams p,
+                                                       const int4 *__restrict__ items,
+                                                       int nitems, int *__restrict__ queue) {
+    __shared__ float4 sA[2][T];
+    __shared__ float4 sB[2][T];
+    __shared__ int it;
+    for (;;) {
+        if (threadIdx.x == 0) it = atomicAdd(queue, 1);
+        __syncthreads();
+        const int k = __builtin_amdgcn_readfirstlane(it);
+        __syncthreads();
+        if (k >= nitems) break;
+        const int4 w = items[k];   // ibase, iend, j0, j1 (chunk 0)
+        tile_work<T, IPL, MFLAG>(p, w.x, w.y, w.z, w.w, 0, sA, sB);
+    }
+}
+
 struct FinParams {
     const double *__restrict__ ib;   // SoA [8][ib_stride]
     int ni, ib_stride;
     const double *__restrict__ rec64;   // sorted AoS [nj][8]
     const int *__restrict__ perm;       // sorted -> caller index
+    const int *__restrict__ jbase;      // per i: index of its cluster's first j (or null)
     int nj;
     int nchunk;
     const double *__restrict__ part;
@@ -435,7 +463,7 @@ __global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
                     const int sj = hs[k];
                     if ((unsigned)sj < (unsigned)f.nj) {
                         isnb = classify<MFLAG>(f.rec64 + (size_t)sj * 8, I, true, s);
-                        jo = f.perm[sj];
+                        jo = f.perm[sj] - (f.jbase ? f.jbase[i] : 0);
                     }
                 }
                 const unsigned long long b = __ballot(isnb);
@@ -478,17 +506,52 @@ __global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
     }
 }
 
-struct Work {
-    DBuf part, cnt, hslot, sslot;
-    int last_chunks = 0;
-} g_work;
+RegfWork g_work;
+int g_last_chunks = 0;
 
-template <int IPL>
-void launch_tile(bool mflag, dim3 grid, const TileParams &tp, hipStream_t s) {
+template <int T, int IPL>
+void launch_tile_t(bool mflag, dim3 grid, const TileParams &tp, hipStream_t s) {
     if (mflag)
-        hipLaunchKernelGGL((regf_tile_kernel<IPL, true>), grid, dim3(RF_THREADS), 0, s, tp);
+        hipLaunchKernelGGL((regf_tile_kernel<T, IPL, true>), grid, dim3(T), 0, s, tp);
     else
-        hipLaunchKernelGGL((regf_tile_kernel<IPL, false>), grid, dim3(RF_THREADS), 0, s, tp);
+        hipLaunchKernelGGL((regf_tile_kernel<T, IPL, false>), grid, dim3(T), 0, s, tp);
+}
+
+template <int T, int IPL>
+void launch_items_t(bool mflag, int grid, const TileParams &tp, const int4 *items, int nitems,
+                    int *queue, hipStream_t s) {
+    if (mflag)
+        hipLaunchKernelGGL((regf_items_kernel<T, IPL, true>), dim3(grid), dim3(T), 0, s, tp, items,
+                           nitems, queue);
+    else
+        hipLaunchKernelGGL((regf_items_kernel<T, IPL, false>), dim3(grid), dim3(T), 0, s, tp, items,
+                           nitems, queue);
+}
+
+// dispatch on the tile configuration (block size T in {128, 256, 512}, IPL in {1, 2})
+void launch_tile(const RegfConfig &cfg, bool mflag, dim3 grid, const TileParams &tp, hipStream_t s) {
+    const int key = cfg.threads * 10 + cfg.ipl;
+    switch (key) {
+        case 1281: launch_tile_t<128, 1>(mflag, grid, tp, s); break;
+        case 1282: launch_tile_t<128, 2>(mflag, grid, tp, s); break;
+        case 2562: launch_tile_t<256, 2>(mflag, grid, tp, s); break;
+        case 5121: launch_tile_t<512, 1>(mflag, grid, tp, s); break;
+        case 5122: launch_tile_t<512, 2>(mflag, grid, tp, s); break;
+        default: launch_tile_t<256, 1>(mflag, grid, tp, s); break;
+    }
+}
+
+void launch_items(const RegfConfig &cfg, bool mflag, int grid, const TileParams &tp,
+                  const int4 *items, int nitems, int *queue, hipStream_t s) {
+    const int key = cfg.threads * 10 + cfg.ipl;
+    switch (key) {
+        case 1281: launch_items_t<128, 1>(mflag, grid, tp, items, nitems, queue, s); break;
+        case 1282: launch_items_t<128, 2>(mflag, grid, tp, items, nitems, queue, s); break;
+        case 2562: launch_items_t<256, 2>(mflag, grid, tp, items, nitems, queue, s); break;
+        case 5121: launch_items_t<512, 1>(mflag, grid, tp, items, nitems, queue, s); break;
+        case 5122: launch_items_t<512, 2>(mflag, grid, tp, items, nitems, queue, s); break;
+        default: launch_items_t<256, 1>(mflag, grid, tp, items, nitems, queue, s); break;
+    }
 }
 
 }  // namespace
@@ -510,17 +573,17 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
         return E_ARG;
     }
     const RegfConfig &cfg = regf_config();
-    const int ipl = cfg.ipl;
-    const int nbi = RF_THREADS * ipl;
+    const int nbi = cfg.threads * cfg.ipl;
     const int nib = (ni + nbi - 1) / nbi;
     const int nj = js.n;
-    const int ntile = (nj + RF_TJ - 1) / RF_TJ;
+    const int unit = std::max(cfg.threads, JTILE);       // chunk granularity (j)
+    const int nunit = (nj + unit - 1) / unit;
     int want = (cfg.min_groups + nib - 1) / nib;
-    int maxc = std::max(1, ntile / 4);
+    int maxc = std::max(1, nunit * unit / (4 * JTILE));
     int nchunk = std::max(1, std::min(want, maxc));
-    const int tpc = (ntile + nchunk - 1) / nchunk;   // tiles per chunk
-    const int clen = tpc * RF_TJ;
-    nchunk = (ntile + tpc - 1) / tpc;
+    const int upc = (nunit + nchunk - 1) / nchunk;        // units per chunk
+    const int clen = upc * unit;
+    nchunk = (nunit + upc - 1) / upc;
     const int nnbeff = std::max(0, std::min(nnbmax, lmax - 1));
     const int caph = nnbeff + 8;
     const int caps = CAP_SOFT;
@@ -530,7 +593,7 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
     int *cnt = static_cast<int *>(g_work.cnt.get(rows * 2 * sizeof(int)));
     int *hslot = static_cast<int *>(g_work.hslot.get(rows * caph * sizeof(int)));
     int *sslot = static_cast<int *>(g_work.sslot.get(rows * caps * sizeof(int)));
-    g_work.last_chunks = nchunk;
+    g_last_chunks = nchunk;
 
     TileParams tp;
     tp.ib = d_i; tp.ni = ni; tp.ib_stride = ni_stride;
@@ -542,17 +605,13 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
     dim3 grid(nib, nchunk);
     KTimer &t1 = ktimer("regf_tile");
     t1.start(s);
-    switch (ipl) {
-        case 2: launch_tile<2>(m_flag != 0, grid, tp, s); break;
-        case 4: launch_tile<4>(m_flag != 0, grid, tp, s); break;
-        default: launch_tile<1>(m_flag != 0, grid, tp, s); break;
-    }
+    launch_tile(cfg, m_flag != 0, grid, tp, s);
     NB6_HIP(hipGetLastError());
     t1.stop(s);
 
     FinParams fp;
     fp.ib = d_i; fp.ni = ni; fp.ib_stride = ni_stride;
-    fp.rec64 = js.rec64.as<double>(); fp.perm = js.perm.as<int>(); fp.nj = nj;
+    fp.rec64 = js.rec64.as<double>(); fp.perm = js.perm.as<int>(); fp.jbase = nullptr; fp.nj = nj;
     fp.nchunk = nchunk;
     fp.part = part; fp.cnt = cnt;
     fp.hslot = hslot; fp.caph = caph; fp.sslot = sslot; fp.caps = caps;
@@ -570,13 +629,63 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
     return OK;
 }
 
-int regf_last_chunks() { return g_work.last_chunks; }
-
-void regf_release() {
-    g_work.part.release();
-    g_work.cnt.release();
-    g_work.hslot.release();
-    g_work.sslot.release();
+// Batched form (ensemble.hip): one persistent launch over work items, nchunk = 1,
+// rows indexed by the particle's position in the concatenated i-array.
+int regf_items_launch(const JSet &js, int ntot, int stride, const double *d_i, const int4 *items,
+                      int nitems, int *queue, const int *jbase, int lmax, int nnbmax, int m_flag,
+                      double *d_out, int *d_list, int grid, RegfWork &w, hipStream_t s) {
+    if (lmax < 1 || lmax > NB_SORT_MAX + 1) {
+        set_error(E_ARG, "nb6_ens_regf_device: lmax outside [1, 2049]");
+        return E_ARG;
+    }
+    const RegfConfig &cfg = regf_config();
+    const int nnbeff = std::max(0, std::min(nnbmax, lmax - 1));
+    const int caph = nnbeff + 8;
+    const int caps = CAP_SOFT;
+    const size_t rows = (size_t)ntot;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    NB6_HIP(hipStreamIsCapturing(s, &cs));
+    const bool capturing = cs != hipStreamCaptureStatusNone;
+    if (capturing && (w.part.bytes < rows * 7 * sizeof(double) || w.hslot.bytes < rows * caph * sizeof(int) ||
+                      w.sslot.bytes < rows * caps * sizeof(int) || w.cnt.bytes < rows * 2 * sizeof(int))) {
+        set_error(E_STATE, "regf_items_launch: work buffers must be sized before stream capture");
+        return E_STATE;
+    }
+    double *part = static_cast<double *>(w.part.get(rows * 7 * sizeof(double)));
+    int *cnt = static_cast<int *>(w.cnt.get(rows * 2 * sizeof(int)));
+    int *hslot = static_cast<int *>(w.hslot.get(rows * caph * sizeof(int)));
+    int *sslot = static_cast<int *>(w.sslot.get(rows * caps * sizeof(int)));
+    TileParams tp;
+    tp.ib = d_i; tp.ni = ntot; tp.ib_stride = stride;
+    tp.rec32 = js.rec32.as<float4>(); tp.torg = js.torg.as<double>(); tp.nj = js.n;
+    tp.chunk_len = js.n;
+    tp.mmax = js.mmax;
+    tp.part = part; tp.cnt = cnt;
+    tp.hslot = hslot; tp.caph = caph; tp.sslot = sslot; tp.caps = caps;
+    KTimer &t1 = ktimer("regf_items");
+    if (!capturing) t1.start(s);
+    launch_items(cfg, m_flag != 0, grid, tp, items, nitems, queue, s);
+    NB6_HIP(hipGetLastError());
+    if (!capturing) t1.stop(s);
+    FinParams fp;
+    fp.ib = d_i; fp.ni = ntot; fp.ib_stride = stride;
+    fp.rec64 = js.rec64.as<double>(); fp.perm = js.perm.as<int>(); fp.jbase = jbase; fp.nj = js.n;
+    fp.nchunk = 1;
+    fp.part = part; fp.cnt = cnt;
+    fp.hslot = hslot; fp.caph = caph; fp.sslot = sslot; fp.caps = caps;
+    fp.lmax = lmax; fp.nnbmax = nnbeff;
+    fp.out = d_out; fp.out_stride = stride; fp.list = d_list;
+    const dim3 fgrid((ntot + 3) / 4);
+    if (m_flag)
+        hipLaunchKernelGGL(regf_finalize_kernel<true>, fgrid, dim3(256), 0, s, fp);
+    else
+        hipLaunchKernelGGL(regf_finalize_kernel<false>, fgrid, dim3(256), 0, s, fp);
+    NB6_HIP(hipGetLastError());
+    return OK;
 }
+
+int regf_last_chunks() { return g_last_chunks; }
+
+void regf_release() { g_work.release(); }
 
 }  // namespace nb6

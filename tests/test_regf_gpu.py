@@ -53,18 +53,19 @@ def gpunb():
         g.close()
 
 
-@pytest.mark.parametrize("ipl", [1, 2, 4])
-def test_regf_plummer_1k_all_i(gpunb, ipl):
+@pytest.mark.parametrize("threads,ipl", [(256, 1), (256, 2), (128, 1), (128, 2), (512, 1),
+                                         (512, 2)])
+def test_regf_plummer_1k_all_i(gpunb, threads, ipl):
     m, x, v, h2, dtr = c1_inputs()
     n = m.size
     lmax, nnbmax = LMAX_SMALL, nnbmax_for(n, LMAX_SMALL)
-    lib.load().nb6_set_regf_config(ipl, 2048)
+    lib.load().nb6_set_regf_config(threads, ipl, 2048)
     g = gpunb(n)
     g.send(m, x, v)
     got = g.regf(h2, dtr, x, v, lmax, nnbmax)
     ref = O.regf(m, x, v, h2, dtr, x, v, lmax, nnbmax)
-    lib.load().nb6_set_regf_config(1, 2048)
-    _compare(got, ref, f"C1 ipl={ipl}")
+    lib.load().nb6_set_regf_config(256, 1, 2048)
+    _compare(got, ref, f"C1 threads={threads} ipl={ipl}")
     # the self pair is always a neighbour (r2 = 0 < h2)
     for i in range(n):
         c = got[3][i, 0]
