@@ -130,9 +130,17 @@ int nb6_pot_device(int n, int n_stride, const double *d_x, int i0, int ni,
                    double *d_pot, void *stream);
 
 /* Tile configuration of the regular-force kernel: work-group size = LDS j-tile
- * (128, 256 or 512), i-particles per lane (1 or 2), and the minimum number of
- * work-groups a call is split into (j-chunks are added until it is reached). */
+ * (128, 256 or 512; 0 = chosen per call: 512 for ni >= 65536, else 256),
+ * i-particles per lane (1 or 2), and the minimum number of work-groups a call is
+ * split into (j-chunks are added until it is reached).  Default: 0, 1, 2048. */
 int nb6_set_regf_config(int threads, int i_per_lane, int min_groups);
+
+/* Ordering and pruning of the regular-force kernel: i-sets of at least isort_min
+ * particles are processed in Morton order of their positions (0 = never), and
+ * far_test = 1 lets a wavefront skip the per-pair neighbour test on a 256-j tile
+ * that is provably outside every lane's near field.  Neither changes results.
+ * Defaults: 2048, 1. */
+int nb6_set_regf_tuning(int isort_min, int far_test);
 
 /* Average device time (ms) of the last launch of the named kernel family
  * ("regf_tile", "regf_finalize", "pot", "irr") measured with HIP events, and

@@ -469,8 +469,8 @@ int nb6_pot_device(int n, int n_stride, const double *d_x, int i0, int ni, doubl
 }
 
 int nb6_set_regf_config(int threads, int i_per_lane, int min_groups) {
-    if (threads != 128 && threads != 256 && threads != 512) {
-        set_error(E_ARG, "nb6_set_regf_config: threads must be 128, 256 or 512");
+    if (threads != 0 && threads != 128 && threads != 256 && threads != 512) {
+        set_error(E_ARG, "nb6_set_regf_config: threads must be 0 (by ni), 128, 256 or 512");
         return E_ARG;
     }
     if (i_per_lane != 1 && i_per_lane != 2) {
@@ -484,6 +484,16 @@ int nb6_set_regf_config(int threads, int i_per_lane, int min_groups) {
     regf_config().threads = threads;
     regf_config().ipl = i_per_lane;
     regf_config().min_groups = min_groups;
+    return OK;
+}
+
+int nb6_set_regf_tuning(int isort_min, int far_test) {
+    if (isort_min < 0 || (far_test != 0 && far_test != 1)) {
+        set_error(E_ARG, "nb6_set_regf_tuning: isort_min must be >= 0 and far_test 0 or 1");
+        return E_ARG;
+    }
+    regf_config().isort_min = isort_min;
+    regf_config().far_test = far_test;
     return OK;
 }
 

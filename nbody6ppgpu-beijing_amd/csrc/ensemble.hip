@@ -156,9 +156,9 @@ __global__ void ens_zero_kernel(int *q) { *q = 0; }
 
 struct GraphKey {
     const void *ptr[4] = {nullptr, nullptr, nullptr, nullptr};   // d_i, d_out, d_list, items
-    long long val[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    long long val[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     bool operator==(const GraphKey &o) const {
-        return std::equal(ptr, ptr + 4, o.ptr) && std::equal(val, val + 11, o.val);
+        return std::equal(ptr, ptr + 4, o.ptr) && std::equal(val, val + 12, o.val);
     }
 };
 
@@ -292,6 +292,7 @@ int ens_send_device(int nc, const int *offsets, const double *d_soa, int stride,
     hipLaunchKernelGGL(ens_build_kernel, dim3((np + 255) / 256), dim3(256), 0, s,
                        e.js.rec64.as<double>(), np, e.js.rec32.as<float4>(), e.js.torg.as<double>());
     NB6_HIP(hipGetLastError());
+    jset_tile_boxes(e.js, np, s);
     e.js.n = np;
     e.js.ntile = np / JTILE;
     // maximum mass, for the m_flag error band
@@ -301,19 +302,20 @@ int ens_send_device(int nc, const int *offsets, const double *d_soa, int stride,
     double mmax = 0.0;
     for (int c = 0; c < nc; c++) mmax = std::max(mmax, box[(size_t)c * 7 + 6]);
     e.js.mmax = mmax;
-    // work items: (cluster, i-block), longest first
-    const RegfConfig &cfg = regf_config();
+    // work items: (cluster, block of Morton-ordered i slots), longest first; the real
+    // particles of cluster c fill slots [padoff[c], padoff[c] + n_c)
+    const RegfConfig cfg = regf_effective(0);
     const int bi = cfg.threads * cfg.ipl;
     std::vector<int4> items;
     std::vector<long long> cost;
     for (int c = 0; c < nc; c++) {
         const int nci = offsets[c + 1] - offsets[c];
         const int plen = e.padoff[c + 1] - e.padoff[c];
-        for (int ib = offsets[c]; ib < offsets[c + 1]; ib += bi) {
-            items.push_back(make_int4(ib, offsets[c + 1], e.padoff[c], e.padoff[c + 1]));
-            cost.push_back((long long)std::min(bi, offsets[c + 1] - ib) * plen);
+        const int send = e.padoff[c] + nci;
+        for (int ib = e.padoff[c]; ib < send; ib += bi) {
+            items.push_back(make_int4(ib, send, e.padoff[c], e.padoff[c + 1]));
+            cost.push_back((long long)std::min(bi, send - ib) * plen);
         }
-        (void)nci;
     }
     std::vector<int> order(items.size());
     std::iota(order.begin(), order.end(), 0);
@@ -334,14 +336,15 @@ int ens_regf_device(const double *d_i, int stride, int lmax, int nnbmax, int m_f
     if (!e.open || e.nc == 0) { set_error(E_STATE, "nb6_ens_regf_device: nothing sent"); return E_STATE; }
     if (stride < e.ntot) { set_error(E_ARG, "nb6_ens_regf_device: stride below the particle count"); return E_ARG; }
     Device &d = device();
-    const RegfConfig &cfg = regf_config();
+    const RegfConfig cfg = regf_effective(0);
     const int per_cu = std::max(1, (8 * 256) / cfg.threads);
     const int grid = std::min(e.nitems, d.num_cu * per_cu);
     auto step = [&](hipStream_t st) {
         hipLaunchKernelGGL(ens_zero_kernel, dim3(1), dim3(1), 0, st, e.d_queue.as<int>());
-        return regf_items_launch(e.js, e.ntot, stride, d_i, e.d_items.as<int4>(), e.nitems,
-                                 e.d_queue.as<int>(), e.d_jbase.as<int>(), lmax, nnbmax, m_flag,
-                                 d_out, d_list, grid, e.work, st);
+        return regf_items_launch(e.js, e.npad, e.js.perm.as<int>(), stride, d_i,
+                                 e.d_items.as<int4>(), e.nitems, e.d_queue.as<int>(),
+                                 e.d_jbase.as<int>(), lmax, nnbmax, m_flag, d_out, d_list, grid,
+                                 e.work, st);
     };
     if (!use_graph) return step(s);
     // The graph holds kernel arguments by value: pointers, sizes, the launch shape and
@@ -350,9 +353,9 @@ int ens_regf_device(const double *d_i, int stride, int lmax, int nnbmax, int m_f
     key.ptr[0] = d_i; key.ptr[1] = d_out; key.ptr[2] = d_list; key.ptr[3] = e.d_items.p;
     long long mbits = 0;
     if (m_flag) std::memcpy(&mbits, &e.js.mmax, sizeof(mbits));
-    const long long v[11] = {stride, lmax, nnbmax, m_flag, cfg.threads, cfg.ipl, e.nitems, grid,
-                             e.ntot, e.npad, mbits};
-    std::copy(v, v + 11, key.val);
+    const long long v[12] = {stride, lmax, nnbmax, m_flag, cfg.threads, cfg.ipl, e.nitems, grid,
+                             e.ntot, e.npad, mbits, cfg.far_test};
+    std::copy(v, v + 12, key.val);
     const bool same = e.exec && key == e.gkey;
     if (!same) {
         drop_graph();

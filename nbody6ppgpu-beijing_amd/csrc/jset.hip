@@ -113,6 +113,67 @@ __global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ a
     if (threadIdx.x < 7) out[threadIdx.x] = sh[threadIdx.x][0];
 }
 
+// Morton key of i-particle positions (SoA rows 0-2) in a given box (clamped)
+__global__ __launch_bounds__(256) void imorton_kernel(const double *__restrict__ ib, int stride,
+                                                      int ni, double lx, double ly, double lz,
+                                                      double sx, double sy, double sz,
+                                                      unsigned *__restrict__ key,
+                                                      int *__restrict__ val) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= ni) return;
+    const double x = ib[i], y = ib[(size_t)stride + i], z = ib[2 * (size_t)stride + i];
+    const double fx = (x - lx) * sx, fy = (y - ly) * sy, fz = (z - lz) * sz;
+    // comparisons in FP64 first: NaN and far-outside values clamp without overflow
+    const int qx = fx > 0.0 ? (fx < 1023.0 ? (int)fx : 1023) : 0;
+    const int qy = fy > 0.0 ? (fy < 1023.0 ? (int)fy : 1023) : 0;
+    const int qz = fz > 0.0 ? (fz < 1023.0 ? (int)fz : 1023) : 0;
+    key[i] = spread10(qx) | (spread10(qy) << 1) | (spread10(qz) << 2);
+    val[i] = i;
+}
+
+// FP32 bounds of one tile's tile-relative records (see JSet::tbox)
+__global__ __launch_bounds__(256) void tile_box_kernel(const float4 *__restrict__ rec32, int n,
+                                                       float *__restrict__ tbox) {
+    __shared__ float sh[14][256];
+    const int t = blockIdx.x;
+    const int s = t * JTILE + threadIdx.x;
+    float v[14];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        v[q] = FLT_MAX; v[3 + q] = -FLT_MAX; v[6 + q] = FLT_MAX; v[9 + q] = -FLT_MAX;
+    }
+    v[12] = 0.f;
+    v[13] = 0.f;
+    if (threadIdx.x < JTILE && s < n) {
+        const float4 a = rec32[2 * (size_t)s];
+        const float4 b = rec32[2 * (size_t)s + 1];
+        if (fabsf(a.x) < 1.0e16f) {
+            v[0] = v[3] = a.x; v[1] = v[4] = a.y; v[2] = v[5] = a.z;
+            v[6] = v[9] = b.x; v[7] = v[10] = b.y; v[8] = v[11] = b.z;
+            v[12] = a.w;
+            v[13] = 1.f;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 14; q++) sh[q][threadIdx.x] = v[q];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            const int u = threadIdx.x, w = threadIdx.x + o;
+            for (int q = 0; q < 3; q++) {
+                sh[q][u] = fminf(sh[q][u], sh[q][w]);
+                sh[3 + q][u] = fmaxf(sh[3 + q][u], sh[3 + q][w]);
+                sh[6 + q][u] = fminf(sh[6 + q][u], sh[6 + q][w]);
+                sh[9 + q][u] = fmaxf(sh[9 + q][u], sh[9 + q][w]);
+            }
+            sh[12][u] = fmaxf(sh[12][u], sh[12][w]);
+            sh[13][u] += sh[13][w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < TBOX) tbox[(size_t)t * TBOX + threadIdx.x] = threadIdx.x < 14 ? sh[threadIdx.x][0] : 0.f;
+}
+
 // plain streaming copy (used to calibrate HBM byte counters against a known size)
 __global__ __launch_bounds__(256) void copy_kernel(const uint4 *__restrict__ src,
                                                    uint4 *__restrict__ dst, size_t n16) {
@@ -143,12 +204,21 @@ void jset_reserve(JSet &js, int cap) {
     js.keys.get((size_t)cap * 2 * sizeof(unsigned));
     js.vals.get((size_t)cap * sizeof(int));
     js.stat.get(8 * sizeof(double));
+    js.tbox.get((size_t)ntile * TBOX * sizeof(float));
+}
+
+void jset_tile_boxes(JSet &js, int n, hipStream_t s) {
+    const int nt = (n + JTILE - 1) / JTILE;
+    if (nt <= 0) return;
+    hipLaunchKernelGGL(tile_box_kernel, dim3(nt), dim3(256), 0, s, js.rec32.as<float4>(), n,
+                       js.tbox.as<float>());
+    NB6_HIP(hipGetLastError());
 }
 
 void jset_release(JSet &js) {
     js.in64.release(); js.rec64.release(); js.rec32.release(); js.torg.release();
     js.perm.release(); js.keys.release(); js.vals.release(); js.stat.release();
-    js.sort_tmp.release();
+    js.sort_tmp.release(); js.tbox.release();
     js.cap = 0; js.n = 0; js.ntile = 0;
 }
 
@@ -168,6 +238,7 @@ void jset_build(JSet &js, int n, const double *box, double mmax, hipStream_t s) 
         mmax = st[6];
     }
     js.mmax = mmax;
+    for (int d = 0; d < 6; d++) js.box[d] = box[d];
     double sc[3];
     for (int d = 0; d < 3; d++) {
         const double w = box[3 + d] - box[d];
@@ -188,6 +259,28 @@ void jset_build(JSet &js, int n, const double *box, double mmax, hipStream_t s) 
                        js.perm.as<int>(), js.rec64.as<double>(), js.rec32.as<float4>(),
                        js.torg.as<double>());
     NB6_HIP(hipGetLastError());
+    jset_tile_boxes(js, n, s);
+}
+
+const int *jset_sort_i(const JSet &js, int ni, int stride, const double *d_i, RegfWork &w,
+                       hipStream_t s) {
+    double sc[3];
+    for (int d = 0; d < 3; d++) {
+        const double wd = js.box[3 + d] - js.box[d];
+        sc[d] = wd > 0.0 ? 1024.0 / (wd * (1.0 + 1e-12)) : 0.0;
+    }
+    unsigned *k0 = static_cast<unsigned *>(w.ikey.get((size_t)ni * 2 * sizeof(unsigned)));
+    unsigned *k1 = k0 + ni;
+    int *v0 = static_cast<int *>(w.ival.get((size_t)ni * 2 * sizeof(int)));
+    int *v1 = v0 + ni;
+    hipLaunchKernelGGL(imorton_kernel, dim3((ni + 255) / 256), dim3(256), 0, s, d_i, stride, ni,
+                       js.box[0], js.box[1], js.box[2], sc[0], sc[1], sc[2], k0, v0);
+    NB6_HIP(hipGetLastError());
+    size_t tb = 0;
+    NB6_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, ni, 0, 30, s));
+    void *tmp = w.itmp.get(tb);
+    NB6_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, v1, ni, 0, 30, s));
+    return v1;
 }
 
 void jset_from_soa(JSet &js, int n, int stride, const double *d_soa, hipStream_t s) {

@@ -73,13 +73,20 @@ constexpr int JTILE = 256;   // j-particles per tile (one tile origin each)
 //   rec32 [cap][2] float4 sorted       (x-P, y-P, z-P, m), (vx-V, vy-V, vz-V, 0)
 //   torg  [ntile][8] FP64 tile origins (Px Py Pz Vx Vy Vz 0 0)
 //   perm  [cap] int: sorted position -> caller's 0-based index
+//   tbox  [ntile][16] FP32 bounds of each tile's records relative to its origin:
+//         pos lo[3], pos hi[3], vel lo[3], vel hi[3], max mass, record count, 0, 0
+//         (far massless padding records, |x| >= 1e16, are left out)
+constexpr int TBOX = 16;
 struct JSet {
     int n = 0;
     int cap = 0;
     int ntile = 0;
     double mmax = 0.0;
-    DBuf in64, rec64, rec32, torg, perm, keys, vals, stat, sort_tmp;
+    double box[6] = {0, 0, 0, 0, 0, 0};   // lo[3], hi[3] of the positions (host copy)
+    DBuf in64, rec64, rec32, torg, perm, keys, vals, stat, sort_tmp, tbox;
 };
+// per-tile bounds of rec32 (after rec32 is built): one work-group per tile
+void jset_tile_boxes(JSet &js, int n, hipStream_t s);
 void jset_reserve(JSet &js, int cap);
 void jset_release(JSet &js);
 // in64 already holds n records; box = {lo x,y,z, hi x,y,z} or nullptr (computed)
@@ -87,11 +94,15 @@ void jset_build(JSet &js, int n, const double *box, double mmax, hipStream_t s);
 void jset_from_soa(JSet &js, int n, int stride, const double *d_soa, hipStream_t s);
 
 struct RegfConfig {
-    int threads = 256;    // work-group size = LDS j-tile (128, 256 or 512)
+    int threads = 0;      // work-group size = LDS j-tile (128, 256 or 512; 0: by ni)
     int ipl = 1;          // i-particles per lane (1 or 2)
     int min_groups = 2048;
+    int isort_min = 2048; // i-sets this large are Morton-ordered (0: never)
+    int far_test = 1;     // skip the pair test on tiles far from the whole wavefront
 };
 RegfConfig &regf_config();
+// the configuration a call with ni i-particles runs with (threads resolved)
+RegfConfig regf_effective(int ni);
 
 // Launch the regular-force pipeline on device data.  d_i SoA[8][ni_stride],
 // d_out SoA[7][ni_stride], d_list [ni][lmax].
@@ -103,15 +114,26 @@ void regf_release();
 // Scratch of one regular-force pipeline: FP32 partials, candidate counts and slots.
 struct RegfWork {
     DBuf part, cnt, hslot, sslot;
-    void release() { part.release(); cnt.release(); hslot.release(); sslot.release(); }
+    DBuf ikey, ival, itmp;           // Morton sort of the i-set
+    void release() {
+        part.release(); cnt.release(); hslot.release(); sslot.release();
+        ikey.release(); ival.release(); itmp.release();
+    }
 };
 
-// One persistent launch over work items (ibase, iend, jbeg, jend) of a padded j-set;
-// queue is a device counter the caller zeroes first; jbase[i] maps j back to the
-// caller's indexing.  Safe under stream capture (no timers, no allocation once w is sized).
-int regf_items_launch(const JSet &js, int ntot, int stride, const double *d_i, const int4 *items,
-                      int nitems, int *queue, const int *jbase, int lmax, int nnbmax, int m_flag,
-                      double *d_out, int *d_list, int grid, RegfWork &w, hipStream_t s);
+// Morton order of ni i-particles (SoA [8][stride] positions) in the j-set's box:
+// returns the device array slot -> i row (jset.hip).
+const int *jset_sort_i(const JSet &js, int ni, int stride, const double *d_i, RegfWork &w,
+                       hipStream_t s);
+
+// One persistent launch over work items (slot begin, slot end, jbeg, jend) of a padded
+// j-set; nslots rows of work arrays, imap[slot] = row of d_i / d_out / d_list (< 0:
+// empty slot); queue is a device counter the caller zeroes first; jbase[row] maps j
+// back to the caller's indexing.  Safe under stream capture once w is sized.
+int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, const double *d_i,
+                      const int4 *items, int nitems, int *queue, const int *jbase, int lmax,
+                      int nnbmax, int m_flag, double *d_out, int *d_list, int grid, RegfWork &w,
+                      hipStream_t s);
 
 // streaming copy of bytes (multiple of 16) on stream s
 void device_copy(void *dst, const void *src, size_t bytes, hipStream_t s);

@@ -56,9 +56,12 @@ constexpr float TINY = 1.1754944e-38f;            // keeps r2 = 0 a candidate
 
 struct TileParams {
     const double *__restrict__ ib;   // SoA [8][ib_stride]: x y z vx vy vz h2 dtr
-    int ni, ib_stride;
+    int ni, ib_stride;               // ni = slot rows of the work arrays
+    const int *__restrict__ imap;    // slot -> row of ib (nullptr: identity; < 0: empty slot)
     const float4 *__restrict__ rec32;
     const double *__restrict__ torg;
+    const float *__restrict__ tbox;  // [ntile][TBOX] per-tile bounds
+    int far_test;                    // 0: always run the pair test
     int nj;
     int chunk_len;                   // multiple of the block size T and of JTILE
     double mmax;
@@ -138,6 +141,41 @@ __device__ __forceinline__ IC tile_ic(const ILane &L, const double *__restrict__
     return c;
 }
 
+// True when no pair of this lane's i with the 256-j tile whose bounds are B can be a
+// hard or soft candidate, so the per-pair test can be skipped for the whole tile.
+// The loop records a candidate only if min(r2, r2p - KAPPA_P r2) < ths, with
+// r2 = |dx|^2 and r2p = |dx + dtr dv|^2 formed from FP32 tile-relative values.
+// Every such dx lies in the position box and dx + dtr dv in the box of predicted
+// positions (lo + dtr vlo, hi + dtr vhi), per component, so |dx| >= D and
+// |dx + dtr dv| >= Dp (FP32 rounding is monotonic).  The test asks
+// min(D, Dp)^2 - 2^-16 X^2 >= (1 + 2^-8) ths with X = Dmax + dtr Wmax >= |dx|, |dtr dv|;
+// its margins exceed KAPPA_P X^2 plus the FP32 rounding of r2, r2p and of the bound.
+template <bool MFLAG>
+__device__ __forceinline__ bool lane_far(const IC &c, bool ok, const float *__restrict__ B) {
+    if (!ok || B[13] == 0.f) return true;          // empty slot or tile without records
+    const float dt = c.dt1;
+    if (!(dt >= 0.f)) return false;
+    const float ci[3] = {c.x, c.y, c.z};
+    const float vi[3] = {c.u, c.v, c.w};
+    float dd = 0.f, dp = 0.f, dm = 0.f, ww = 0.f;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const float lo = B[q], hi = B[3 + q], vlo = B[6 + q], vhi = B[9 + q];
+        const float e = fmaxf(fmaxf(lo - ci[q], ci[q] - hi), 0.f);
+        dd = fmaf(e, e, dd);
+        const float cp = fmaf(dt, vi[q], ci[q]);
+        const float ep = fmaxf(fmaxf(fmaf(dt, vlo, lo) - cp, cp - fmaf(dt, vhi, hi)), 0.f);
+        dp = fmaf(ep, ep, dp);
+        const float f = fmaxf(fabsf(lo - ci[q]), fabsf(hi - ci[q]));
+        dm = fmaf(f, f, dm);
+        const float g = fmaxf(fabsf(vlo - vi[q]), fabsf(vhi - vi[q]));
+        ww = fmaf(g, g, ww);
+    }
+    const float X = sqrtf(dm) + dt * sqrtf(ww);
+    const float ths = MFLAG ? fmaf(B[12], c.hS, c.dS) : c.hS;
+    return fmaf(-0x1p-16f * X, X, fminf(dd, dp)) >= ths * (1.0f + 0x1p-8f);
+}
+
 struct PairBase {
     float dx, dy, dz, du, dv, dw, r2, rv;
 };
@@ -149,6 +187,68 @@ __device__ __forceinline__ PairBase pair_base(const IC &c, const float4 &a, cons
     q.r2 = fmaf(q.dz, q.dz, fmaf(q.dy, q.dy, q.dx * q.dx));
     q.rv = fmaf(q.dz, q.dw, fmaf(q.dy, q.dv, q.dx * q.du));
     return q;
+}
+
+// One j against the IPL i-particles of this lane.  TEST=false leaves out the
+// candidate test: only used where lane_far holds for every lane of the wavefront,
+// where the test could not have recorded anything, so both forms give the same sums.
+template <bool MFLAG, bool TEST>
+__device__ __forceinline__ void pair_step(const TileParams &p, const IC &ck, const float4 a,
+                                          const float4 b, int jg, int j1, int &nhk, int &nsk,
+                                          int *__restrict__ hpk, int *__restrict__ spk,
+                                          float *__restrict__ acc) {
+    {
+        const PairBase q = pair_base(ck, a, b);
+        // v_rsq_f32 plus one Newton-Raphson step
+        float rinv = __builtin_amdgcn_rsqf(q.r2);
+        {
+            const float h = rinv * rinv;
+            const float e = fmaf(-q.r2, h, 1.0f);
+            rinv = fmaf(rinv, 0.5f * e, rinv);
+        }
+        if (TEST) {
+            // cheap conservative bound on the soft test:
+            // r2p >= r2 + 2 dtr min(rv, 0)  (dt2 carries the 1/(1-kappa) slack)
+            const float t1 = fmaf(ck.dt2, fminf(q.rv, 0.f), q.r2);
+            const float thq = MFLAG ? fmaf(a.w, ck.hQ, ck.dQ) : ck.hQ;
+            if (t1 < thq) {
+                // keep the full test inside the rare branch: the empty asm
+                // makes dtr opaque here, so the compiler cannot hoist it
+                float dtl = ck.dt1;
+                asm volatile("" : "+v"(dtl));
+                const float ex = fmaf(dtl, q.du, q.dx);
+                const float ey = fmaf(dtl, q.dv, q.dy);
+                const float ez = fmaf(dtl, q.dw, q.dz);
+                const float r2p = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+                const float mn = fminf(q.r2, fmaf(-KAPPA_P, q.r2, r2p));
+                const float thh = MFLAG ? fmaf(a.w, ck.hH, ck.dH) : ck.hH;
+                const float ths = MFLAG ? fmaf(a.w, ck.hS, ck.dS) : ck.hS;
+                if (mn < thh && jg < j1) {
+                    if (nhk < p.caph) hpk[nhk] = jg;
+                    nhk++;
+                    rinv = 0.f;
+                } else if (mn < ths && jg < j1 && nsk < p.caps) {
+                    spk[nsk] = jg;
+                    nsk++;
+                    rinv = 0.f;
+                }
+            }
+        }
+        const float rinv2 = rinv * rinv;
+        const float mr1 = a.w * rinv;
+        const float mr3 = mr1 * rinv2;
+        const float be = (q.rv * rinv2) * mr3;
+        acc[0] = fmaf(mr3, q.dx, acc[0]);
+        acc[1] = fmaf(mr3, q.dy, acc[1]);
+        acc[2] = fmaf(mr3, q.dz, acc[2]);
+        acc[3] += mr1;
+        acc[4] = fmaf(mr3, q.du, acc[4]);
+        acc[5] = fmaf(mr3, q.dv, acc[5]);
+        acc[6] = fmaf(mr3, q.dw, acc[6]);
+        acc[7] = fmaf(be, q.dx, acc[7]);
+        acc[8] = fmaf(be, q.dy, acc[8]);
+        acc[9] = fmaf(be, q.dz, acc[9]);
+    }
 }
 
 // One work item: i in [ibase, min(ibase + T*IPL, iend)) against sorted j in
@@ -166,12 +266,15 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
 
 #pragma unroll
     for (int k = 0; k < IPL; k++) {
-        const int i = ibase + k * T + tid;
-        const bool ok = i < iend;
-        L[k] = make_lane<MFLAG>(p.ib, p.ib_stride, i, ok, p.mmax);
+        const int i = ibase + k * T + tid;   // slot
+        const bool in = i < iend;
+        int r = i;
+        if (in && p.imap) r = p.imap[i];
+        const bool ok = in && r >= 0;
+        L[k] = make_lane<MFLAG>(p.ib, p.ib_stride, r, ok, p.mmax);
         nh[k] = 0;
         ns[k] = 0;
-        const size_t row = (size_t)chunk * p.ni + (ok ? i : 0);
+        const size_t row = (size_t)chunk * p.ni + (in ? i : 0);
         hp[k] = p.hslot + row * p.caph;
         sp[k] = p.sslot + row * p.caps;
 #pragma unroll
@@ -203,71 +306,38 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
         const int buf = t & 1;
         if (t + 1 < ntile) gload(t + 1);
         const int jt = j0 + t * T;
-        // i relative to the origin of the 256-j tile the LDS tile starts in
         IC c[IPL];
-#pragma unroll
-        for (int k = 0; k < IPL; k++) c[k] = tile_ic<MFLAG>(L[k], p.torg + (size_t)(jt / JTILE) * 8);
+        bool allfar = false;
         for (int js0 = 0; js0 < T; js0 += RF_FLUSH) {
-            if (T > JTILE && js0 > 0 && js0 % JTILE == 0) {
-#pragma unroll
-                for (int k = 0; k < IPL; k++)
-                    c[k] = tile_ic<MFLAG>(L[k], p.torg + (size_t)((jt + js0) / JTILE) * 8);
-            }
-#pragma unroll 2
-            for (int jj = js0; jj < js0 + RF_FLUSH; jj++) {
-                const float4 a = sA[buf][jj];
-                const float4 b = sB[buf][jj];
+            if (js0 == 0 || (T > JTILE && js0 % JTILE == 0)) {
+                // i relative to the origin of the 256-j tile this segment lies in,
+                // and whether the whole wavefront is outside its near field
+                const size_t tt = (size_t)((jt + js0) / JTILE);
+                bool far = p.far_test != 0;
 #pragma unroll
                 for (int k = 0; k < IPL; k++) {
-                    const PairBase q = pair_base(c[k], a, b);
-                    // v_rsq_f32 plus one Newton-Raphson step
-                    float rinv = __builtin_amdgcn_rsqf(q.r2);
-                    {
-                        const float h = rinv * rinv;
-                        const float e = fmaf(-q.r2, h, 1.0f);
-                        rinv = fmaf(rinv, 0.5f * e, rinv);
-                    }
-                    // cheap conservative bound on the soft test:
-                    // r2p >= r2 + 2 dtr min(rv, 0)  (dt2 carries the 1/(1-kappa) slack)
-                    const float t1 = fmaf(c[k].dt2, fminf(q.rv, 0.f), q.r2);
-                    const float thq = MFLAG ? fmaf(a.w, c[k].hQ, c[k].dQ) : c[k].hQ;
-                    if (t1 < thq) {
-                        // keep the full test inside the rare branch: the empty asm
-                        // makes dtr opaque here, so the compiler cannot hoist it
-                        float dtl = c[k].dt1;
-                        asm volatile("" : "+v"(dtl));
-                        const float ex = fmaf(dtl, q.du, q.dx);
-                        const float ey = fmaf(dtl, q.dv, q.dy);
-                        const float ez = fmaf(dtl, q.dw, q.dz);
-                        const float r2p = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
-                        const float mn = fminf(q.r2, fmaf(-KAPPA_P, q.r2, r2p));
-                        const float thh = MFLAG ? fmaf(a.w, c[k].hH, c[k].dH) : c[k].hH;
-                        const float ths = MFLAG ? fmaf(a.w, c[k].hS, c[k].dS) : c[k].hS;
-                        const int jg = jt + jj;
-                        if (mn < thh && jg < j1) {
-                            if (nh[k] < p.caph) hp[k][nh[k]] = jg;
-                            nh[k]++;
-                            rinv = 0.f;
-                        } else if (mn < ths && jg < j1 && ns[k] < p.caps) {
-                            sp[k][ns[k]] = jg;
-                            ns[k]++;
-                            rinv = 0.f;
-                        }
-                    }
-                    const float rinv2 = rinv * rinv;
-                    const float mr1 = a.w * rinv;
-                    const float mr3 = mr1 * rinv2;
-                    const float be = (q.rv * rinv2) * mr3;
-                    a32[k][0] = fmaf(mr3, q.dx, a32[k][0]);
-                    a32[k][1] = fmaf(mr3, q.dy, a32[k][1]);
-                    a32[k][2] = fmaf(mr3, q.dz, a32[k][2]);
-                    a32[k][3] += mr1;
-                    a32[k][4] = fmaf(mr3, q.du, a32[k][4]);
-                    a32[k][5] = fmaf(mr3, q.dv, a32[k][5]);
-                    a32[k][6] = fmaf(mr3, q.dw, a32[k][6]);
-                    a32[k][7] = fmaf(be, q.dx, a32[k][7]);
-                    a32[k][8] = fmaf(be, q.dy, a32[k][8]);
-                    a32[k][9] = fmaf(be, q.dz, a32[k][9]);
+                    c[k] = tile_ic<MFLAG>(L[k], p.torg + tt * 8);
+                    if (p.far_test) far = far && lane_far<MFLAG>(c[k], L[k].ok, p.tbox + tt * TBOX);
+                }
+                allfar = __ballot(!far) == 0ull;
+            }
+            if (allfar) {
+#pragma unroll 4
+                for (int jj = js0; jj < js0 + RF_FLUSH; jj++) {
+                    const float4 a = sA[buf][jj], b = sB[buf][jj];
+#pragma unroll
+                    for (int k = 0; k < IPL; k++)
+                        pair_step<MFLAG, false>(p, c[k], a, b, jt + jj, j1, nh[k], ns[k], hp[k],
+                                                sp[k], a32[k]);
+                }
+            } else {
+#pragma unroll 2
+                for (int jj = js0; jj < js0 + RF_FLUSH; jj++) {
+                    const float4 a = sA[buf][jj], b = sB[buf][jj];
+#pragma unroll
+                    for (int k = 0; k < IPL; k++)
+                        pair_step<MFLAG, true>(p, c[k], a, b, jt + jj, j1, nh[k], ns[k], hp[k],
+                                               sp[k], a32[k]);
                 }
             }
             // flush FP32 partials into FP64 (jerk = J1 - 3 J2)
@@ -305,8 +375,12 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
     }
 }
 
+// waves per SIMD the register budget is sized for: 5 at one i per lane (<= 96 VGPRs)
+template <int IPL>
+constexpr int regf_waves() { return IPL == 1 ? 5 : 2; }
+
 template <int T, int IPL, bool MFLAG>
-__global__ __launch_bounds__(T) void regf_tile_kernel(TileParams p) {
+__global__ __launch_bounds__(T, regf_waves<IPL>()) void regf_tile_kernel(TileParams p) {
     __shared__ float4 sA[2][T];   // x', y', z', m
     __shared__ float4 sB[2][T];   // vx', vy', vz', 0
     const int chunk = blockIdx.y;
@@ -318,7 +392,7 @@ __global__ __launch_bounds__(T) void regf_tile_kernel(TileParams p) {
 // Persistent form for a batch of independent clusters: each work-group takes the
 // next work item from a queue (items are ordered longest first) until none is left.
 template <int T, int IPL, bool MFLAG>
-__global__ __launch_bounds__(T) void regf_items_kernel(TileParams p,
+__global__ __launch_bounds__(T, regf_waves<IPL>()) void regf_items_kernel(TileParams p,
                                                        const int4 *__restrict__ items,
                                                        int nitems, int *__restrict__ queue) {
     __shared__ float4 sA[2][T];
@@ -337,7 +411,8 @@ __global__ __launch_bounds__(T) void regf_items_kernel(TileParams p,
 
 struct FinParams {
     const double *__restrict__ ib;   // SoA [8][ib_stride]
-    int ni, ib_stride;
+    int ni, ib_stride;               // ni = slot rows of the work arrays
+    const int *__restrict__ imap;    // slot -> caller row (nullptr: identity; < 0: empty)
     const double *__restrict__ rec64;   // sorted AoS [nj][8]
     const int *__restrict__ perm;       // sorted -> caller index
     const int *__restrict__ jbase;      // per i: index of its cluster's first j (or null)
@@ -419,8 +494,10 @@ __global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
     __shared__ int nbuf[4][NB_SORT_MAX];
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
-    const int i = blockIdx.x * 4 + wid;
+    const int i = blockIdx.x * 4 + wid;   // slot
     if (i >= f.ni) return;   // whole wavefront leaves together
+    const int r = f.imap ? f.imap[i] : i;   // caller row
+    if (r < 0) return;
     int *buf = nbuf[wid];
 
     // 1. FP64 sum of the chunk partials
@@ -436,9 +513,10 @@ __global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
     // 2. FP64 work on the candidates
     const int is = f.ib_stride;
     IF64 I;
-    I.x = f.ib[0 * is + i]; I.y = f.ib[1 * is + i]; I.z = f.ib[2 * is + i];
-    I.u = f.ib[3 * is + i]; I.v = f.ib[4 * is + i]; I.w = f.ib[5 * is + i];
-    I.h2 = f.ib[6 * is + i]; I.dtr = f.ib[7 * is + i];
+    I.x = f.ib[0 * is + r]; I.y = f.ib[1 * is + r]; I.z = f.ib[2 * is + r];
+    I.u = f.ib[3 * is + r]; I.v = f.ib[4 * is + r]; I.w = f.ib[5 * is + r];
+    I.h2 = f.ib[6 * is + r]; I.dtr = f.ib[7 * is + r];
+    const int jb = f.jbase ? f.jbase[r] : 0;
 
     int nnb = 0;          // exact neighbours found (wave-uniform)
     int dropped = 0;      // hard candidates beyond the slot capacity
@@ -463,7 +541,7 @@ __global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
                     const int sj = hs[k];
                     if ((unsigned)sj < (unsigned)f.nj) {
                         isnb = classify<MFLAG>(f.rec64 + (size_t)sj * 8, I, true, s);
-                        jo = f.perm[sj] - (f.jbase ? f.jbase[i] : 0);
+                        jo = f.perm[sj] - jb;
                     }
                 }
                 const unsigned long long b = __ballot(isnb);
@@ -482,10 +560,10 @@ __global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
 
 #pragma unroll
     for (int q = 0; q < 7; q++) s[q] = wave_sum(s[q]);
-    int *row = f.list + (size_t)i * f.lmax;
+    int *row = f.list + (size_t)r * f.lmax;
     const int tot = nnb + dropped;
     if (lane == 0) {
-        double *o = f.out + i;
+        double *o = f.out + r;
         const int os = f.out_stride;
         o[0 * os] = s[0]; o[1 * os] = s[1]; o[2 * os] = s[2];
         o[3 * os] = s[3]; o[4 * os] = s[4]; o[5 * os] = s[5];
@@ -561,6 +639,14 @@ RegfConfig &regf_config() {
     return c;
 }
 
+// 512-wide work-groups amortise the per-tile set-up best once there are enough
+// i-blocks to fill the machine; smaller i-sets keep more j-chunks with 256.
+RegfConfig regf_effective(int ni) {
+    RegfConfig c = regf_config();
+    if (c.threads == 0) c.threads = ni >= 65536 ? 512 : 256;
+    return c;
+}
+
 int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lmax,
                 int nnbmax, int m_flag, double *d_out, int *d_list, hipStream_t s) {
     if (ni <= 0) return OK;
@@ -572,7 +658,7 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
         set_error(E_ARG, "gpunb_regf: lmax outside [1, 2049]");
         return E_ARG;
     }
-    const RegfConfig &cfg = regf_config();
+    const RegfConfig cfg = regf_effective(ni);
     const int nbi = cfg.threads * cfg.ipl;
     const int nib = (ni + nbi - 1) / nbi;
     const int nj = js.n;
@@ -588,6 +674,12 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
     const int caph = nnbeff + 8;
     const int caps = CAP_SOFT;
 
+    // Morton-ordered i: a wavefront's particles are neighbours in space, so their
+    // candidates fall on the same j and most tiles are far from all of them
+    const int *imap = cfg.isort_min > 0 && ni >= cfg.isort_min
+                          ? jset_sort_i(js, ni, ni_stride, d_i, g_work, s)
+                          : nullptr;
+
     const size_t rows = (size_t)nchunk * ni;
     double *part = static_cast<double *>(g_work.part.get(rows * 7 * sizeof(double)));
     int *cnt = static_cast<int *>(g_work.cnt.get(rows * 2 * sizeof(int)));
@@ -596,8 +688,10 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
     g_last_chunks = nchunk;
 
     TileParams tp;
-    tp.ib = d_i; tp.ni = ni; tp.ib_stride = ni_stride;
+    tp.ib = d_i; tp.ni = ni; tp.ib_stride = ni_stride; tp.imap = imap;
     tp.rec32 = js.rec32.as<float4>(); tp.torg = js.torg.as<double>(); tp.nj = nj;
+    tp.tbox = js.tbox.as<float>();
+    tp.far_test = cfg.far_test;
     tp.chunk_len = clen;
     tp.mmax = js.mmax;
     tp.part = part; tp.cnt = cnt;
@@ -610,7 +704,7 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
     t1.stop(s);
 
     FinParams fp;
-    fp.ib = d_i; fp.ni = ni; fp.ib_stride = ni_stride;
+    fp.ib = d_i; fp.ni = ni; fp.ib_stride = ni_stride; fp.imap = imap;
     fp.rec64 = js.rec64.as<double>(); fp.perm = js.perm.as<int>(); fp.jbase = nullptr; fp.nj = nj;
     fp.nchunk = nchunk;
     fp.part = part; fp.cnt = cnt;
@@ -630,19 +724,20 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
 }
 
 // Batched form (ensemble.hip): one persistent launch over work items, nchunk = 1,
-// rows indexed by the particle's position in the concatenated i-array.
-int regf_items_launch(const JSet &js, int ntot, int stride, const double *d_i, const int4 *items,
-                      int nitems, int *queue, const int *jbase, int lmax, int nnbmax, int m_flag,
-                      double *d_out, int *d_list, int grid, RegfWork &w, hipStream_t s) {
+// work rows indexed by slot, caller rows by imap[slot].
+int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, const double *d_i,
+                      const int4 *items, int nitems, int *queue, const int *jbase, int lmax,
+                      int nnbmax, int m_flag, double *d_out, int *d_list, int grid, RegfWork &w,
+                      hipStream_t s) {
     if (lmax < 1 || lmax > NB_SORT_MAX + 1) {
         set_error(E_ARG, "nb6_ens_regf_device: lmax outside [1, 2049]");
         return E_ARG;
     }
-    const RegfConfig &cfg = regf_config();
+    const RegfConfig cfg = regf_effective(0);   // clusters are small: 256 unless set
     const int nnbeff = std::max(0, std::min(nnbmax, lmax - 1));
     const int caph = nnbeff + 8;
     const int caps = CAP_SOFT;
-    const size_t rows = (size_t)ntot;
+    const size_t rows = (size_t)nslots;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     NB6_HIP(hipStreamIsCapturing(s, &cs));
     const bool capturing = cs != hipStreamCaptureStatusNone;
@@ -656,8 +751,10 @@ int regf_items_launch(const JSet &js, int ntot, int stride, const double *d_i, c
     int *hslot = static_cast<int *>(w.hslot.get(rows * caph * sizeof(int)));
     int *sslot = static_cast<int *>(w.sslot.get(rows * caps * sizeof(int)));
     TileParams tp;
-    tp.ib = d_i; tp.ni = ntot; tp.ib_stride = stride;
+    tp.ib = d_i; tp.ni = nslots; tp.ib_stride = stride; tp.imap = imap;
     tp.rec32 = js.rec32.as<float4>(); tp.torg = js.torg.as<double>(); tp.nj = js.n;
+    tp.tbox = js.tbox.as<float>();
+    tp.far_test = cfg.far_test;
     tp.chunk_len = js.n;
     tp.mmax = js.mmax;
     tp.part = part; tp.cnt = cnt;
@@ -668,14 +765,14 @@ int regf_items_launch(const JSet &js, int ntot, int stride, const double *d_i, c
     NB6_HIP(hipGetLastError());
     if (!capturing) t1.stop(s);
     FinParams fp;
-    fp.ib = d_i; fp.ni = ntot; fp.ib_stride = stride;
+    fp.ib = d_i; fp.ni = nslots; fp.ib_stride = stride; fp.imap = imap;
     fp.rec64 = js.rec64.as<double>(); fp.perm = js.perm.as<int>(); fp.jbase = jbase; fp.nj = js.n;
     fp.nchunk = 1;
     fp.part = part; fp.cnt = cnt;
     fp.hslot = hslot; fp.caph = caph; fp.sslot = sslot; fp.caps = caps;
     fp.lmax = lmax; fp.nnbmax = nnbeff;
     fp.out = d_out; fp.out_stride = stride; fp.list = d_list;
-    const dim3 fgrid((ntot + 3) / 4);
+    const dim3 fgrid((nslots + 3) / 4);
     if (m_flag)
         hipLaunchKernelGGL(regf_finalize_kernel<true>, fgrid, dim3(256), 0, s, fp);
     else

@@ -69,6 +69,7 @@ def load():
     L.nb6_pot_device.argtypes = [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int,
                                  _vp, _vp]
     L.nb6_set_regf_config.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.nb6_set_regf_tuning.argtypes = [ctypes.c_int, ctypes.c_int]
     L.nb6_device_copy.argtypes = [_vp, _vp, ctypes.c_ulonglong, _vp]
     L.nb6_ens_send_device.argtypes = [ctypes.c_int, _ip, _vp, ctypes.c_int, _vp]
     L.nb6_ens_regf_device.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,

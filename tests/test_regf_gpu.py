@@ -64,7 +64,7 @@ def test_regf_plummer_1k_all_i(gpunb, threads, ipl):
     g.send(m, x, v)
     got = g.regf(h2, dtr, x, v, lmax, nnbmax)
     ref = O.regf(m, x, v, h2, dtr, x, v, lmax, nnbmax)
-    lib.load().nb6_set_regf_config(256, 1, 2048)
+    lib.load().nb6_set_regf_config(0, 1, 2048)
     _compare(got, ref, f"C1 threads={threads} ipl={ipl}")
     # the self pair is always a neighbour (r2 = 0 < h2)
     for i in range(n):
@@ -83,6 +83,43 @@ def test_regf_kroupa_64k_subsets(gpunb, ni, m_flag):
     got = g.regf(h2, dtr, x[idx], v[idx], lmax, nnbmax, m_flag)
     ref = O.regf(m, x, v, h2, dtr, x[idx], v[idx], lmax, nnbmax, m_flag)
     _compare(got, ref, f"C2 ni={ni} m_flag={m_flag}")
+
+
+@pytest.mark.parametrize("threads,ipl", [(128, 1), (256, 1), (256, 2), (512, 1)])
+@pytest.mark.parametrize("m_flag", [0, 1])
+def test_regf_fast_movers_and_outsiders(gpunb, threads, ipl, m_flag):
+    """Neighbours reached only through the velocity criterion (|x + dtr v| inside RS,
+    x far outside) and i-particles outside the j-set's box: the per-tile far test
+    must never skip a tile that holds a candidate.  ni >= 2048, so the i-set is
+    Morton-ordered as well."""
+    m, x, v, idx, h2, dtr = c2_inputs(n=16384, ni=4096, m_flag=m_flag, counts_median=40.0)
+    rng = np.random.Generator(np.random.PCG64(515))
+    xi, vi = x[idx].copy(), v[idx].copy()
+    dtr = dtr.copy()
+    fast = rng.random(idx.size) < 0.25
+    d = rng.normal(size=(idx.size, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    dtr[fast] = 0.05
+    vi[fast] += 20.0 * d[fast]                      # moves ~1 length unit within dtr
+    out = (rng.random(idx.size) < 0.05) & ~fast
+    xi[out] += 3.0 * d[out] + np.array([5.0, 0.0, 0.0])
+    vi[out] -= (3.0 * d[out] + np.array([5.0, 0.0, 0.0])) / 0.0625   # comes back at dtr
+    dtr[out] = 0.0625
+    n = m.size
+    lmax, nnbmax = LMAX_SMALL + 100, nnbmax_for(n, LMAX_SMALL + 100)
+    lib.load().nb6_set_regf_config(threads, ipl, 2048)
+    try:
+        g = gpunb(n)
+        g.send(m, x, v)
+        got = g.regf(h2, dtr, xi, vi, lmax, nnbmax, m_flag)
+    finally:
+        lib.load().nb6_set_regf_config(0, 1, 2048)
+    ref = O.regf(m, x, v, h2, dtr, xi, vi, lmax, nnbmax, m_flag)
+    # the velocity criterion must actually matter in this case
+    ref0 = O.regf(m, x, v, h2, np.zeros_like(dtr), xi, vi, lmax, nnbmax, m_flag)
+    assert (ref[3][fast, 0] != ref0[3][fast, 0]).mean() > 0.5
+    assert (ref[3][out, 0] != 0).all()           # the self pair meets at dtr
+    _compare(got, ref, f"fast movers threads={threads} ipl={ipl} m_flag={m_flag}")
 
 
 def test_regf_overflow_exact_negative_counts(gpunb):
