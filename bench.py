@@ -239,8 +239,8 @@ def main():
         print(json.dumps({"copy_bytes": nb}), flush=True)
         return 0
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              f"(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: measures one GPU only; --gpus {args.gpus} is not supported "
+              f"(sharding across GPUs is not implemented)", file=sys.stderr)
         return 2
 
     if args.workload == "ensemble":

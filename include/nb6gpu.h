@@ -172,6 +172,67 @@ int nb6_ens_items(void);
  * kernel; used to calibrate HBM byte counters against a known size. */
 int nb6_device_copy(void *dst, const void *src, unsigned long long bytes, void *stream);
 
+/* ---------------------------------------------------------------------------
+ * Part 3 (extension): learned neighbour radius.  A network 8 -> 64 -> 64 -> 1
+ * (ReLU; bfloat16 weights, FP32 accumulation; 4801 parameters, layout W1[64][8],
+ * b1[64], W2[64][64], b2[64], w3[64], b3) predicts y = ln(RS_next / RS_prev)
+ * from eight features per particle.  The reference's rule (regcor_gpu.F:95-238)
+ * stays available on the device as the fallback.  All arrays are device
+ * pointers; FP32 feature arrays are SoA float[8][n].
+ * ------------------------------------------------------------------------- */
+int nb6_rsnet_nparam(void);
+/* Allocate (once) and initialise: He-uniform weights from a SplitMix64 stream of
+ * `seed`, zero biases, zero Adam moments, identity normalisation. */
+int nb6_rsnet_open(unsigned long long seed);
+int nb6_rsnet_close(void);
+/* Host copies of the FP32 master weights (4801 floats); set also refreshes the
+ * bfloat16 copy used by the forward pass. */
+int nb6_rsnet_set_params(const float *params);
+int nb6_rsnet_get_params(float *params);
+/* Feature normalisation: x' = (x - mu) / sigma, mu[8], sigma[8] on the host. */
+int nb6_rsnet_set_norm(const float *mu, const float *sigma);
+/* y[n] = network(features) */
+int nb6_rsnet_forward_device(int n, const float *d_feat, float *d_y, void *stream);
+/* Gradient of mean((y - t)^2) with respect to the (bfloat16-rounded) weights:
+ * d_grad float[4801], d_loss float[1] (device). */
+int nb6_rsnet_grad_device(int n, const float *d_feat, const float *d_t, float *d_grad,
+                          float *d_loss, void *stream);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the minibatch; *h_loss (host,
+ * may be null) receives the minibatch loss before the step. */
+int nb6_rsnet_train_step_device(int n, const float *d_feat, const float *d_t, float lr,
+                                float *h_loss, void *stream);
+/* Features of n particles (SoA double arrays with `stride`; x, acc, jrk [3][stride]):
+ * log10 of (nnb+1)/(4/3 pi RS^3), nnb+1, |acc|, |jrk|, |x - centre|, RS, dtr, mass;
+ * nnb = previous neighbour count (int[n]), centre = density centre (host, 3). */
+int nb6_rs_features_device(int n, int stride, const double *d_x, const double *d_acc,
+                           const double *d_jrk, const double *d_mass, const double *d_rs,
+                           const double *d_dtr, const int *d_nnb, const double *centre,
+                           float *d_feat, void *stream);
+/* Inference fused ahead of gpunb_regf: rs_out = use_rule[i] ? rs_rule[i] :
+ * rs_prev[i] exp(y_i) (y clamped to [-4, 4]); when d_ib is given, row 6 of that
+ * regf i-block (double[8][ib_stride]) receives h2 = RS^2 (RS^2 / bodym if m_flag).
+ * d_rs_rule / d_use_rule may be null (network everywhere). */
+int nb6_rs_predict_device(int n, const float *d_feat, const double *d_rs_prev,
+                          const double *d_rs_rule, const int *d_use_rule, double *d_rs_out,
+                          double *d_ib, int ib_stride, int m_flag, double bodym, void *stream);
+/* The reference's RS update after a regular step, regcor_gpu.F:95-238 (single
+ * stars, NB_FLAG = 1, M_FLAG = 0; the c.m.-body branch and CHECKL are not part of
+ * it): x, v [3][stride]; dtr; nnb = new count without self, nnb0 = previous count;
+ * list int[n][lmax] (0-based j without self, used for nnb <= 3, may be null) into
+ * xj, vj [3][jstride]; glob = {rdens x, y, z, RSCALE, RC^2, RSFAC, ZNBMIN} (host).
+ * d_rs is updated in place. */
+int nb6_rs_rule_device(int n, int stride, const double *d_x, const double *d_v,
+                       const double *d_dtr, const int *d_nnb, const int *d_nnb0,
+                       const int *d_list, int lmax, const double *d_xj, const double *d_vj,
+                       int jstride, const double *glob, int nnbopt, int nnbmax, double *d_rs,
+                       void *stream);
+/* Training target: for each i of a regf i-block (double[8][ib_stride]) and its
+ * list from nb6_regf_device against the currently sent j-set, the radius giving
+ * exactly k neighbours under min(r^2, |dx + dtr dv|^2) (FP64).  Rows with fewer
+ * than k entries (or overflow) get -(count) - 1. */
+int nb6_rs_kth_device(int ni, int ib_stride, const double *d_ib, const int *d_list, int lmax,
+                      int k, double *d_rs, void *stream);
+
 /* Device allocation / copies for callers without another GPU runtime
  * (tests, benchmarks). */
 void *nb6_dev_alloc(unsigned long long bytes);

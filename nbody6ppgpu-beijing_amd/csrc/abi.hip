@@ -196,6 +196,8 @@ static void par_for(int n, F f) {
     for (auto &x : th) x.join();
 }
 
+const JSet *gpunb_jset() { return g_nb.open ? &g_nb.js : nullptr; }
+
 }  // namespace nb6
 
 using namespace nb6;
@@ -526,6 +528,71 @@ int nb6_device_copy(void *dst, const void *src, unsigned long long bytes, void *
     return OK;
 }
 int nb6_last_regf_chunks(void) { return regf_last_chunks(); }
+
+static hipStream_t pick(void *stream) {
+    return stream ? static_cast<hipStream_t>(stream) : device().stream;
+}
+
+int nb6_rsnet_nparam(void) { return rsnet_nparam(); }
+int nb6_rsnet_open(unsigned long long seed) { return rsnet_open(seed); }
+int nb6_rsnet_close(void) { return rsnet_close(); }
+int nb6_rsnet_set_params(const float *params) { return rsnet_set_params(params); }
+int nb6_rsnet_get_params(float *params) { return rsnet_get_params(params); }
+int nb6_rsnet_set_norm(const float *mu, const float *sigma) { return rsnet_set_norm(mu, sigma); }
+int nb6_rsnet_forward_device(int n, const float *d_feat, float *d_y, void *stream) {
+    if (!device().ready) return E_NODEV;
+    return rsnet_forward(n, d_feat, d_y, pick(stream));
+}
+int nb6_rsnet_grad_device(int n, const float *d_feat, const float *d_t, float *d_grad,
+                          float *d_loss, void *stream) {
+    if (!device().ready) return E_NODEV;
+    return rsnet_grad(n, d_feat, d_t, d_grad, d_loss, pick(stream));
+}
+int nb6_rsnet_train_step_device(int n, const float *d_feat, const float *d_t, float lr,
+                                float *h_loss, void *stream) {
+    if (!device().ready) return E_NODEV;
+    return rsnet_train_step(n, d_feat, d_t, lr, h_loss, pick(stream));
+}
+int nb6_rs_features_device(int n, int stride, const double *d_x, const double *d_acc,
+                           const double *d_jrk, const double *d_mass, const double *d_rs,
+                           const double *d_dtr, const int *d_nnb, const double *centre,
+                           float *d_feat, void *stream) {
+    if (!device().ready) return E_NODEV;
+    if (stride < n) { set_error(E_ARG, "nb6_rs_features_device: stride below n"); return E_ARG; }
+    return rs_features(n, stride, d_x, d_acc, d_jrk, d_mass, d_rs, d_dtr, d_nnb, centre, d_feat,
+                       pick(stream));
+}
+int nb6_rs_predict_device(int n, const float *d_feat, const double *d_rs_prev,
+                          const double *d_rs_rule, const int *d_use_rule, double *d_rs_out,
+                          double *d_ib, int ib_stride, int m_flag, double bodym, void *stream) {
+    if (!device().ready) return E_NODEV;
+    if (d_use_rule && !d_rs_rule) {
+        set_error(E_ARG, "nb6_rs_predict_device: use_rule given without rs_rule");
+        return E_ARG;
+    }
+    return rs_predict(n, d_feat, d_rs_prev, d_rs_rule, d_use_rule, d_rs_out, d_ib, ib_stride,
+                      m_flag, bodym, pick(stream));
+}
+int nb6_rs_rule_device(int n, int stride, const double *d_x, const double *d_v,
+                       const double *d_dtr, const int *d_nnb, const int *d_nnb0,
+                       const int *d_list, int lmax, const double *d_xj, const double *d_vj,
+                       int jstride, const double *glob, int nnbopt, int nnbmax, double *d_rs,
+                       void *stream) {
+    if (!device().ready) return E_NODEV;
+    if (stride < n || (d_list && (lmax < 2 || !d_xj || !d_vj))) {
+        set_error(E_ARG, "nb6_rs_rule_device: bad stride or list arguments");
+        return E_ARG;
+    }
+    return rs_rule(n, stride, d_x, d_v, d_dtr, d_nnb, d_nnb0, d_list, lmax, d_xj, d_vj, jstride, glob,
+                   nnbopt, nnbmax, d_rs, pick(stream));
+}
+int nb6_rs_kth_device(int ni, int ib_stride, const double *d_ib, const int *d_list, int lmax,
+                      int k, double *d_rs, void *stream) {
+    const JSet *js = nb6::gpunb_jset();
+    if (!js) { set_error(E_STATE, "nb6_rs_kth_device: library not open"); return E_STATE; }
+    if (ib_stride < ni) { set_error(E_ARG, "nb6_rs_kth_device: stride below ni"); return E_ARG; }
+    return rs_kth(*js, ni, ib_stride, d_ib, d_list, lmax, k, d_rs, pick(stream));
+}
 
 void *nb6_dev_alloc(unsigned long long bytes) {
     if (!device().ready) return nullptr;

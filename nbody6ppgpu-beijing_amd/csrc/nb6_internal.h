@@ -138,6 +138,32 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
 // streaming copy of bytes (multiple of 16) on stream s
 void device_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 
+// ---------------------------------------------------------------- learned RS (rsnet.hip)
+int rsnet_nparam();
+int rsnet_open(unsigned long long seed);
+int rsnet_close();
+int rsnet_set_params(const float *p);
+int rsnet_get_params(float *p);
+int rsnet_set_norm(const float *mu, const float *sigma);
+int rsnet_forward(int n, const float *d_feat, float *d_y, hipStream_t s);
+int rsnet_grad(int n, const float *d_feat, const float *d_t, float *d_grad, float *d_loss,
+               hipStream_t s);
+int rsnet_train_step(int n, const float *d_feat, const float *d_t, float lr, float *h_loss,
+                     hipStream_t s);
+int rs_features(int n, int stride, const double *x, const double *acc, const double *jrk,
+                const double *mass, const double *rs, const double *dtr, const int *nnb,
+                const double *centre, float *feat, hipStream_t s);
+int rs_predict(int n, const float *feat, const double *rs_prev, const double *rs_rule,
+               const int *use_rule, double *rs_out, double *ib, int ib_stride, int m_flag,
+               double bodym, hipStream_t s);
+int rs_rule(int n, int stride, const double *x, const double *v, const double *dtr, const int *nnb,
+            const int *nnb0, const int *list, int lmax, const double *xj, const double *vj,
+            int jstride, const double *glob, int nnbopt, int nnbmax, double *rs, hipStream_t s);
+int rs_kth(const JSet &js, int ni, int ib_stride, const double *ib, const int *list, int lmax, int k,
+           double *rs_out, hipStream_t s);
+// the j-set of the open gpunb library (nullptr when closed)
+const JSet *gpunb_jset();
+
 // ---------------------------------------------------------------- pot
 int pot_launch(int n, int n_stride, const double *d_x, int i0, int ni, double *d_pot,
                hipStream_t s);

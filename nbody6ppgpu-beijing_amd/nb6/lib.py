@@ -74,6 +74,19 @@ def load():
     L.nb6_ens_send_device.argtypes = [ctypes.c_int, _ip, _vp, ctypes.c_int, _vp]
     L.nb6_ens_regf_device.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       _vp, _vp, ctypes.c_int, _vp]
+    i, d, f, u = ctypes.c_int, ctypes.c_double, ctypes.c_float, ctypes.c_ulonglong
+    L.nb6_rsnet_open.argtypes = [u]
+    L.nb6_rsnet_set_params.argtypes = [_vp]
+    L.nb6_rsnet_get_params.argtypes = [_vp]
+    L.nb6_rsnet_set_norm.argtypes = [_vp, _vp]
+    L.nb6_rsnet_forward_device.argtypes = [i, _vp, _vp, _vp]
+    L.nb6_rsnet_grad_device.argtypes = [i, _vp, _vp, _vp, _vp, _vp]
+    L.nb6_rsnet_train_step_device.argtypes = [i, _vp, _vp, f, _vp, _vp]
+    L.nb6_rs_features_device.argtypes = [i, i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.nb6_rs_predict_device.argtypes = [i, _vp, _vp, _vp, _vp, _vp, _vp, i, i, d, _vp]
+    L.nb6_rs_rule_device.argtypes = [i, i, _vp, _vp, _vp, _vp, _vp, _vp, i, _vp, _vp, i, _vp, i, i,
+                                     _vp, _vp]
+    L.nb6_rs_kth_device.argtypes = [i, i, _vp, _vp, i, i, _vp, _vp]
     _L = L
     return L
 

@@ -10,7 +10,7 @@ this module.  It wraps
   loaded RTLD_LOCAL so that its gpunb_*/gpupot_/irr_simd_* symbols never clash
   with the product library's.
 
-Padding rules of the reference AVX code are applied here (SURVEY hard part 8):
+Padding rules of the reference AVX code are applied here:
 reg.avx.cpp reads and writes i-arrays up to the next multiple of 4
 (reg.avx.cpp:204-226,308-314); pot.avx.cpp writes 8 at a time from pot+istart-1
 (pot.avx.cpp:87-137), so it is always called with istart=1 on padded buffers.
