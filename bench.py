@@ -142,10 +142,21 @@ def bench_ensemble(steps, warmup):
     dl = lib.DeviceArray(ntot * lmax * 4)
     pairs = float((sizes.astype(float) ** 2).sum())
 
-    # batched: send + one graph-replayed regular step per ensemble step
+    # batched: send + one graph-replayed regular step per ensemble step, in the
+    # configuration the autotuner picks on this ensemble
     L.nb6_ens_open(nc, ntot)
     L.nb6_ens_items.restype = ctypes.c_int
     offp = off.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    if L.nb6_ens_send_device(nc, offp, dj.ptr, ntot, None) != 0:
+        lib.check_error()
+    best = (ctypes.c_int * 4)()
+    t_tune = time.perf_counter()
+    if L.nb6_ens_autotune(di.ptr, ntot, lmax, nnbmax, 0, do.ptr, dl.ptr, best, None) != 0:
+        lib.check_error()
+    t_tune = time.perf_counter() - t_tune
+    tuned = {"threads": best[0], "i_per_lane": best[1], "workgroups_per_cu": best[2],
+             "step_us": best[3], "tuning_s": round(t_tune, 2)}
+    log(f"ensemble autotune: {tuned}")
 
     def ens_step():
         if L.nb6_ens_send_device(nc, offp, dj.ptr, ntot, None) != 0:
@@ -169,6 +180,7 @@ def bench_ensemble(steps, warmup):
     out_b = do.to_host(np.float64, (7, ntot))
     lst_b = dl.to_host(np.int32, (ntot, lmax))
     L.nb6_ens_close()
+    L.nb6_set_regf_config(0, 1, 2048)      # the looped path runs the default configuration
 
     # looped: the plain path, one cluster after the other
     g = lib.Gpunb(int(sizes.max()))
@@ -210,12 +222,184 @@ def bench_ensemble(steps, warmup):
                        "pairs_per_step": pairs, "lmax": lmax, "nnbmax": nnbmax},
             "batched": {"clusters_per_s": nc / t_ens, "ms_per_step": 1e3 * t_ens,
                         "regf_only_ms": 1e3 * t_ens_regf, "pairs_per_s": pairs / t_ens,
-                        "work_items": L.nb6_ens_items()},
+                        "work_items": L.nb6_ens_items(), "autotuned": tuned},
             "looped": {"clusters_per_s": nc / t_loop, "ms_per_step": 1e3 * t_loop,
                        "pairs_per_s": pairs / t_loop},
             "speedup": t_loop / t_ens,
             "batched_vs_looped": {"acc_max_rel": float(da.max()), "pot_max_rel": float(dp.max()),
                                   "lists_identical": same_lists}}
+
+
+def bench_learned_rs(train_samples, eval_steps, no_cpu):
+    """Learned neighbour radius: King W0=7, N=262144, NNBOPT=100, LMAX=600.
+    Pairs (features, radius giving NNBOPT neighbours) come from regular block steps
+    of one realisation under the reference rule; the network trains on the GPU;
+    then a second realisation runs the same number of block steps under the rule
+    and under the network (rule as fallback), counting the util_gpu.F overflow
+    re-runs of gpunb_regf."""
+    from nb6.rsdrive import RSNet, Sim
+    n, nnbopt, lmax = 262144, 100, 600
+    t = time.time()
+    m, x, v = ics.king(n, SEED + 50, 7.0, masses=ics.kroupa_masses(n, SEED + 51))
+    rec = []
+    s = Sim(m, x, v, nnbopt=nnbopt, lmax=lmax)
+    s.start()
+    nsteps = 0
+    while sum(r[1].size for r in rec) < train_samples:
+        s.block_step("rule", record=rec)
+        nsteps += 1
+        if nsteps % 50 == 0:
+            log(f"training data: {nsteps} block steps, {sum(r[1].size for r in rec)} pairs, "
+                f"t = {s.t:.4f}, {time.time() - t:.0f} s")
+    s.close()
+    feat = np.concatenate([r[0] for r in rec])
+    tgt = np.concatenate([r[1] for r in rec])
+    t_data = time.time() - t
+    t = time.time()
+    net = RSNet(seed=SEED)
+    hist = net.train(feat, tgt, epochs=4, batch=8192, lr=2e-3, log=log)
+    t_train = time.time() - t
+    m2, x2, v2 = ics.king(n, SEED + 60, 7.0, masses=ics.kroupa_masses(n, SEED + 61))
+    res = {}
+    for pol in ("rule", "net"):
+        t = time.time()
+        s = Sim(m2, x2, v2, nnbopt=nnbopt, lmax=lmax)
+        s.start()
+        s.run(eval_steps, pol, net=net if pol == "net" else None)
+        st = dict(s.stats)
+        st["wall_s"] = time.time() - t
+        st["mean_abs_ln_count_ratio"] = st["absdev"] / max(st["samples"], 1)
+        st["miss30_frac"] = st["miss30"] / max(st["samples"], 1)
+        st["sim_time"] = s.t
+        if not no_cpu:
+            st["force_accuracy_vs_fp64"] = _rs_accuracy(s)
+        s.close()
+        res[pol] = st
+        log(f"{pol}: {st}")
+    net.close()
+    r, q = res["rule"], res["net"]
+    ratio = r["overflow_events"] / max(q["overflow_events"], 1)
+    return {"metric": "neighbour-list overflow retries: reference RS rule / learned RS "
+                      "(King W0=7 N=262144, NNBOPT=100)",
+            "value": ratio, "unit": "x fewer gpunb_regf re-runs", "n_gpus": 1,
+            "higher_is_better": True, "dtype": "bf16 weights / fp32 accumulate",
+            "data": "synthetic (seeded King W0=7, Kroupa masses)",
+            "config": {"workload": "learned-rs", "n": n, "nnbopt": nnbopt, "lmax": lmax,
+                       "nnbmax": lmax - 50, "eval_block_steps": eval_steps},
+            "training": {"pairs": int(tgt.size), "block_steps": nsteps, "data_s": t_data,
+                         "train_s": t_train, "loss": hist, "target_var": float(tgt.var())},
+            "rule": r, "learned": q}
+
+
+def bench_precision(no_cpu):
+    """Precision study at N=1M (King W0=6, 50% binaries, Kroupa): block i-sets with
+    sizes drawn from p(n) ~ 1/n on [1, 2^17]; each block through the FP32 production
+    kernel, the float-float kernel and the FP64 kernel.  Errors of acc, jerk, pot
+    against the FP64 kernel on every particle of the block (the FP64 kernel is checked
+    against the CPU oracle on a sample), list agreement, and throughput per mode;
+    plus gpupot over all N against the FP64 kernel's all-pairs potential."""
+    m, x, v, idx_all, h2_all, dtr_all = make_workload()
+    L = lib.load()
+    nj = m.size
+    rng = np.random.Generator(np.random.PCG64(SEED + 70))
+    sizes = np.unique(np.exp(rng.uniform(0.0, np.log(2.0 ** 17), 6)).astype(int).clip(1, 2 ** 17))
+    sizes = np.r_[sizes, 2 ** 17]
+    g = lib.Gpunb(nj)
+    dj = lib.DeviceArray.from_host(np.ascontiguousarray(np.concatenate([x.T, v.T, m[None]])))
+    if L.nb6_send_device(nj, nj, dj.ptr, None) != 0:
+        lib.check_error()
+    nmax = int(sizes.max())
+    di = lib.DeviceArray(8 * nmax * 8)
+    do = lib.DeviceArray(7 * nmax * 8)
+    dl = lib.DeviceArray(nmax * LMAX * 4)
+    names = {0: "fp32_fp64acc", 1: "float_float", 2: "fp64"}
+    blocks = []
+    for ni in sizes:
+        sel = np.sort(rng.choice(idx_all.size, int(ni), replace=False))
+        ib = np.ascontiguousarray(np.concatenate([x[idx_all[sel]].T, v[idx_all[sel]].T,
+                                                  h2_all[sel][None], dtr_all[sel][None]]))
+        L.nb6_memcpy_htod(di.ptr, ib.ctypes.data, ib.nbytes)
+        res = {}
+        for mode in (2, 0, 1):
+            L.nb6_set_regf_precision(mode)
+            if L.nb6_regf_device(int(ni), int(ni), di.ptr, LMAX, NNBMAX, 0, do.ptr, dl.ptr, None) != 0:
+                lib.check_error()
+            L.nb6_sync()
+            ms = L.nb6_kernel_ms(b"regf_tile") + L.nb6_kernel_ms(b"regf_finalize")
+            res[mode] = (do.to_host(np.float64, (7, int(ni))), dl.to_host(np.int32, (int(ni), LMAX)), ms)
+        L.nb6_set_regf_precision(0)
+        ref_out, ref_lst, _ = res[2]
+        row = {"ni": int(ni)}
+        for mode in (0, 1, 2):
+            out, lst, ms = res[mode]
+            ea = np.linalg.norm(out[0:3] - ref_out[0:3], axis=0) / np.linalg.norm(ref_out[0:3], axis=0)
+            ej = np.linalg.norm(out[3:6] - ref_out[3:6], axis=0) / np.linalg.norm(ref_out[3:6], axis=0)
+            ep = np.abs(out[6] - ref_out[6]) / ref_out[6]
+            row[names[mode]] = {"pairs_per_s": float(ni) * nj / (ms * 1e-3), "ms": ms,
+                                "acc_max": float(ea.max()), "jrk_p99": float(np.percentile(ej, 99)),
+                                "jrk_max": float(ej.max()), "pot_max": float(ep.max()),
+                                "lists_equal_fp64": bool(np.array_equal(lst[:, 0], ref_lst[:, 0]) and
+                                                         all(np.array_equal(lst[i, 1:1 + max(lst[i, 0], 0)],
+                                                                            ref_lst[i, 1:1 + max(lst[i, 0], 0)])
+                                                             for i in range(0, int(ni), max(1, int(ni) // 512))))}
+        if not no_cpu and ni == sizes[0]:
+            # checker: the FP64 kernel against the CPU oracle on this small block
+            import oracle as O
+            k = min(int(ni), 64)
+            racc, rjrk, rpot, _ = O.regf(m, x, v, h2_all[sel][:k], dtr_all[sel][:k], x[idx_all[sel][:k]],
+                                         v[idx_all[sel][:k]], LMAX, NNBMAX)
+            e = np.linalg.norm(ref_out[0:3, :k].T - racc, axis=1) / np.linalg.norm(racc, axis=1)
+            row["fp64_kernel_vs_cpu_oracle_acc_max"] = float(e.max())
+        blocks.append(row)
+        log(f"precision block ni={ni}: " + ", ".join(
+            f"{names[md]} {row[names[md]]['pairs_per_s']:.3e} pairs/s acc {row[names[md]]['acc_max']:.1e} "
+            f"jrk99 {row[names[md]]['jrk_p99']:.1e}" for md in (0, 1)))
+    # gpupot over all N against the FP64 kernel's all-pairs potential (h2 = 0)
+    dx = lib.DeviceArray.from_host(np.ascontiguousarray(np.concatenate([x.T, m[None]])))
+    dp = lib.DeviceArray(nj * 8)
+    if L.nb6_pot_device(nj, nj, dx.ptr, 0, nj, dp.ptr, None) != 0:
+        lib.check_error()
+    L.nb6_sync()
+    pot_ms = L.nb6_kernel_ms(b"pot")
+    pot = dp.to_host(np.float64, nj)
+    ks = np.arange(0, nj, nj // 4096)
+    ib = np.ascontiguousarray(np.concatenate([x[ks].T, v[ks].T, np.zeros((2, ks.size))]))
+    L.nb6_memcpy_htod(di.ptr, ib.ctypes.data, ib.nbytes)
+    L.nb6_set_regf_precision(2)
+    if L.nb6_regf_device(ks.size, ks.size, di.ptr, 2, 1, 0, do.ptr, dl.ptr, None) != 0:
+        lib.check_error()
+    L.nb6_set_regf_precision(0)
+    ref_pot = do.to_host(np.float64, (7, ks.size))[6]
+    ep = np.abs(pot[ks] - ref_pot) / ref_pot
+    for d in (dj, di, do, dl, dx, dp):
+        d.free()
+    g.close()
+    return {"metric": "precision study: regf pair-interactions/s and error vs the FP64 kernel "
+                      "(King W0=6 N=1M, 50% binaries)",
+            "value": blocks[-1]["fp32_fp64acc"]["pairs_per_s"], "unit": "pair-interactions/s",
+            "n_gpus": 1, "higher_is_better": True, "dtype": "fp32 / float-float / fp64",
+            "data": "synthetic (seeded King W0=6, 50% binaries, Kroupa masses)",
+            "config": {"workload": "precision", "n_j": nj, "block_sizes": [int(s) for s in sizes]},
+            "blocks": blocks,
+            "gpupot_all_n": {"n": nj, "ms": pot_ms, "pairs_per_s": float(nj) * nj / (pot_ms * 1e-3),
+                             "sample": int(ks.size), "max_rel_err_vs_fp64": float(ep.max()),
+                             "median_rel_err_vs_fp64": float(np.median(ep))}}
+
+
+def _rs_accuracy(s, k=128):
+    """Regular force of k particles with the policy's current RS against FP64 (checker)."""
+    import oracle as O
+    idx = np.arange(0, s.n, s.n // k)[:k]
+    xp, vp = s.predict(s.t)
+    h2 = s.rs[idx] ** 2
+    dtr = s.dt[idx]
+    s._send(xp, vp)
+    out, rows = s._regf(xp[idx], vp[idx], h2, dtr, s.d.lst, s.lmax, s.nnbmax)
+    racc, rjrk, rpot, rlst = O.regf(s.m, xp, vp, h2, dtr, xp[idx], vp[idx], s.lmax, s.nnbmax)
+    ea = np.linalg.norm(out[0:3].T - racc, axis=1) / np.linalg.norm(racc, axis=1)
+    ep = np.abs(out[6] - rpot) / rpot
+    return {"acc_max": float(ea.max()), "pot_max": float(ep.max()),
+            "counts_identical": bool(np.array_equal(rows[:, 0], rlst[:, 0]))}
 
 
 def main():
@@ -224,8 +408,12 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", default="regf", choices=["regf", "ensemble"],
-                    help="regf: the headline N=1M step; ensemble: 256-cluster batch vs loop")
+    ap.add_argument("--workload", default="regf",
+                    choices=["regf", "ensemble", "learned-rs", "precision"],
+                    help="regf: the headline N=1M step; ensemble: 256-cluster batch vs loop; "
+                         "learned-rs: overflow retries of the RS network vs the reference rule")
+    ap.add_argument("--rs-train-samples", type=int, default=10_000_000)
+    ap.add_argument("--rs-eval-steps", type=int, default=300)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one plain copy kernel of MIB MiB (byte-counter calibration)")
     args = ap.parse_args()
@@ -245,6 +433,13 @@ def main():
 
     if args.workload == "ensemble":
         print(json.dumps(bench_ensemble(args.steps, args.warmup)), flush=True)
+        return 0
+    if args.workload == "precision":
+        print(json.dumps(bench_precision(args.no_cpu)), flush=True)
+        return 0
+    if args.workload == "learned-rs":
+        print(json.dumps(bench_learned_rs(args.rs_train_samples, args.rs_eval_steps,
+                                          args.no_cpu)), flush=True)
         return 0
 
     m, x, v, idx, h2, dtr = make_workload()
@@ -283,7 +478,6 @@ def main():
 
     out = do.to_host(np.float64, (7, ni))
     lst = dl.to_host(np.int32, (ni, LMAX))
-    acc = accuracy(m, x, v, idx, h2, dtr, out, lst)
     for d in (dj, di, do, dl):
         d.free()
     g.close()
@@ -297,7 +491,12 @@ def main():
             "traffic": traffic_from_profiles(),
             "kernel": "regf_tile_kernel", "avg_launch_ms": round(t_tile * 1e3, 3),
             "finalize_avg_ms": round(float(np.mean(fin_ms)), 3), "j_chunks": chunks}
-    cpu = None if args.no_cpu else cpu_baseline(m, x, v, idx, h2, dtr)
+    # CPU leg: the reference's AVX back end timed on a sample, and the FP64 oracle
+    # as the checker of the GPU result on the first 256 i-particles
+    cpu, acc = None, None
+    if not args.no_cpu:
+        cpu = cpu_baseline(m, x, v, idx, h2, dtr)
+        acc = accuracy(m, x, v, idx, h2, dtr, out, lst)
     rec = {"metric": "regf pair-interactions/s at N=1M (%FP32 peak) at <=1e-6 rel err vs FP64",
            "value": value, "unit": "pair-interactions/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,

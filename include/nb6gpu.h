@@ -142,6 +142,12 @@ int nb6_set_regf_config(int threads, int i_per_lane, int min_groups);
  * Defaults: 2048, 1. */
 int nb6_set_regf_tuning(int isort_min, int far_test);
 
+/* Pair arithmetic of the regular-force kernel, for precision studies: 0 = FP32
+ * with FP64 differencing and accumulation (default, the production kernel),
+ * 1 = float-float (FP32 hi/lo pairs), 2 = FP64.  Lists are the exact FP64 lists
+ * in every mode; modes 1 and 2 run 256-wide with one i per lane. */
+int nb6_set_regf_precision(int mode);
+
 /* Average device time (ms) of the last launch of the named kernel family
  * ("regf_tile", "regf_finalize", "pot", "irr") measured with HIP events, and
  * the number of j-chunks the last regf call used. */
@@ -167,6 +173,18 @@ int nb6_ens_send_device(int nclusters, const int *offsets, const double *d_soa, 
 int nb6_ens_regf_device(const double *d_i, int stride, int lmax, int nnbmax, int m_flag,
                         double *d_out, int *d_list, int use_graph, void *stream);
 int nb6_ens_items(void);
+
+/* Autotuners (run on the caller's own, already sent, inputs; outputs are
+ * overwritten).  They time every combination of work-group size = LDS j-tile
+ * (128, 256, 512), i-particles per lane (1, 2; i-block 128..1024) and occupancy
+ * (regf: minimum work-groups 1024/2048/4096 through j-chunking; ensemble: 8, 16
+ * or 32 wavefronts per CU in the persistent launch), keep the fastest as the
+ * library's configuration and report best = {threads, i_per_lane, min_groups or
+ * work-groups per CU, time in microseconds}. */
+int nb6_regf_autotune_device(int ni, int ni_stride, const double *d_i, int lmax, int nnbmax,
+                             int m_flag, double *d_out, int *d_list, int *best, void *stream);
+int nb6_ens_autotune(const double *d_i, int stride, int lmax, int nnbmax, int m_flag,
+                     double *d_out, int *d_list, int *best, void *stream);
 
 /* Streaming device-to-device copy of bytes (a multiple of 16) by a plain copy
  * kernel; used to calibrate HBM byte counters against a known size. */
@@ -218,9 +236,10 @@ int nb6_rs_predict_device(int n, const float *d_feat, const double *d_rs_prev,
 /* The reference's RS update after a regular step, regcor_gpu.F:95-238 (single
  * stars, NB_FLAG = 1, M_FLAG = 0; the c.m.-body branch and CHECKL are not part of
  * it): x, v [3][stride]; dtr; nnb = new count without self, nnb0 = previous count;
- * list int[n][lmax] (0-based j without self, used for nnb <= 3, may be null) into
- * xj, vj [3][jstride]; glob = {rdens x, y, z, RSCALE, RC^2, RSFAC, ZNBMIN} (host).
- * d_rs is updated in place. */
+ * list int[n][lmax] = the rows gpunb_regf returned (count, 0-based j including the
+ * particle itself, which is skipped; used for nnb <= 3; may be null) into the
+ * j-set positions xj, vj [3][jstride]; glob = {rdens x, y, z, RSCALE, RC^2, RSFAC,
+ * ZNBMIN} (host).  d_rs is updated in place. */
 int nb6_rs_rule_device(int n, int stride, const double *d_x, const double *d_v,
                        const double *d_dtr, const int *d_nnb, const int *d_nnb0,
                        const int *d_list, int lmax, const double *d_xj, const double *d_vj,

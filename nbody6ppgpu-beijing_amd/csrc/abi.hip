@@ -499,6 +499,15 @@ int nb6_set_regf_tuning(int isort_min, int far_test) {
     return OK;
 }
 
+int nb6_set_regf_precision(int mode) {
+    if (mode < 0 || mode > 2) {
+        set_error(E_ARG, "nb6_set_regf_precision: mode must be 0 (FP32), 1 (float-float) or 2 (FP64)");
+        return E_ARG;
+    }
+    regf_config().prec = mode;
+    return OK;
+}
+
 double nb6_kernel_ms(const char *name) { return ktimer(name).ms(); }
 
 int nb6_ens_open(int max_clusters, int max_particles) { return ens_open(max_clusters, max_particles); }
@@ -518,6 +527,22 @@ int nb6_ens_regf_device(const double *d_i, int stride, int lmax, int nnbmax, int
                            stream ? static_cast<hipStream_t>(stream) : d.stream);
 }
 int nb6_ens_items(void) { return ens_items(); }
+int nb6_ens_autotune(const double *d_i, int stride, int lmax, int nnbmax, int m_flag,
+                     double *d_out, int *d_list, int *best, void *stream) {
+    Device &d = device();
+    if (!d.ready) return E_NODEV;
+    return ens_autotune(d_i, stride, lmax, nnbmax, m_flag, d_out, d_list, best,
+                        stream ? static_cast<hipStream_t>(stream) : d.stream);
+}
+int nb6_regf_autotune_device(int ni, int ni_stride, const double *d_i, int lmax, int nnbmax,
+                             int m_flag, double *d_out, int *d_list, int *best, void *stream) {
+    Gpunb &g = g_nb;
+    if (!g.open) { set_error(E_STATE, "nb6_regf_autotune_device: library not open"); return E_STATE; }
+    if (ni <= 0 || ni_stride < ni) { set_error(E_ARG, "nb6_regf_autotune_device: bad ni / stride"); return E_ARG; }
+    Device &d = device();
+    return regf_autotune(g.js, ni, ni_stride, d_i, lmax, nnbmax, m_flag, d_out, d_list, best,
+                         stream ? static_cast<hipStream_t>(stream) : d.stream);
+}
 
 int nb6_device_copy(void *dst, const void *src, unsigned long long bytes, void *stream) {
     Device &d = device();

@@ -172,11 +172,44 @@ struct Ens {
     DBuf d_items, d_queue;
     RegfWork work;                   // own scratch: a captured graph holds its addresses
     int nitems = 0;
+    int items_bi = 0;                // i-block the items were cut for
+    int per_cu = 0;                  // persistent work-groups per CU (0: 8 waves per SIMD)
     // graph of the last regular step and everything it captured by address or value
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     GraphKey gkey;
 } g_ens;
+
+// work items: (cluster, block of Morton-ordered i slots), longest first; the real
+// particles of cluster c fill slots [padoff[c], padoff[c] + n_c); the i-block is the
+// current work-group size times i-particles per lane
+int build_items(Ens &e, hipStream_t s) {
+    const RegfConfig cfg = regf_effective(0);
+    const int bi = cfg.threads * cfg.ipl;
+    std::vector<int4> items;
+    std::vector<long long> cost;
+    for (int c = 0; c < e.nc; c++) {
+        const int nci = e.off[c + 1] - e.off[c];
+        const int plen = e.padoff[c + 1] - e.padoff[c];
+        const int send = e.padoff[c] + nci;
+        for (int ib = e.padoff[c]; ib < send; ib += bi) {
+            items.push_back(make_int4(ib, send, e.padoff[c], e.padoff[c + 1]));
+            cost.push_back((long long)std::min(bi, send - ib) * plen);
+        }
+    }
+    std::vector<int> order(items.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<int4> sorted(items.size());
+    for (size_t k = 0; k < items.size(); k++) sorted[k] = items[order[k]];
+    e.nitems = (int)sorted.size();
+    e.items_bi = bi;
+    e.d_items.get(sorted.size() * sizeof(int4));
+    NB6_HIP(hipMemcpyAsync(e.d_items.p, sorted.data(), sorted.size() * sizeof(int4),
+                           hipMemcpyHostToDevice, s));
+    NB6_HIP(hipStreamSynchronize(s));
+    return OK;
+}
 
 void drop_graph() {
     if (g_ens.exec) (void)hipGraphExecDestroy(g_ens.exec);
@@ -227,6 +260,8 @@ int ens_close() {
     e.d_tmp.release(); e.d_jbase.release(); e.d_items.release(); e.d_queue.release();
     e.work.release();
     e.nc = 0;
+    e.per_cu = 0;
+    e.items_bi = 0;
     e.open = false;
     return OK;
 }
@@ -302,32 +337,7 @@ int ens_send_device(int nc, const int *offsets, const double *d_soa, int stride,
     double mmax = 0.0;
     for (int c = 0; c < nc; c++) mmax = std::max(mmax, box[(size_t)c * 7 + 6]);
     e.js.mmax = mmax;
-    // work items: (cluster, block of Morton-ordered i slots), longest first; the real
-    // particles of cluster c fill slots [padoff[c], padoff[c] + n_c)
-    const RegfConfig cfg = regf_effective(0);
-    const int bi = cfg.threads * cfg.ipl;
-    std::vector<int4> items;
-    std::vector<long long> cost;
-    for (int c = 0; c < nc; c++) {
-        const int nci = offsets[c + 1] - offsets[c];
-        const int plen = e.padoff[c + 1] - e.padoff[c];
-        const int send = e.padoff[c] + nci;
-        for (int ib = e.padoff[c]; ib < send; ib += bi) {
-            items.push_back(make_int4(ib, send, e.padoff[c], e.padoff[c + 1]));
-            cost.push_back((long long)std::min(bi, send - ib) * plen);
-        }
-    }
-    std::vector<int> order(items.size());
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-    std::vector<int4> sorted(items.size());
-    for (size_t k = 0; k < items.size(); k++) sorted[k] = items[order[k]];
-    e.nitems = (int)sorted.size();
-    e.d_items.get(sorted.size() * sizeof(int4));
-    NB6_HIP(hipMemcpyAsync(e.d_items.p, sorted.data(), sorted.size() * sizeof(int4),
-                           hipMemcpyHostToDevice, s));
-    NB6_HIP(hipStreamSynchronize(s));
-    return OK;
+    return build_items(e, s);
 }
 
 int ens_regf_device(const double *d_i, int stride, int lmax, int nnbmax, int m_flag, double *d_out,
@@ -337,7 +347,12 @@ int ens_regf_device(const double *d_i, int stride, int lmax, int nnbmax, int m_f
     if (stride < e.ntot) { set_error(E_ARG, "nb6_ens_regf_device: stride below the particle count"); return E_ARG; }
     Device &d = device();
     const RegfConfig cfg = regf_effective(0);
-    const int per_cu = std::max(1, (8 * 256) / cfg.threads);
+    if (e.items_bi != cfg.threads * cfg.ipl) {   // configuration changed since the send
+        drop_graph();
+        const int rc = build_items(e, s);
+        if (rc != OK) return rc;
+    }
+    const int per_cu = e.per_cu > 0 ? e.per_cu : std::max(1, (8 * 256) / cfg.threads);
     const int grid = std::min(e.nitems, d.num_cu * per_cu);
     auto step = [&](hipStream_t st) {
         hipLaunchKernelGGL(ens_zero_kernel, dim3(1), dim3(1), 0, st, e.d_queue.as<int>());
@@ -376,5 +391,50 @@ int ens_regf_device(const double *d_i, int stride, int lmax, int nnbmax, int m_f
 }
 
 int ens_items() { return g_ens.nitems; }
+
+// Sweep work-group size (= LDS j-tile) 128/256/512, i-particles per lane 1/2 (i-block
+// 128..1024) and persistent work-groups per CU on the sent ensemble; keep the fastest.
+int ens_autotune(const double *d_i, int stride, int lmax, int nnbmax, int m_flag, double *d_out,
+                 int *d_list, int *best, hipStream_t s) {
+    Ens &e = g_ens;
+    if (!e.open || e.nc == 0) { set_error(E_STATE, "nb6_ens_autotune: nothing sent"); return E_STATE; }
+    RegfConfig &cfg = regf_config();
+    const RegfConfig keep = cfg;
+    hipEvent_t a, b;
+    NB6_HIP(hipEventCreate(&a));
+    NB6_HIP(hipEventCreate(&b));
+    float best_ms = 1e30f;
+    int bt = 256, bipl = 1, bcu = 0;
+    const int threads[3] = {128, 256, 512};
+    for (int ti = 0; ti < 3; ti++)
+        for (int ipl = 1; ipl <= 2; ipl++) {
+            const int waves_per_group = threads[ti] / 64;
+            for (int wpc : {8, 16, 32}) {        // wavefronts per CU
+                const int pcu = std::max(1, wpc / waves_per_group);
+                cfg.threads = threads[ti];
+                cfg.ipl = ipl;
+                e.per_cu = pcu;
+                int rc = ens_regf_device(d_i, stride, lmax, nnbmax, m_flag, d_out, d_list, 0, s);
+                if (rc != OK) { cfg = keep; e.per_cu = 0; return rc; }
+                NB6_HIP(hipEventRecord(a, s));
+                rc = ens_regf_device(d_i, stride, lmax, nnbmax, m_flag, d_out, d_list, 0, s);
+                NB6_HIP(hipEventRecord(b, s));
+                NB6_HIP(hipEventSynchronize(b));
+                if (rc != OK) { cfg = keep; e.per_cu = 0; return rc; }
+                float ms = 0.f;
+                NB6_HIP(hipEventElapsedTime(&ms, a, b));
+                if (ms < best_ms) { best_ms = ms; bt = threads[ti]; bipl = ipl; bcu = pcu; }
+            }
+        }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    cfg.threads = bt;
+    cfg.ipl = bipl;
+    e.per_cu = bcu;
+    drop_graph();
+    const int rc = build_items(e, s);
+    if (best) { best[0] = bt; best[1] = bipl; best[2] = bcu; best[3] = (int)(best_ms * 1000.f); }
+    return rc;
+}
 
 }  // namespace nb6

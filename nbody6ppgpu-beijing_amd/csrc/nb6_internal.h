@@ -99,6 +99,7 @@ struct RegfConfig {
     int min_groups = 2048;
     int isort_min = 2048; // i-sets this large are Morton-ordered (0: never)
     int far_test = 1;     // skip the pair test on tiles far from the whole wavefront
+    int prec = 0;         // pair arithmetic: 0 FP32, 1 float-float, 2 FP64 (study variants)
 };
 RegfConfig &regf_config();
 // the configuration a call with ni i-particles runs with (threads resolved)
@@ -161,6 +162,10 @@ int rs_rule(int n, int stride, const double *x, const double *v, const double *d
             int jstride, const double *glob, int nnbopt, int nnbmax, double *rs, hipStream_t s);
 int rs_kth(const JSet &js, int ni, int ib_stride, const double *ib, const int *list, int lmax, int k,
            double *rs_out, hipStream_t s);
+int regf_autotune(const JSet &js, int ni, int ni_stride, const double *d_i, int lmax, int nnbmax,
+                  int m_flag, double *d_out, int *d_list, int *best, hipStream_t s);
+int ens_autotune(const double *d_i, int stride, int lmax, int nnbmax, int m_flag, double *d_out,
+                 int *d_list, int *best, hipStream_t s);
 // the j-set of the open gpunb library (nullptr when closed)
 const JSet *gpunb_jset();
 

@@ -62,6 +62,7 @@ struct TileParams {
     const double *__restrict__ torg;
     const float *__restrict__ tbox;  // [ntile][TBOX] per-tile bounds
     int far_test;                    // 0: always run the pair test
+    const double *__restrict__ rec64;   // sorted AoS [nj][8] (precision variants)
     int nj;
     int chunk_len;                   // multiple of the block size T and of JTILE
     double mmax;
@@ -409,6 +410,181 @@ __global__ __launch_bounds__(T, regf_waves<IPL>()) void regf_items_kernel(TilePa
     }
 }
 
+// ------------------------------------------------------------------ precision variants
+// The same pipeline (chunks, slots, finalize) with the pair arithmetic in float-float
+// (PREC = 1: every FP64 input split into an FP32 hi/lo pair, error-free sums and
+// products, one float-float Newton-Raphson step on v_rsq_f32) or in FP64 (PREC = 2:
+// the oracle's formulas).  They exist for the precision study: each pair's neighbour
+// test runs in the variant's arithmetic, every pair near the threshold is recorded as a
+// candidate and left out of the sums, and the finalize stage decides and adds those in
+// FP64 exactly as for the FP32 kernel, so the lists are the exact FP64 lists.
+struct ff {
+    float h, l;
+};
+
+__device__ __forceinline__ ff ff_two_sum(float a, float b) {
+#pragma clang fp contract(off)
+    const float s = a + b;
+    const float bb = s - a;
+    return {s, (a - (s - bb)) + (b - bb)};
+}
+__device__ __forceinline__ ff ff_fast(float a, float b) {
+#pragma clang fp contract(off)
+    const float s = a + b;
+    return {s, b - (s - a)};
+}
+__device__ __forceinline__ ff ff_add(ff x, ff y) {
+#pragma clang fp contract(off)
+    const ff s = ff_two_sum(x.h, y.h);
+    return ff_fast(s.h, s.l + (x.l + y.l));
+}
+__device__ __forceinline__ ff ff_neg(ff x) { return {-x.h, -x.l}; }
+__device__ __forceinline__ ff ff_mul(ff x, ff y) {
+#pragma clang fp contract(off)
+    const float p = x.h * y.h;
+    const float e = fmaf(x.h, y.h, -p);
+    return ff_fast(p, e + (x.h * y.l + x.l * y.h));
+}
+__device__ __forceinline__ ff ff_of(double d) {
+    const float h = (float)d;
+    return {h, (float)(d - (double)h)};
+}
+__device__ __forceinline__ double ff_d(ff x) { return (double)x.h + (double)x.l; }
+
+template <int PREC, bool MFLAG>
+__global__ __launch_bounds__(256) void regf_var_kernel(TileParams p) {
+    __shared__ double4 sA[256], sB[256];
+    const int tid = threadIdx.x;
+    const int chunk = blockIdx.y;
+    const int j0 = chunk * p.chunk_len;
+    const int j1 = min(j0 + p.chunk_len, p.nj);
+    const int i = blockIdx.x * 256 + tid;   // slot
+    const bool in = i < p.ni;
+    int r = i;
+    if (in && p.imap) r = p.imap[i];
+    const bool ok = in && r >= 0;
+    const int rr = ok ? r : 0;
+    const size_t is = p.ib_stride;
+    const double xi = p.ib[rr], yi = p.ib[is + rr], zi = p.ib[2 * is + rr];
+    const double ui = p.ib[3 * is + rr], vi = p.ib[4 * is + rr], wi = p.ib[5 * is + rr];
+    const double h2 = p.ib[6 * is + rr], dtr = p.ib[7 * is + rr];
+    const size_t row = (size_t)chunk * p.ni + (in ? i : 0);
+    int *hp = p.hslot + row * p.caph;
+    int nh = 0;
+    double a64[7] = {0, 0, 0, 0, 0, 0, 0};
+    ff f[10];
+#pragma unroll
+    for (int q = 0; q < 10; q++) f[q] = {0.f, 0.f};
+    const ff fxi = ff_of(xi), fyi = ff_of(yi), fzi = ff_of(zi);
+    const ff fui = ff_of(ui), fvi = ff_of(vi), fwi = ff_of(wi);
+    const ff fdt = ff_of(dtr);
+    for (int jt = j0; jt < j1; jt += 256) {
+        __syncthreads();
+        const int j = jt + tid;
+        if (j < j1) {
+            sA[tid] = reinterpret_cast<const double4 *>(p.rec64)[2 * (size_t)j];
+            sB[tid] = reinterpret_cast<const double4 *>(p.rec64)[2 * (size_t)j + 1];
+        } else {
+            sA[tid] = make_double4(1.0e17, 0.0, 0.0, 0.0);
+            sB[tid] = make_double4(0.0, 0.0, 0.0, 0.0);
+        }
+        __syncthreads();
+        if (!ok) continue;
+        const int jn = min(256, j1 - jt);
+        for (int jj = 0; jj < jn; jj++) {
+            const double4 a = sA[jj], b = sB[jj];
+            const int jg = jt + jj;
+            if (PREC == 2) {
+#pragma clang fp contract(off)
+                const double dx = a.x - xi, dy = a.y - yi, dz = a.z - zi;
+                const double du = b.x - ui, dv = b.y - vi, dw = b.z - wi;
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                const double ex = dx + dtr * du, ey = dy + dtr * dv, ez = dz + dtr * dw;
+                const double r2p = ex * ex + ey * ey + ez * ez;
+                const double rmin = r2 < r2p ? r2 : r2p;
+                const double thr = MFLAG ? a.w * h2 : h2;
+                if (rmin < thr) {
+                    if (nh < p.caph) hp[nh] = jg;
+                    nh++;
+                } else if (r2 > 0.0) {
+                    const double rinv2 = 1.0 / r2;
+                    const double rinv = sqrt(rinv2);
+                    const double mr1 = a.w * rinv;
+                    const double mr3 = mr1 * rinv2;
+                    const double rv = dx * du + dy * dv + dz * dw;
+                    const double al = -3.0 * rv * rinv2;
+                    a64[0] += mr3 * dx;
+                    a64[1] += mr3 * dy;
+                    a64[2] += mr3 * dz;
+                    a64[3] += mr3 * (du + al * dx);
+                    a64[4] += mr3 * (dv + al * dy);
+                    a64[5] += mr3 * (dw + al * dz);
+                    a64[6] += mr1;
+                }
+            } else {
+                const ff dx = ff_add(ff_of(a.x), ff_neg(fxi));
+                const ff dy = ff_add(ff_of(a.y), ff_neg(fyi));
+                const ff dz = ff_add(ff_of(a.z), ff_neg(fzi));
+                const ff du = ff_add(ff_of(b.x), ff_neg(fui));
+                const ff dv = ff_add(ff_of(b.y), ff_neg(fvi));
+                const ff dw = ff_add(ff_of(b.z), ff_neg(fwi));
+                const ff r2 = ff_add(ff_add(ff_mul(dx, dx), ff_mul(dy, dy)), ff_mul(dz, dz));
+                const ff ex = ff_add(dx, ff_mul(fdt, du));
+                const ff ey = ff_add(dy, ff_mul(fdt, dv));
+                const ff ez = ff_add(dz, ff_mul(fdt, dw));
+                const ff r2p = ff_add(ff_add(ff_mul(ex, ex), ff_mul(ey, ey)), ff_mul(ez, ez));
+                const double d2 = ff_d(r2), d2p = ff_d(r2p);
+                const double rmin = d2 < d2p ? d2 : d2p;
+                const double thr = MFLAG ? a.w * h2 : h2;
+                // float-float keeps ~2^-44 relative: 2^-30 covers the decision
+                if (rmin < thr * (1.0 + 0x1p-30) + 0x1p-120 && jg < j1) {
+                    if (nh < p.caph) hp[nh] = jg;
+                    nh++;
+                } else if (d2 > 0.0) {
+                    // rinv: v_rsq_f32 refined by one float-float Newton-Raphson step
+                    const float y0 = __builtin_amdgcn_rsqf(r2.h);
+                    const ff y = {y0, 0.f};
+                    const ff e = ff_add({1.f, 0.f}, ff_neg(ff_mul(r2, ff_mul(y, y))));
+                    const ff rinv = ff_add(y, ff_mul(y, {0.5f * e.h, 0.5f * e.l}));
+                    const ff rinv2 = ff_mul(rinv, rinv);
+                    const ff mr1 = ff_mul(ff_of(a.w), rinv);
+                    const ff mr3 = ff_mul(mr1, rinv2);
+                    const ff rv = ff_add(ff_add(ff_mul(dx, du), ff_mul(dy, dv)), ff_mul(dz, dw));
+                    const ff be = ff_mul(ff_mul(rv, rinv2), mr3);
+                    f[0] = ff_add(f[0], ff_mul(mr3, dx));
+                    f[1] = ff_add(f[1], ff_mul(mr3, dy));
+                    f[2] = ff_add(f[2], ff_mul(mr3, dz));
+                    f[3] = ff_add(f[3], mr1);
+                    f[4] = ff_add(f[4], ff_mul(mr3, du));
+                    f[5] = ff_add(f[5], ff_mul(mr3, dv));
+                    f[6] = ff_add(f[6], ff_mul(mr3, dw));
+                    f[7] = ff_add(f[7], ff_mul(be, dx));
+                    f[8] = ff_add(f[8], ff_mul(be, dy));
+                    f[9] = ff_add(f[9], ff_mul(be, dz));
+                }
+            }
+        }
+        if (PREC == 1) {   // float-float partials into FP64 after every tile
+            a64[0] += ff_d(f[0]);
+            a64[1] += ff_d(f[1]);
+            a64[2] += ff_d(f[2]);
+            a64[3] += ff_d(f[4]) - 3.0 * ff_d(f[7]);
+            a64[4] += ff_d(f[5]) - 3.0 * ff_d(f[8]);
+            a64[5] += ff_d(f[6]) - 3.0 * ff_d(f[9]);
+            a64[6] += ff_d(f[3]);
+#pragma unroll
+            for (int q = 0; q < 10; q++) f[q] = {0.f, 0.f};
+        }
+    }
+    if (in) {
+        double *pp = p.part + (size_t)chunk * 7 * p.ni + i;
+#pragma unroll
+        for (int q = 0; q < 7; q++) pp[(size_t)q * p.ni] = a64[q];
+        p.cnt[((size_t)chunk * 2 + 0) * p.ni + i] = nh;
+        p.cnt[((size_t)chunk * 2 + 1) * p.ni + i] = 0;
+    }
+}
+
 struct FinParams {
     const double *__restrict__ ib;   // SoA [8][ib_stride]
     int ni, ib_stride;               // ni = slot rows of the work arrays
@@ -608,6 +784,16 @@ void launch_items_t(bool mflag, int grid, const TileParams &tp, const int4 *item
 
 // dispatch on the tile configuration (block size T in {128, 256, 512}, IPL in {1, 2})
 void launch_tile(const RegfConfig &cfg, bool mflag, dim3 grid, const TileParams &tp, hipStream_t s) {
+    if (cfg.prec == 1 || cfg.prec == 2) {
+        if (cfg.prec == 1) {
+            if (mflag) hipLaunchKernelGGL((regf_var_kernel<1, true>), grid, dim3(256), 0, s, tp);
+            else hipLaunchKernelGGL((regf_var_kernel<1, false>), grid, dim3(256), 0, s, tp);
+        } else {
+            if (mflag) hipLaunchKernelGGL((regf_var_kernel<2, true>), grid, dim3(256), 0, s, tp);
+            else hipLaunchKernelGGL((regf_var_kernel<2, false>), grid, dim3(256), 0, s, tp);
+        }
+        return;
+    }
     const int key = cfg.threads * 10 + cfg.ipl;
     switch (key) {
         case 1281: launch_tile_t<128, 1>(mflag, grid, tp, s); break;
@@ -644,6 +830,10 @@ RegfConfig &regf_config() {
 RegfConfig regf_effective(int ni) {
     RegfConfig c = regf_config();
     if (c.threads == 0) c.threads = ni >= 65536 ? 512 : 256;
+    if (c.prec != 0) {   // the precision variants run 256-wide, one i per lane
+        c.threads = 256;
+        c.ipl = 1;
+    }
     return c;
 }
 
@@ -692,6 +882,7 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
     tp.rec32 = js.rec32.as<float4>(); tp.torg = js.torg.as<double>(); tp.nj = nj;
     tp.tbox = js.tbox.as<float>();
     tp.far_test = cfg.far_test;
+    tp.rec64 = js.rec64.as<double>();
     tp.chunk_len = clen;
     tp.mmax = js.mmax;
     tp.part = part; tp.cnt = cnt;
@@ -755,6 +946,7 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
     tp.rec32 = js.rec32.as<float4>(); tp.torg = js.torg.as<double>(); tp.nj = js.n;
     tp.tbox = js.tbox.as<float>();
     tp.far_test = cfg.far_test;
+    tp.rec64 = js.rec64.as<double>();
     tp.chunk_len = js.n;
     tp.mmax = js.mmax;
     tp.part = part; tp.cnt = cnt;
@@ -782,6 +974,39 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
 }
 
 int regf_last_chunks() { return g_last_chunks; }
+
+// Sweep work-group size (= LDS j-tile) 128/256/512, i-particles per lane 1/2 and the
+// minimum number of work-groups (occupancy through j-chunking) on the caller's own
+// inputs; the fastest becomes the configuration.  best = {threads, ipl, min_groups, us}.
+int regf_autotune(const JSet &js, int ni, int ni_stride, const double *d_i, int lmax, int nnbmax,
+                  int m_flag, double *d_out, int *d_list, int *best, hipStream_t s) {
+    RegfConfig &cfg = regf_config();
+    const RegfConfig keep = cfg;
+    if (cfg.prec != 0) { set_error(E_STATE, "nb6_regf_autotune: set the FP32 kernel first"); return E_STATE; }
+    float best_ms = 1e30f;
+    int bt = 256, bipl = 1, bmg = 2048;
+    const int threads[3] = {128, 256, 512};
+    for (int ti = 0; ti < 3; ti++)
+        for (int ipl = 1; ipl <= 2; ipl++)
+            for (int mg : {1024, 2048, 4096}) {
+                cfg.threads = threads[ti];
+                cfg.ipl = ipl;
+                cfg.min_groups = mg;
+                float ms = 0.f;
+                for (int rep = 0; rep < 2; rep++) {   // the first run warms up
+                    const int rc = regf_launch(js, ni, ni_stride, d_i, lmax, nnbmax, m_flag, d_out,
+                                               d_list, s);
+                    if (rc != OK) { cfg = keep; return rc; }
+                    ms = (float)(ktimer("regf_tile").ms() + ktimer("regf_finalize").ms());
+                }
+                if (ms < best_ms) { best_ms = ms; bt = threads[ti]; bipl = ipl; bmg = mg; }
+            }
+    cfg.threads = bt;
+    cfg.ipl = bipl;
+    cfg.min_groups = bmg;
+    if (best) { best[0] = bt; best[1] = bipl; best[2] = bmg; best[3] = (int)(best_ms * 1000.f); }
+    return OK;
+}
 
 void regf_release() { g_work.release(); }
 

@@ -402,14 +402,19 @@ __global__ __launch_bounds__(256) void rs_rule_kernel(RuleParams q) {
     }
     if (nnb > nbave && ridot < 0.0) rs = rs * (1.0 - (fn - nbave) / (3.0 * fn));
     if (nnb <= 3 && ri2 < 100.0 * rh2 && q.list) {   // regcor_gpu.F:222-238
+        // rows as gpunb_regf returns them: the particle itself (zero distance) is
+        // among the entries and is skipped
         double b = 2.0 * rs;
         const int *row = q.list + (size_t)i * q.lmax;
+        const int c = min(row[0], q.lmax - 1);
         const size_t js = q.jstride;
-        for (int l = 0; l < nnb; l++) {
+        for (int l = 0; l < c; l++) {
             const int j = row[1 + l];
+            if ((unsigned)j >= (unsigned)q.jstride) continue;
             const double d[3] = {xi[0] - q.xj[j], xi[1] - q.xj[js + j], xi[2] - q.xj[2 * js + j]};
             const double w[3] = {vi[0] - q.vj[j], vi[1] - q.vj[js + j], vi[2] - q.vj[2 * js + j]};
             const double rij = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            if (rij == 0.0) continue;
             const double rsdot = (d[0] * w[0] + d[1] * w[1] + d[2] * w[2]) / rij;
             b = fmin(b, rij + rsdot * dtr);
         }

@@ -70,11 +70,14 @@ def load():
                                  _vp, _vp]
     L.nb6_set_regf_config.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.nb6_set_regf_tuning.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.nb6_set_regf_precision.argtypes = [ctypes.c_int]
     L.nb6_device_copy.argtypes = [_vp, _vp, ctypes.c_ulonglong, _vp]
     L.nb6_ens_send_device.argtypes = [ctypes.c_int, _ip, _vp, ctypes.c_int, _vp]
     L.nb6_ens_regf_device.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       _vp, _vp, ctypes.c_int, _vp]
     i, d, f, u = ctypes.c_int, ctypes.c_double, ctypes.c_float, ctypes.c_ulonglong
+    L.nb6_regf_autotune_device.argtypes = [i, i, _vp, i, i, i, _vp, _vp, _vp, _vp]
+    L.nb6_ens_autotune.argtypes = [_vp, i, i, i, i, _vp, _vp, _vp, _vp]
     L.nb6_rsnet_open.argtypes = [u]
     L.nb6_rsnet_set_params.argtypes = [_vp]
     L.nb6_rsnet_get_params.argtypes = [_vp]

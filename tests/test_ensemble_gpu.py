@@ -70,3 +70,39 @@ def test_ensemble_graph_replays_equal_eager():
     for out, lst in graph:
         assert np.array_equal(out, eager[0])
         assert rows_as_sets(lst) == rows_as_sets(eager[1])
+
+
+def test_ensemble_autotune_keeps_results():
+    """The autotuner only changes the launch shape: lists stay identical and the
+    forces agree with the default configuration to FP32 summation-order level."""
+    import ctypes
+    sizes = [700, 2500, 300, 1200]
+    ms, xs, vs, h2s, dts, off = make_ensemble(sizes, seed=5)
+    base = run_ensemble(ms, xs, vs, h2s, dts, off, 150, 120, use_graph=0)[0]
+    L = lib.load()
+    ntot = int(off[-1])
+    m, x, v = np.concatenate(ms), np.concatenate(xs), np.concatenate(vs)
+    h2, dtr = np.concatenate(h2s), np.concatenate(dts)
+    dj = lib.DeviceArray.from_host(np.ascontiguousarray(np.concatenate([x.T, v.T, m[None]])))
+    di = lib.DeviceArray.from_host(np.ascontiguousarray(np.concatenate([x.T, v.T, h2[None], dtr[None]])))
+    do = lib.DeviceArray(7 * ntot * 8)
+    dl = lib.DeviceArray(ntot * 150 * 4)
+    best = (ctypes.c_int * 4)()
+    try:
+        assert L.nb6_ens_open(len(sizes), ntot) == 0
+        assert L.nb6_ens_send_device(len(sizes), off.ctypes.data_as(lib._ip), dj.ptr, ntot, None) == 0
+        assert L.nb6_ens_autotune(di.ptr, ntot, 150, 120, 0, do.ptr, dl.ptr, best, None) == 0
+        assert best[0] in (128, 256, 512) and best[1] in (1, 2) and best[2] >= 1
+        assert L.nb6_ens_regf_device(di.ptr, ntot, 150, 120, 0, do.ptr, dl.ptr, 1, None) == 0
+        L.nb6_sync()
+        out = do.to_host(np.float64, (7, ntot))
+        lst = dl.to_host(np.int32, (ntot, 150))
+    finally:
+        L.nb6_ens_close()
+        L.nb6_set_regf_config(0, 1, 2048)
+        for d in (dj, di, do, dl):
+            d.free()
+    print("autotuned", list(best))
+    assert rows_as_sets(lst) == rows_as_sets(base[1])
+    ea = vec_rel_err(out[0:3].T, base[0][0:3].T)
+    assert ea.max() < 1e-6

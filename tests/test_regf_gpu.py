@@ -122,6 +122,61 @@ def test_regf_fast_movers_and_outsiders(gpunb, threads, ipl, m_flag):
     _compare(got, ref, f"fast movers threads={threads} ipl={ipl} m_flag={m_flag}")
 
 
+@pytest.mark.parametrize("mode,tol", [(1, 2e-11), (2, 1e-13)])
+@pytest.mark.parametrize("m_flag", [0, 1])
+def test_regf_precision_variants(gpunb, mode, tol, m_flag):
+    """Float-float (mode 1) and FP64 (mode 2) pair arithmetic: the exact FP64
+    lists and forces within tol of the FP64 oracle for every particle, jerk
+    included (no FP32 tail)."""
+    m, x, v, idx, h2, dtr = c2_inputs(n=16384, ni=3000, m_flag=m_flag)
+    n = m.size
+    lmax, nnbmax = LMAX_SMALL + 100, nnbmax_for(n, LMAX_SMALL + 100)
+    L = lib.load()
+    L.nb6_set_regf_precision(mode)
+    try:
+        g = gpunb(n)
+        g.send(m, x, v)
+        acc, jrk, pot, lst = g.regf(h2, dtr, x[idx], v[idx], lmax, nnbmax, m_flag)
+    finally:
+        L.nb6_set_regf_precision(0)
+    racc, rjrk, rpot, rlst = O.regf(m, x, v, h2, dtr, x[idx], v[idx], lmax, nnbmax, m_flag)
+    ea, ej = vec_rel_err(acc, racc), vec_rel_err(jrk, rjrk)
+    ep = np.abs(pot - rpot) / rpot
+    print(f"mode {mode} m_flag {m_flag}: acc {ea.max():.2e} jrk {ej.max():.2e} pot {ep.max():.2e}")
+    assert rows_as_sets(lst) == rows_as_sets(rlst)
+    assert ea.max() <= tol and ej.max() <= 10 * tol and ep.max() <= tol
+
+
+def test_regf_autotune_keeps_results(gpunb):
+    """The regf autotuner picks a launch shape on the caller's inputs; lists stay
+    identical and forces stay within the FP32 bar of the FP64 oracle."""
+    import ctypes
+    m, x, v, idx, h2, dtr = c2_inputs(n=16384, ni=2048)
+    n = m.size
+    lmax, nnbmax = 500, 450
+    g = gpunb(n)
+    g.send(m, x, v)
+    ni = idx.size
+    L = lib.load()
+    ib = np.concatenate([x[idx].T, v[idx].T, h2[None], dtr[None]]).astype(np.float64)
+    di = lib.DeviceArray.from_host(ib)
+    do = lib.DeviceArray(7 * ni * 8)
+    dl = lib.DeviceArray(ni * lmax * 4)
+    best = (ctypes.c_int * 4)()
+    try:
+        assert L.nb6_regf_autotune_device(ni, ni, di.ptr, lmax, nnbmax, 0, do.ptr, dl.ptr, best,
+                                          None) == 0
+        got = g.regf(h2, dtr, x[idx], v[idx], lmax, nnbmax)
+    finally:
+        L.nb6_set_regf_config(0, 1, 2048)
+        for d in (di, do, dl):
+            d.free()
+    print("autotuned", list(best))
+    assert best[0] in (128, 256, 512) and best[1] in (1, 2) and best[2] in (1024, 2048, 4096)
+    ref = O.regf(m, x, v, h2, dtr, x[idx], v[idx], lmax, nnbmax)
+    _compare(got, ref, "autotuned")
+
+
 def test_regf_overflow_exact_negative_counts(gpunb):
     """Counts above nnbmax come back as -count with the exact count
     (reg.avx.cpp:320-322); the caller's RS rescale (util_gpu.F:82-86) needs it."""
