@@ -191,6 +191,30 @@ int nb6_ens_autotune(const double *d_i, int stride, int lmax, int nnbmax, int m_
 int nb6_device_copy(void *dst, const void *src, unsigned long long bytes, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Device-resident j-particles with a GPU predictor (extension; replaces the host
+ * prediction xbpredall.F:52-60 plus gpunb_send_ per regular block,
+ * intgrt.F:875-896).  Fortran-callable, every argument by reference.
+ *   nb6_jp_open_(nmax) / nb6_jp_close_(): predictor state for nmax particles.
+ *   nb6_jp_set_(n, addr, x0, x0dot, f, fdot, m, t0): store the corrected state of n
+ *     particles (1-based addresses; F = a/2 and FDOT = adot/6 as in common6.h).
+ *   nb6_jp_predict_send_(time, nj): predict particles 1..nj to `time` on the GPU,
+ *     x = ((FDOT s + F) s + x0dot) s + x0, xdot = (1.5 FDOT s + F) 2s + x0dot,
+ *     s = time - t0 (xbpredall.F:55-59, FP64), and make them the j-set of the next
+ *     gpunb_regf_ calls (as gpunb_send_ with those positions would; gpunb must be
+ *     open).
+ *   nb6_jp_regf_(ni, addr, h2, dtr, acc, jrk, pot, lmax, nnbmax, list, m_flag):
+ *     gpunb_regf_ for the particles at 1-based addresses addr[ni], their positions
+ *     and velocities taken from the last prediction (no x_i / v_i from the host).
+ * ------------------------------------------------------------------------- */
+void nb6_jp_open_(int *nmax);
+void nb6_jp_close_(void);
+void nb6_jp_set_(int *n, int addr[], double x0[][3], double x0dot[][3], double f[][3],
+                 double fdot[][3], double m[], double t0[]);
+void nb6_jp_predict_send_(double *time, int *nj);
+void nb6_jp_regf_(int *ni, int addr[], double h2[], double dtr[], double acc[][3],
+                  double jrk[][3], double pot[], int *lmax, int *nnbmax, int *list, int *m_flag);
+
+/* ---------------------------------------------------------------------------
  * Part 3 (extension): learned neighbour radius.  A network 8 -> 64 -> 64 -> 1
  * (ReLU; bfloat16 weights, FP32 accumulation; 4801 parameters, layout W1[64][8],
  * b1[64], W2[64][64], b2[64], w3[64], b3) predicts y = ln(RS_next / RS_prev)

@@ -289,6 +289,10 @@ void gpunb_send_(int *nj, double mj[], double xj[][3], double vj[][3]) {
     g.time_send += wtime();
 }
 
+static void regf_finish(Gpunb &g, int ni, const double *di, int lmax, int nnbmax, int m_flag,
+                        double acc[][3], double jrk[][3], double pot[], int *list, hipStream_t s,
+                        double t0);
+
 void gpunb_regf_(int *ni_, double h2[], double dtr[], double xi[][3], double vi[][3],
                  double acc[][3], double jrk[][3], double pot[], int *lmax_, int *nnbmax_,
                  int *list, int *m_flag) {
@@ -322,10 +326,17 @@ void gpunb_regf_(int *ni_, double h2[], double dtr[], double xi[][3], double vi[
         }
     });
     double *di = static_cast<double *>(g.di.get((size_t)8 * ni * sizeof(double)));
+    NB6_HIP(hipMemcpyAsync(di, hi, (size_t)8 * ni * sizeof(double), hipMemcpyHostToDevice, s));
+    regf_finish(g, ni, di, lmax, nnbmax, *m_flag, acc, jrk, pot, list, s, t0);
+}
+
+// regf on a staged device i-block, results copied into the caller's arrays
+static void regf_finish(Gpunb &g, int ni, const double *di, int lmax, int nnbmax, int m_flag,
+                        double acc[][3], double jrk[][3], double pot[], int *list, hipStream_t s,
+                        double t0) {
     double *dout = static_cast<double *>(g.dout.get((size_t)7 * ni * sizeof(double)));
     int *dlist = static_cast<int *>(g.dlist.get((size_t)ni * lmax * sizeof(int)));
-    NB6_HIP(hipMemcpyAsync(di, hi, (size_t)8 * ni * sizeof(double), hipMemcpyHostToDevice, s));
-    if (regf_launch(g.js, ni, ni, di, lmax, nnbmax, *m_flag, dout, dlist, s) != OK) return;
+    if (regf_launch(g.js, ni, ni, di, lmax, nnbmax, m_flag, dout, dlist, s) != OK) return;
     double *ho = static_cast<double *>(g.hout.get((size_t)7 * ni * sizeof(double)));
     int *hl = static_cast<int *>(g.hlist.get((size_t)ni * lmax * sizeof(int)));
     NB6_HIP(hipMemcpyAsync(ho, dout, (size_t)7 * ni * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -556,6 +567,49 @@ int nb6_last_regf_chunks(void) { return regf_last_chunks(); }
 
 static hipStream_t pick(void *stream) {
     return stream ? static_cast<hipStream_t>(stream) : device().stream;
+}
+
+void nb6_jp_open_(int *nmax) { (void)jp_open(*nmax); }
+void nb6_jp_close_(void) { (void)jp_close(); }
+void nb6_jp_set_(int *n, int addr[], double x0[][3], double x0dot[][3], double f[][3],
+                 double fdot[][3], double m[], double t0[]) {
+    if (!check_finite(&x0[0][0], 3 * (size_t)*n) || !check_finite(&x0dot[0][0], 3 * (size_t)*n) ||
+        !check_finite(&f[0][0], 3 * (size_t)*n) || !check_finite(&fdot[0][0], 3 * (size_t)*n)) {
+        set_error(E_ARG, "nb6_jp_set: NaN in particle data");
+        return;
+    }
+    (void)jp_set(*n, addr, x0, x0dot, f, fdot, m, t0);
+}
+void nb6_jp_predict_send_(double *time, int *nj) {
+    Gpunb &g = g_nb;
+    if (!g.open) { set_error(E_STATE, "nb6_jp_predict_send: gpunb library not open"); return; }
+    const int n = *nj;
+    if (n < 0 || n > g.nbmax) { set_error(E_ARG, "nb6_jp_predict_send: nj outside [0, nbmax]"); return; }
+    const double t0 = wtime();
+    Device &d = device();
+    if (jp_predict(*time, n, g.js.in64.as<double>(), d.stream) != OK) return;
+    jset_build(g.js, n, nullptr, 0.0, d.stream);
+    g.isend++;
+    g.time_send += wtime() - t0;
+}
+void nb6_jp_regf_(int *ni_, int addr[], double h2[], double dtr[], double acc[][3],
+                  double jrk[][3], double pot[], int *lmax_, int *nnbmax_, int *list, int *m_flag) {
+    Gpunb &g = g_nb;
+    if (!g.open) { set_error(E_STATE, "nb6_jp_regf: gpunb library not open"); return; }
+    const int ni = *ni_;
+    if (ni <= 0 || *lmax_ < 1) { set_error(E_ARG, "nb6_jp_regf: ni and lmax must be positive"); return; }
+    if (!check_finite(h2, ni) || !check_finite(dtr, ni)) {
+        set_error(E_ARG, "nb6_jp_regf: NaN in h2 or dtr");
+        return;
+    }
+    const double t0 = wtime();
+    g.num_inter += (long long)ni * g.js.n;
+    g.ini += ni;
+    g.icall++;
+    Device &d = device();
+    double *di = static_cast<double *>(g.di.get((size_t)8 * ni * sizeof(double)));
+    if (jp_iblock(ni, addr, h2, dtr, di, d.stream) != OK) return;
+    regf_finish(g, ni, di, *lmax_, *nnbmax_, *m_flag, acc, jrk, pot, list, d.stream, t0);
 }
 
 int nb6_rsnet_nparam(void) { return rsnet_nparam(); }

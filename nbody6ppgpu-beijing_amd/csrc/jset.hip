@@ -88,12 +88,15 @@ __global__ __launch_bounds__(256) void soa_to_aos_kernel(const double *__restric
     o[7] = 0.0;
 }
 
-// bounding box of positions and maximum mass: one work-group, 256 threads
+// bounding box of positions and maximum mass, pass 1: each work-group reduces a
+// strided share of the records into one partial row of 7
+constexpr int STATS_GROUPS = 256;
+
 __global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ aos, int n,
-                                                    double *__restrict__ out /* 7 */) {
+                                                    double *__restrict__ part /* [groups][7] */) {
     __shared__ double sh[7][256];
     double v[7] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX, 0.0};
-    for (int j = threadIdx.x; j < n; j += 256) {
+    for (int j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) {
         const double *r = aos + (size_t)j * 8;
         for (int d = 0; d < 3; d++) {
             v[d] = fmin(v[d], r[d]);
@@ -110,7 +113,27 @@ __global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ a
         }
         __syncthreads();
     }
-    if (threadIdx.x < 7) out[threadIdx.x] = sh[threadIdx.x][0];
+    if (threadIdx.x < 7) part[blockIdx.x * 7 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// pass 2: one work-group reduces the partial rows
+__global__ __launch_bounds__(256) void stats_final_kernel(const double *__restrict__ part, int groups,
+                                                          double *__restrict__ out /* 7 */) {
+    __shared__ double sh[7][256];
+    const int t = threadIdx.x;
+    for (int q = 0; q < 7; q++) {
+        const double id = q < 3 ? DBL_MAX : (q < 6 ? -DBL_MAX : 0.0);
+        sh[q][t] = t < groups ? part[t * 7 + q] : id;
+    }
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) {
+            for (int d = 0; d < 3; d++) sh[d][t] = fmin(sh[d][t], sh[d][t + o]);
+            for (int d = 3; d < 7; d++) sh[d][t] = fmax(sh[d][t], sh[d][t + o]);
+        }
+        __syncthreads();
+    }
+    if (t < 7) out[t] = sh[t][0];
 }
 
 // Morton key of i-particle positions (SoA rows 0-2) in a given box (clamped)
@@ -203,7 +226,7 @@ void jset_reserve(JSet &js, int cap) {
     js.perm.get((size_t)cap * sizeof(int));
     js.keys.get((size_t)cap * 2 * sizeof(unsigned));
     js.vals.get((size_t)cap * sizeof(int));
-    js.stat.get(8 * sizeof(double));
+    js.stat.get((8 + 7 * STATS_GROUPS) * sizeof(double));
     js.tbox.get((size_t)ntile * TBOX * sizeof(float));
 }
 
@@ -229,7 +252,10 @@ void jset_build(JSet &js, int n, const double *box, double mmax, hipStream_t s) 
     if (n <= 0) return;
     double st[7];
     if (!box) {
-        hipLaunchKernelGGL(stats_kernel, dim3(1), dim3(256), 0, s, js.in64.as<double>(), n,
+        const int groups = std::min(STATS_GROUPS, (n + 255) / 256);
+        double *part = js.stat.as<double>() + 8;
+        hipLaunchKernelGGL(stats_kernel, dim3(groups), dim3(256), 0, s, js.in64.as<double>(), n, part);
+        hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(256), 0, s, part, groups,
                            js.stat.as<double>());
         NB6_HIP(hipGetLastError());
         NB6_HIP(hipMemcpyAsync(st, js.stat.p, 7 * sizeof(double), hipMemcpyDeviceToHost, s));

@@ -166,6 +166,14 @@ int regf_autotune(const JSet &js, int ni, int ni_stride, const double *d_i, int 
                   int m_flag, double *d_out, int *d_list, int *best, hipStream_t s);
 int ens_autotune(const double *d_i, int stride, int lmax, int nnbmax, int m_flag, double *d_out,
                  int *d_list, int *best, hipStream_t s);
+// device-resident predictor (jpred.hip)
+int jp_open(int nmax);
+int jp_close();
+int jp_set(int n, const int *addr, const double (*x0)[3], const double (*v0)[3],
+           const double (*f)[3], const double (*fdot)[3], const double *m, const double *t0);
+int jp_predict(double time, int nj, double *aos, hipStream_t s);
+int jp_iblock(int ni, const int *addr, const double *h2, const double *dtr, double *d_ib,
+              hipStream_t s);
 // the j-set of the open gpunb library (nullptr when closed)
 const JSet *gpunb_jset();
 

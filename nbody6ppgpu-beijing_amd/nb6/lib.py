@@ -200,6 +200,41 @@ class IrrSimd:
             self.L.irr_simd_close_(_ri(self.rank))
 
 
+class JPred:
+    """Device-resident predictor (nb6_jp_*): upload corrected particles, predict all
+    of them to a block time on the GPU and run gpunb_regf for blocks by address.
+    Needs an open Gpunb."""
+
+    def __init__(self, nmax: int):
+        self.L = load()
+        self.L.nb6_jp_open_(_ri(nmax))
+        check_error()
+
+    def set(self, addr, x0, v0, f, fdot, m, t0):
+        addr = np.ascontiguousarray(addr, np.int32)
+        x0, v0, f, fdot, m, t0 = (c64(a) for a in (x0, v0, f, fdot, m, t0))
+        self.L.nb6_jp_set_(_ri(addr.size), _i(addr), _d(x0), _d(v0), _d(f), _d(fdot), _d(m), _d(t0))
+        check_error()
+
+    def predict_send(self, time, nj):
+        self.L.nb6_jp_predict_send_(_rd(time), _ri(nj))
+        check_error()
+
+    def regf(self, addr, h2, dtr, lmax, nnbmax, m_flag=0):
+        addr = np.ascontiguousarray(addr, np.int32)
+        h2, dtr = c64(h2), c64(dtr)
+        ni = addr.size
+        acc = np.zeros((ni, 3)); jrk = np.zeros((ni, 3)); pot = np.zeros(ni)
+        lst = np.zeros((ni, lmax), dtype=np.int32)
+        self.L.nb6_jp_regf_(_ri(ni), _i(addr), _d(h2), _d(dtr), _d(acc), _d(jrk), _d(pot),
+                            _ri(lmax), _ri(nnbmax), _i(lst), _ri(m_flag))
+        check_error()
+        return acc, jrk, pot, lst
+
+    def close(self):
+        self.L.nb6_jp_close_()
+
+
 class DeviceArray:
     """Minimal owned device buffer for the device-resident entry points."""
 
