@@ -264,7 +264,7 @@ def bench_learned_rs(train_samples, eval_steps, no_cpu):
     for pol in ("rule", "net"):
         t = time.time()
         s = Sim(m2, x2, v2, nnbopt=nnbopt, lmax=lmax)
-        s.start()
+        s.start(pol, net if pol == "net" else None)
         s.run(eval_steps, pol, net=net if pol == "net" else None)
         st = dict(s.stats)
         st["wall_s"] = time.time() - t

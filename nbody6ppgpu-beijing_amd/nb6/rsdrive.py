@@ -129,8 +129,10 @@ class Sim:
         return xp, vp
 
     # ------------------------------------------------------------ start-up
-    def start(self):
-        """fpoly0.F: full force, then lists of everybody with block re-runs."""
+    def start(self, policy="rule", net=None):
+        """fpoly0.F: full force, then lists of everybody with block re-runs.  The
+        start-up lists give every particle its first features; with policy "net"
+        the network also sets the radius of the first regular step."""
         self._send(self.x, self.v)
         out, _ = self._regf(self.x, self.v, np.zeros(self.n), np.zeros(self.n), self.d.tmp, 2, 1)
         self.a, self.j = out[0:3].T.copy(), out[3:6].T.copy()
@@ -154,6 +156,18 @@ class Sim:
         self.stats["startup_reruns"] = reruns
         self._globals()
         self.t = 0.0
+        allp = np.arange(self.n)
+        feats = np.empty((self.n, 8), np.float32)
+        for b in range(0, self.n, 65536):
+            sl = allp[b:b + 65536]
+            feats[sl] = self.features(sl, self.x[sl], self.a[sl], self.j[sl], self.rs[sl],
+                                      self.dt[sl], self.nnb[sl])
+        self.feat_prev[:] = feats
+        self.rs_used_prev[:] = self.rs
+        self.has_prev[:] = True
+        if policy == "net":
+            self.rs = net.predict(feats, self.rs, self.rs, np.zeros(self.n, np.int32))
+            self.used_net[:] = True
         self.stats["calls"] = 0
 
     def _globals(self):
@@ -218,6 +232,8 @@ class Sim:
         self.stats["miss30"] += int((dev > 0.3).sum())
         self.stats["absdev"] += float(np.abs(np.log((nnb_new + 1.0) / (self.nnbopt + 1.0))).sum())
         self.stats["samples"] += ni
+        if len(self.stats.setdefault("overflow_by_step", [])) < 8:
+            self.stats["overflow_by_step"].append(self.stats["overflow_events"])
 
         # the rule's inputs: block x, v, dtr, counts, the lists just built and every
         # particle's predicted position and velocity

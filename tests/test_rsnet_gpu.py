@@ -233,7 +233,7 @@ def test_learned_radius_beats_the_rule_on_a_short_run():
         for pol in ("rule", "net"):
             s = Sim(m, x, v, nnbopt=50, lmax=300)
             try:
-                s.start()
+                s.start(pol, net if pol == "net" else None)
                 st = s.run(120, pol, net=net if pol == "net" else None)
                 stats[pol] = dict(st)
             finally:
@@ -241,7 +241,8 @@ def test_learned_radius_beats_the_rule_on_a_short_run():
     finally:
         net.close()
     for pol, st in stats.items():
-        print(pol, {k: st[k] for k in ("samples", "overflow_events", "miss30", "absdev", "fallback")},
+        print(pol, {k: st[k] for k in ("samples", "overflow_events", "miss30", "absdev", "fallback",
+                                       "overflow_by_step")},
               "mean |ln(c/NNBOPT)|", st["absdev"] / st["samples"])
     r, q = stats["rule"], stats["net"]
     assert q["absdev"] / q["samples"] < 0.7 * r["absdev"] / r["samples"]
