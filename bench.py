@@ -386,6 +386,91 @@ def bench_precision(no_cpu):
                              "median_rel_err_vs_fp64": float(np.median(ep))}}
 
 
+def bench_irr(steps, warmup, no_cpu):
+    """Irregular (neighbour-list) force gather at N=1M (the regf workload): lists of
+    the ni=131072 i-set come straight from the device regf rows, predictor records
+    from the device predictor store; one step = irr_simd_firr for all ni.
+    Algorithmic bytes per active i: nnb (4 B index + 72 B record) + 72 B + 52 B."""
+    m, x, v, idx, h2, dtr = make_workload()
+    L = lib.load()
+    n, ni = m.size, idx.size
+    rng = np.random.Generator(np.random.PCG64(SEED + 80))
+    f = rng.normal(0.0, 0.1, (n, 3))
+    fd = rng.normal(0.0, 0.1, (n, 3))
+    t0 = np.zeros(n)
+    ti = 2.0 ** -8
+    g = lib.Gpunb(n)
+    jp = lib.JPred(n)
+    irr = lib.IrrSimd(n, LMAX)
+    addr = np.arange(1, n + 1, dtype=np.int32)
+    jp.set(addr, x, v, f, fd, m, t0)
+    jp.predict_send(ti, n)
+    L.nb6_irr_sync_jp_(lib._ri(n))
+    lib.check_error()
+    # lists: the regf call of the i-set at the predicted state (device rows)
+    xp = x + ti * (v + ti * (f + ti * fd))
+    vp = v + 2 * ti * (f + 1.5 * ti * fd)
+    ib = np.ascontiguousarray(np.concatenate([xp[idx].T, vp[idx].T, h2[None], dtr[None]]))
+    di = lib.DeviceArray.from_host(ib)
+    do = lib.DeviceArray(7 * ni * 8)
+    dl = lib.DeviceArray(ni * LMAX * 4)
+    if L.nb6_regf_device(ni, ni, di.ptr, LMAX, NNBMAX, 0, do.ptr, dl.ptr, None) != 0:
+        lib.check_error()
+    da = lib.DeviceArray.from_host((idx + 1).astype(np.int32))
+    if L.nb6_irr_set_lists_device(ni, da.ptr, dl.ptr, LMAX, 1, None) != 0:
+        lib.check_error()
+    lst = dl.to_host(np.int32, (ni, LMAX))
+    nnb_tot = float(np.maximum(lst[:, 0] - 1, 0).sum())
+    dout = lib.DeviceArray(ni * 8 * 8)
+    for _ in range(warmup):
+        L.nb6_irr_firr_device(ti, ni, da.ptr, dout.ptr, None)
+    L.nb6_sync()
+    kms = []
+    t_0 = time.perf_counter()
+    for _ in range(steps):
+        if L.nb6_irr_firr_device(ti, ni, da.ptr, dout.ptr, None) != 0:
+            lib.check_error()
+        kms.append(L.nb6_kernel_ms(b"irr"))
+    L.nb6_sync()
+    wall = (time.perf_counter() - t_0) / steps
+    out = dout.to_host(np.float64, (ni, 8))
+    kt = float(np.mean(kms)) * 1e-3
+    alg = nnb_tot * (4 + 72) + ni * (72 + 52)
+    check = None
+    if not no_cpu:
+        import oracle as O
+        k = 128
+        lists = np.zeros((k, LMAX), np.int32)
+        for q in range(k):
+            c = lst[q, 0]
+            row = [j + 1 for j in lst[q, 1:1 + c] if j != idx[q]]
+            lists[q, 0] = len(row)
+            lists[q, 1:1 + len(row)] = row
+        full = np.zeros((n, LMAX), np.int32)
+        full[idx[:k]] = lists
+        racc, rjrk, rnn = O.firr(ti, idx[:k] + 1, x, v, f, fd, m, t0, full, LMAX)
+        ea = np.linalg.norm(out[:k, 0:3] - racc, axis=1) / np.linalg.norm(racc, axis=1)
+        ej = np.linalg.norm(out[:k, 3:6] - rjrk, axis=1) / np.linalg.norm(rjrk, axis=1)
+        check = {"sample_i": k, "acc_max": float(ea.max()), "jrk_max": float(ej.max()),
+                 "nnbid_equal": bool(np.array_equal(out[:k, 6].astype(int), rnn))}
+    for d in (di, do, dl, da, dout):
+        d.free()
+    irr.close()
+    jp.close()
+    g.close()
+    return {"metric": "irregular-force gather: algorithmic HBM GB/s (King W0=6 N=1M, ni=131072)",
+            "value": alg / kt / 1e9, "unit": "GB/s", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": wall * 1e3, "higher_is_better": True,
+            "dtype": "fp32 pair math, fp64 positions and sums",
+            "data": "synthetic (seeded King W0=6, 50% binaries, Kroupa masses)",
+            "config": {"workload": "irr", "n": n, "n_i": ni, "mean_nnb": nnb_tot / ni, "lmax": LMAX},
+            "roofline": {"bound": "hbm", "achieved": round(alg / kt / 1e9, 1), "peak": 8000.0,
+                         "unit": "GB/s", "frac": round(alg / kt / 8e12, 4),
+                         "algorithmic_bytes": alg, "kernel": "irr_force_kernel",
+                         "avg_launch_ms": kt * 1e3},
+            "check_vs_fp64": check}
+
+
 def _rs_accuracy(s, k=128):
     """Regular force of k particles with the policy's current RS against FP64 (checker)."""
     import oracle as O
@@ -409,7 +494,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--workload", default="regf",
-                    choices=["regf", "ensemble", "learned-rs", "precision"],
+                    choices=["regf", "ensemble", "learned-rs", "precision", "irr"],
                     help="regf: the headline N=1M step; ensemble: 256-cluster batch vs loop; "
                          "learned-rs: overflow retries of the RS network vs the reference rule")
     ap.add_argument("--rs-train-samples", type=int, default=10_000_000)
@@ -433,6 +518,9 @@ def main():
 
     if args.workload == "ensemble":
         print(json.dumps(bench_ensemble(args.steps, args.warmup)), flush=True)
+        return 0
+    if args.workload == "irr":
+        print(json.dumps(bench_irr(args.steps, args.warmup, args.no_cpu)), flush=True)
         return 0
     if args.workload == "precision":
         print(json.dumps(bench_precision(args.no_cpu)), flush=True)

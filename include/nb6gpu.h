@@ -214,6 +214,23 @@ void nb6_jp_predict_send_(double *time, int *nj);
 void nb6_jp_regf_(int *ni, int addr[], double h2[], double dtr[], double acc[][3],
                   double jrk[][3], double pot[], int *lmax, int *nnbmax, int *list, int *m_flag);
 
+/* Irregular force fed from the device (extension; replaces the host list round
+ * trip util_gpu.F:101-110 -> IRR_SIMD_SET_LIST, intgrt.F:932, and the per-particle
+ * IRR_SIMD_SET_JP uploads):
+ *   nb6_irr_sync_jp_(n): predictor records of addresses 1..n from the
+ *     nb6_jp_* store (both libraries open).
+ *   nb6_irr_set_lists_device(ni, d_addr, d_list, lmax, jbase, stream): the lists of
+ *     the particles at 1-based addresses d_addr[ni] from gpunb_regf rows d_list
+ *     [ni][lmax] (device), j -> address j + jbase (jbase = IFIRST), self dropped,
+ *     overflow rows skipped; the same effect as IRR_SIMD_SET_LIST with the list the
+ *     caller would have built.
+ *   nb6_irr_firr_device(ti, ni, d_addr, d_out, stream): irr_simd_firr_vec_ with
+ *     device addresses and d_out double[ni][8] = acc[3], jrk[3], nnbid (1-based), nnb. */
+void nb6_irr_sync_jp_(int *n);
+int nb6_irr_set_lists_device(int ni, const int *d_addr, const int *d_list, int lmax, int jbase,
+                             void *stream);
+int nb6_irr_firr_device(double ti, int ni, const int *d_addr, double *d_out, void *stream);
+
 /* ---------------------------------------------------------------------------
  * Part 3 (extension): learned neighbour radius.  A network 8 -> 64 -> 64 -> 1
  * (ReLU; bfloat16 weights, FP32 accumulation; 4801 parameters, layout W1[64][8],

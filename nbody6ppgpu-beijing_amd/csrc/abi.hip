@@ -612,6 +612,21 @@ void nb6_jp_regf_(int *ni_, int addr[], double h2[], double dtr[], double acc[][
     regf_finish(g, ni, di, *lmax_, *nnbmax_, *m_flag, acc, jrk, pot, list, d.stream, t0);
 }
 
+void nb6_irr_sync_jp_(int *n) {
+    Device &d = device();
+    if (!d.ready) return;
+    (void)irr_sync_jp(*n, d.stream);
+}
+int nb6_irr_set_lists_device(int ni, const int *d_addr, const int *d_list, int lmax, int jbase,
+                             void *stream) {
+    if (!device().ready) return E_NODEV;
+    return irr_set_lists_device(ni, d_addr, d_list, lmax, jbase, pick(stream));
+}
+int nb6_irr_firr_device(double ti, int ni, const int *d_addr, double *d_out, void *stream) {
+    if (!device().ready) return E_NODEV;
+    return irr_firr_device(ti, ni, d_addr, d_out, pick(stream));
+}
+
 int nb6_rsnet_nparam(void) { return rsnet_nparam(); }
 int nb6_rsnet_open(unsigned long long seed) { return rsnet_open(seed); }
 int nb6_rsnet_close(void) { return rsnet_close(); }

@@ -47,7 +47,7 @@ struct IrrState {
     std::vector<size_t> ls_off;    // offsets (ints) of live entries in ls_log
     size_t ls_used = 0;            // ints used
     DBuf d_jp, d_ls, d_lsoff;
-    DBuf d_addr, d_out;
+    DBuf d_addr, d_out, d_bad;
     HBuf h_addr, h_out;
     // profile
     double time_grav = 0.0;
@@ -82,6 +82,64 @@ __global__ void irr_scatter_list(const int *__restrict__ log, const long long *_
     int *row = lists + (size_t)a * lmax;
     if (lane == 0) row[0] = nnb;
     for (int k = lane; k < nnb; k += 64) row[1 + k] = e[2 + k] - 1;
+}
+
+// regf rows -> irregular-force lists, one wavefront per row: row i holds the
+// gpunb_regf list (count, 0-based j of the sent j-set, the particle itself
+// included) of the particle at 1-based address addr[i]; j becomes the 0-based
+// address j + jbase - 1 (util_gpu.F:101-110 adds IFIRST and drops self).  Rows
+// with a negative (overflow) count leave the particle's list unchanged.
+__global__ __launch_bounds__(256) void irr_lists_from_regf(int ni, const int *__restrict__ addr,
+                                                           const int *__restrict__ rl, int lmax_r,
+                                                           int jbase, int *__restrict__ lists,
+                                                           int lmax, int nmax,
+                                                           int *__restrict__ bad) {
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (w >= ni) return;
+    const int a = addr[w] - 1;
+    if (a < 0 || a >= nmax) {
+        if (lane == 0) atomicAdd(bad, 1);
+        return;
+    }
+    const int *src = rl + (size_t)w * lmax_r;
+    const int c = min(src[0], lmax_r - 1);
+    if (c < 0) return;
+    int *row = lists + (size_t)a * lmax;
+    int n = 0;
+    for (int kb = 0; kb < c; kb += 64) {
+        const int k = kb + lane;
+        int e = -1;
+        if (k < c) e = src[1 + k] + jbase - 1;
+        const bool keep = k < c && e != a && e >= 0 && e < nmax;
+        const unsigned long long b = __ballot(keep);
+        const int pos = n + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((unsigned)b, 0u));
+        if (keep && pos < lmax - 1) row[1 + pos] = e;
+        n += __popcll(b);
+    }
+    if (lane == 0) {
+        row[0] = min(n, lmax - 1);
+        if (n > lmax - 1) atomicAdd(bad, 1);
+    }
+}
+
+// predictor records from the device predictor store (SoA [14][jmax]: x0[3], x0dot[3],
+// F = a/2 [3], FDOT = adot/6 [3], t0, m) for addresses [0, n)
+__global__ __launch_bounds__(256) void irr_from_jp(int n, const double *__restrict__ st, int jmax,
+                                                   IrrRec *__restrict__ rec,
+                                                   double *__restrict__ tim) {
+    const int a = blockIdx.x * 256 + threadIdx.x;
+    if (a >= n) return;
+    const size_t M = jmax;
+    IrrRec r;
+    r.x = st[a]; r.y = st[M + a]; r.z = st[2 * M + a];
+    r.vx = (float)st[3 * M + a]; r.vy = (float)st[4 * M + a]; r.vz = (float)st[5 * M + a];
+    r.ax = (float)st[6 * M + a]; r.ay = (float)st[7 * M + a]; r.az = (float)st[8 * M + a];
+    r.jx = (float)st[9 * M + a]; r.jy = (float)st[10 * M + a]; r.jz = (float)st[11 * M + a];
+    r.m = (float)st[13 * M + a];
+    rec[a] = r;
+    tim[a] = st[12 * M + a];
 }
 
 struct Pred {
@@ -223,7 +281,7 @@ void irr_release() {
     IrrState &g = g_irr;
     g.rec.release(); g.tim.release(); g.lists.release();
     g.d_jp.release(); g.d_ls.release(); g.d_lsoff.release();
-    g.d_addr.release(); g.d_out.release();
+    g.d_addr.release(); g.d_out.release(); g.d_bad.release();
     g.jp_log.release(); g.ls_log.release(); g.h_addr.release(); g.h_out.release();
     g.jp_pos.clear(); g.ls_pos.clear(); g.ls_off.clear();
     g.jp_n = 0; g.ls_used = 0;
@@ -379,6 +437,63 @@ int irr_firr_vec(double ti, int ni, const int *addr, double (*acc)[3], double (*
     }
     g.num_inter += ninter;
     g.time_grav += t.ms() * 1e-3;
+    g.num_fcall++;
+    g.num_steps += ni;
+    return OK;
+}
+
+// lists of ni particles straight from gpunb_regf rows on the device (no host round trip)
+int irr_set_lists_device(int ni, const int *d_addr, const int *d_list, int lmax_r, int jbase,
+                         hipStream_t s) {
+    IrrState &g = g_irr;
+    if (!g.open) { set_error(E_STATE, "nb6_irr_set_lists_device: irr library not open"); return E_STATE; }
+    if (ni <= 0) return OK;
+    if (lmax_r < 1) { set_error(E_ARG, "nb6_irr_set_lists_device: lmax must be >= 1"); return E_ARG; }
+    flush_logs(s);   // earlier host set_list calls land first
+    int *bad = static_cast<int *>(g.d_bad.get(sizeof(int)));
+    NB6_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
+    hipLaunchKernelGGL(irr_lists_from_regf, dim3((ni + 3) / 4), dim3(256), 0, s, ni, d_addr, d_list,
+                       lmax_r, jbase, g.lists.as<int>(), g.lmax, g.nmax, bad);
+    NB6_HIP(hipGetLastError());
+    int hb = 0;
+    NB6_HIP(hipMemcpyAsync(&hb, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    NB6_HIP(hipStreamSynchronize(s));
+    if (hb) {
+        set_error(E_ARG, "nb6_irr_set_lists_device: address out of range or list longer than lmax-1");
+        return E_ARG;
+    }
+    return OK;
+}
+
+// predictor records of addresses [0, n) from the device predictor store
+int irr_sync_jp(int n, hipStream_t s) {
+    IrrState &g = g_irr;
+    if (!g.open) { set_error(E_STATE, "nb6_irr_sync_jp: irr library not open"); return E_STATE; }
+    int jmax = 0;
+    const double *st = jp_state(&jmax);
+    if (!st) { set_error(E_STATE, "nb6_irr_sync_jp: predictor store not open"); return E_STATE; }
+    if (n < 0 || n > g.nmax || n > jmax) { set_error(E_ARG, "nb6_irr_sync_jp: n outside both stores"); return E_ARG; }
+    flush_logs(s);
+    if (n > 0) {
+        hipLaunchKernelGGL(irr_from_jp, dim3((n + 255) / 256), dim3(256), 0, s, n, st, jmax,
+                           g.rec.as<IrrRec>(), g.tim.as<double>());
+        NB6_HIP(hipGetLastError());
+    }
+    return OK;
+}
+
+// irregular force of ni particles (1-based device addresses), results on the device
+int irr_firr_device(double ti, int ni, const int *d_addr, double *d_out, hipStream_t s) {
+    IrrState &g = g_irr;
+    if (!g.open) { set_error(E_STATE, "nb6_irr_firr_device: irr library not open"); return E_STATE; }
+    if (ni <= 0) return OK;
+    flush_logs(s);
+    KTimer &t = ktimer("irr");
+    t.start(s);
+    hipLaunchKernelGGL(irr_force_kernel, dim3((ni + 3) / 4), dim3(256), 0, s, ti, ni, d_addr,
+                       g.rec.as<IrrRec>(), g.tim.as<double>(), g.lists.as<int>(), g.lmax, d_out);
+    NB6_HIP(hipGetLastError());
+    t.stop(s);
     g.num_fcall++;
     g.num_steps += ni;
     return OK;

@@ -161,6 +161,12 @@ int jp_predict(double time, int nj, double *aos, hipStream_t s) {
     return OK;
 }
 
+const double *jp_state(int *nmax) {
+    if (!g_jp.open) return nullptr;
+    *nmax = g_jp.nmax;
+    return g_jp.st.as<double>();
+}
+
 // i-block SoA [8][ni] on the device from 1-based addresses and host h2, dtr
 int jp_iblock(int ni, const int *addr, const double *h2, const double *dtr, double *d_ib,
               hipStream_t s) {

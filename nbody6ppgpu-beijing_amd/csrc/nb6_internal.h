@@ -174,6 +174,12 @@ int jp_set(int n, const int *addr, const double (*x0)[3], const double (*v0)[3],
 int jp_predict(double time, int nj, double *aos, hipStream_t s);
 int jp_iblock(int ni, const int *addr, const double *h2, const double *dtr, double *d_ib,
               hipStream_t s);
+const double *jp_state(int *nmax);   // predictor store SoA [14][nmax] (nullptr when closed)
+// irregular force fed from the device (irr.hip)
+int irr_set_lists_device(int ni, const int *d_addr, const int *d_list, int lmax_r, int jbase,
+                         hipStream_t s);
+int irr_sync_jp(int n, hipStream_t s);
+int irr_firr_device(double ti, int ni, const int *d_addr, double *d_out, hipStream_t s);
 // the j-set of the open gpunb library (nullptr when closed)
 const JSet *gpunb_jset();
 

@@ -97,3 +97,54 @@ def test_firr_golden_inputs():
     e_avx = np.linalg.norm(g["acc"] - racc, axis=1) / np.linalg.norm(racc, axis=1)
     assert ea.max() <= REL_TOL and ea.max() <= e_avx.max()
     assert (nn == rnn).all()
+
+
+def test_firr_fed_from_device_equals_host_path():
+    """Predictor records from the nb6_jp_* store and lists straight from the device
+    gpunb_regf rows give the same bits as irr_simd_set_jp_ / irr_simd_set_list_
+    with the host lists (util_gpu.F:101-110: +IFIRST, self dropped) and
+    irr_simd_firr_vec_."""
+    m, x, v, a2, j6, t0, _ = irr_state(n=2048, seed=8)
+    n = m.size
+    ti = 64.0 / 1024.0
+    rng = np.random.default_rng(9)
+    blk = np.sort(rng.choice(n, 600, replace=False))
+    from nb6 import ics
+    rs = ics.rs_for_counts(x, blk, np.full(blk.size, 30))
+    h2, dtr = rs * rs, np.full(blk.size, 0.02)
+    addr = np.arange(1, n + 1, dtype=np.int32)
+    L = lib.load()
+    g = lib.Gpunb(n)
+    jp = lib.JPred(n)
+    irr = lib.IrrSimd(n, 400)
+    try:
+        jp.set(addr, x, v, a2, j6, m, t0)
+        jp.predict_send(ti, n)
+        acc, jrk, pot, lst = jp.regf(blk + 1, h2, dtr, 400, 350)
+        # host path
+        for i in range(n):
+            irr.set_jp(i + 1, x[i], v[i], a2[i], j6[i], m[i], t0[i])
+        for k, i in enumerate(blk):
+            c = lst[k, 0]
+            row = [j + 1 for j in lst[k, 1:1 + c] if j != i]
+            irr.set_list(i + 1, np.r_[len(row), row])
+        ha, hj, hn = irr.firr_vec(ti, blk + 1)
+        # device path: overwrite the records and lists from the device, same call
+        L.nb6_irr_sync_jp_(lib._ri(n))
+        lib.check_error()
+        ib = np.ascontiguousarray(np.concatenate([np.zeros((6, blk.size)), h2[None], dtr[None]]))
+        da = lib.DeviceArray.from_host((blk + 1).astype(np.int32))
+        dl = lib.DeviceArray.from_host(np.ascontiguousarray(lst))
+        do = lib.DeviceArray(blk.size * 8 * 8)
+        assert L.nb6_irr_set_lists_device(blk.size, da.ptr, dl.ptr, 400, 1, None) == 0
+        assert L.nb6_irr_firr_device(ti, blk.size, da.ptr, do.ptr, None) == 0
+        out = do.to_host(np.float64, (blk.size, 8))
+        for d in (da, dl, do):
+            d.free()
+        del ib
+    finally:
+        irr.close()
+        jp.close()
+        g.close()
+    assert np.array_equal(out[:, 0:3], ha) and np.array_equal(out[:, 3:6], hj)
+    assert np.array_equal(out[:, 6].astype(int), hn)
