@@ -176,46 +176,70 @@ __device__ __forceinline__ double wsum(double v) {
     return v;
 }
 
+// One group of IRR_LPI lanes per active i (4 i per wavefront): with ~100 neighbours a
+// row keeps its lanes busy, and each lane keeps two neighbour gathers in flight.
+constexpr int IRR_LPI = 16;
+
+// pair terms in FP32, the lane's running sums in FP64 (a lane sums several pairs)
+__device__ __forceinline__ void irr_pair(const Pred &pi, const Pred &pj, int j, double &ax,
+                                         double &ay, double &az, double &jx, double &jy, double &jz,
+                                         unsigned long long &best) {
+    const float dx = (float)(pj.x - pi.x) + (pj.dx - pi.dx);
+    const float dy = (float)(pj.y - pi.y) + (pj.dy - pi.dy);
+    const float dz = (float)(pj.z - pi.z) + (pj.dz - pi.dz);
+    const float du = pj.vx - pi.vx, dv = pj.vy - pi.vy, dw = pj.vz - pi.vz;
+    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    const float rv = fmaf(dz, dw, fmaf(dy, dv, dx * du));
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float rinv2 = rinv * rinv;
+    const float mr3 = pj.m * rinv * rinv2;
+    const float al = -3.0f * rv * rinv2;
+    ax += (double)(mr3 * dx);
+    ay += (double)(mr3 * dy);
+    az += (double)(mr3 * dz);
+    jx += (double)(mr3 * fmaf(al, dx, du));
+    jy += (double)(mr3 * fmaf(al, dy, dv));
+    jz += (double)(mr3 * fmaf(al, dz, dw));
+    const unsigned long long key = ((unsigned long long)__float_as_uint(r2) << 32) | (unsigned)j;
+    best = key < best ? key : best;
+}
+
+__device__ __forceinline__ double gsum(double v) {
+#pragma unroll
+    for (int o = IRR_LPI / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
 __global__ __launch_bounds__(256) void irr_force_kernel(
     double ti, int ni, const int *__restrict__ addr, const IrrRec *__restrict__ rec,
     const double *__restrict__ tim, const int *__restrict__ lists, int lmax,
     double *__restrict__ out /* [ni][8]: acc3, jrk3, nnbid, nnb */) {
-    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
+    const int w = blockIdx.x * (256 / IRR_LPI) + (threadIdx.x / IRR_LPI);
+    const int lane = threadIdx.x % IRR_LPI;
+    // the whole group leaves together; the shuffles below stay within a group
     if (w >= ni) return;
     const int ia = addr[w] - 1;
     const Pred pi = predict(rec, tim, ia, ti);
     const int *row = lists + (size_t)ia * lmax;
     const int nnb = row[0];
-    float ax = 0.f, ay = 0.f, az = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
+    double ax = 0.0, ay = 0.0, az = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
     unsigned long long best = ~0ull;
-    for (int k = lane; k < nnb; k += 64) {
-        const int j = row[1 + k];
-        const Pred pj = predict(rec, tim, j, ti);
-        const float dx = (float)(pj.x - pi.x) + (pj.dx - pi.dx);
-        const float dy = (float)(pj.y - pi.y) + (pj.dy - pi.dy);
-        const float dz = (float)(pj.z - pi.z) + (pj.dz - pi.dz);
-        const float du = pj.vx - pi.vx, dv = pj.vy - pi.vy, dw = pj.vz - pi.vz;
-        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        const float rv = fmaf(dz, dw, fmaf(dy, dv, dx * du));
-        const float rinv = __builtin_amdgcn_rsqf(r2);
-        const float rinv2 = rinv * rinv;
-        const float mr3 = pj.m * rinv * rinv2;
-        const float al = -3.0f * rv * rinv2;
-        ax = fmaf(mr3, dx, ax);
-        ay = fmaf(mr3, dy, ay);
-        az = fmaf(mr3, dz, az);
-        jx = fmaf(mr3, fmaf(al, dx, du), jx);
-        jy = fmaf(mr3, fmaf(al, dy, dv), jy);
-        jz = fmaf(mr3, fmaf(al, dz, dw), jz);
-        const unsigned long long key =
-            ((unsigned long long)__float_as_uint(r2) << 32) | (unsigned)j;
-        best = key < best ? key : best;
+    int k = lane;
+    for (; k + IRR_LPI < nnb; k += 2 * IRR_LPI) {
+        const int j0 = row[1 + k], j1 = row[1 + k + IRR_LPI];
+        const Pred p0 = predict(rec, tim, j0, ti);
+        const Pred p1 = predict(rec, tim, j1, ti);
+        irr_pair(pi, p0, j0, ax, ay, az, jx, jy, jz, best);
+        irr_pair(pi, p1, j1, ax, ay, az, jx, jy, jz, best);
     }
-    const double sax = wsum((double)ax), say = wsum((double)ay), saz = wsum((double)az);
-    const double sjx = wsum((double)jx), sjy = wsum((double)jy), sjz = wsum((double)jz);
+    if (k < nnb) {
+        const int j0 = row[1 + k];
+        irr_pair(pi, predict(rec, tim, j0, ti), j0, ax, ay, az, jx, jy, jz, best);
+    }
+    const double sax = gsum(ax), say = gsum(ay), saz = gsum(az);
+    const double sjx = gsum(jx), sjy = gsum(jy), sjz = gsum(jz);
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
+    for (int o = IRR_LPI / 2; o > 0; o >>= 1) {
         const unsigned long long other = __shfl_xor(best, o, 64);
         best = other < best ? other : best;
     }
@@ -421,7 +445,7 @@ int irr_firr_vec(double ti, int ni, const int *addr, double (*acc)[3], double (*
     NB6_HIP(hipMemcpyAsync(da, ha, (size_t)ni * sizeof(int), hipMemcpyHostToDevice, s));
     KTimer &t = ktimer("irr");
     t.start(s);
-    hipLaunchKernelGGL(irr_force_kernel, dim3((ni + 3) / 4), dim3(256), 0, s, ti, ni, da,
+    hipLaunchKernelGGL(irr_force_kernel, dim3((ni + 256 / IRR_LPI - 1) / (256 / IRR_LPI)), dim3(256), 0, s, ti, ni, da,
                        g.rec.as<IrrRec>(), g.tim.as<double>(), g.lists.as<int>(), g.lmax, dout);
     NB6_HIP(hipGetLastError());
     t.stop(s);
@@ -490,7 +514,7 @@ int irr_firr_device(double ti, int ni, const int *d_addr, double *d_out, hipStre
     flush_logs(s);
     KTimer &t = ktimer("irr");
     t.start(s);
-    hipLaunchKernelGGL(irr_force_kernel, dim3((ni + 3) / 4), dim3(256), 0, s, ti, ni, d_addr,
+    hipLaunchKernelGGL(irr_force_kernel, dim3((ni + 256 / IRR_LPI - 1) / (256 / IRR_LPI)), dim3(256), 0, s, ti, ni, d_addr,
                        g.rec.as<IrrRec>(), g.tim.as<double>(), g.lists.as<int>(), g.lmax, d_out);
     NB6_HIP(hipGetLastError());
     t.stop(s);

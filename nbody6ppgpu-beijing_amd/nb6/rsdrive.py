@@ -92,6 +92,7 @@ class Sim:
         self.rs_used_prev = np.ones(n)
         self.has_prev = np.zeros(n, bool)
         self.used_net = np.zeros(n, bool)
+        self.hook = None
 
     # ------------------------------------------------------------ device helpers
     def close(self):
@@ -103,6 +104,7 @@ class Sim:
         return 2.0 ** np.floor(np.log2(dt))
 
     def _send(self, xp, vp):
+        self._jx, self._jv = xp, vp
         soa = np.ascontiguousarray(np.concatenate([xp.T, vp.T, self.m[None]]))
         self.L.nb6_memcpy_htod(self.d.j.ptr, soa.ctypes.data, soa.nbytes)
         if self.L.nb6_send_device(self.n, self.n, self.d.j.ptr, None) != 0:
@@ -195,7 +197,9 @@ class Sim:
         lists come from one call for the block plus calls on the overflowed subset,
         while the re-runs of the reference are counted (events, pairs)."""
         ni = rs.size
-        _, rows = self._regf(xi, vi, rs ** 2, dtr, self.d.lst, self.lmax, self.nnbmax)
+        out, rows = self._regf(xi, vi, rs ** 2, dtr, self.d.lst, self.lmax, self.nnbmax)
+        if self.hook is not None:   # e.g. a check against an FP64 reference at every call
+            self.hook(self._jx, self._jv, xi, vi, rs ** 2, dtr, out, rows)
         cnt = rows[:, 0].copy()
         todo = np.nonzero(cnt < 0)[0]
         while todo.size:
@@ -354,6 +358,14 @@ class Sim:
         for _ in range(nsteps):
             self.block_step(policy, net, record)
         return self.stats
+
+    def energy(self):
+        """Kinetic + potential energy at the common time of the last full block."""
+        xp, vp = self.predict(self.t)
+        self._send(xp, vp)
+        out, _ = self._regf(xp, vp, np.zeros(self.n), np.zeros(self.n), self.d.tmp, 2, 1)
+        ekin = 0.5 * (self.m[:, None] * vp ** 2).sum()
+        return ekin - 0.5 * (self.m * out[6]).sum()
 
 
 def _overflow(rs, nnb, nnbopt, nnbmax):
