@@ -21,31 +21,43 @@ namespace nb6 {
 
 namespace {
 
-__device__ __forceinline__ unsigned spread10(unsigned a) {
-    a &= 0x3ffu;
-    a = (a | (a << 16)) & 0x030000FFu;
-    a = (a | (a << 8)) & 0x0300F00Fu;
-    a = (a | (a << 4)) & 0x030C30C3u;
-    a = (a | (a << 2)) & 0x09249249u;
+// 21 bits per axis: a cluster with escapers far out still gets a fine order in its core
+__device__ __forceinline__ unsigned long long spread21(unsigned long long a) {
+    a &= 0x1fffffull;
+    a = (a | (a << 32)) & 0x1f00000000ffffull;
+    a = (a | (a << 16)) & 0x1f0000ff0000ffull;
+    a = (a | (a << 8)) & 0x100f00f00f00f00full;
+    a = (a | (a << 4)) & 0x10c30c30c30c30c3ull;
+    a = (a | (a << 2)) & 0x1249249249249249ull;
     return a;
+}
+
+constexpr double MORTON_CELLS = 2097152.0;   // 2^21 per axis
+
+__device__ __forceinline__ unsigned long long morton63(double x, double y, double z, double lx,
+                                                       double ly, double lz, double sx, double sy,
+                                                       double sz) {
+    const double fx = (x - lx) * sx, fy = (y - ly) * sy, fz = (z - lz) * sz;
+    // comparisons in FP64 first: NaN and far-outside values clamp without overflow
+    const unsigned qx = fx > 0.0 ? (fx < MORTON_CELLS - 1.0 ? (unsigned)fx : 2097151u) : 0u;
+    const unsigned qy = fy > 0.0 ? (fy < MORTON_CELLS - 1.0 ? (unsigned)fy : 2097151u) : 0u;
+    const unsigned qz = fz > 0.0 ? (fz < MORTON_CELLS - 1.0 ? (unsigned)fz : 2097151u) : 0u;
+    return spread21(qx) | (spread21(qy) << 1) | (spread21(qz) << 2);
 }
 
 __global__ __launch_bounds__(256) void morton_kernel(const double *__restrict__ in, int n,
                                                      double lx, double ly, double lz, double sx,
                                                      double sy, double sz,
-                                                     unsigned *__restrict__ key,
+                                                     unsigned long long *__restrict__ key,
                                                      int *__restrict__ val) {
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= n) return;
     const double *r = in + (size_t)j * 8;
-    const int qx = min(1023, max(0, (int)((r[0] - lx) * sx)));
-    const int qy = min(1023, max(0, (int)((r[1] - ly) * sy)));
-    const int qz = min(1023, max(0, (int)((r[2] - lz) * sz)));
-    key[j] = spread10(qx) | (spread10(qy) << 1) | (spread10(qz) << 2);
+    key[j] = morton63(r[0], r[1], r[2], lx, ly, lz, sx, sy, sz);
     val[j] = j;
 }
 
-// sorted position s <- original j = perm[s]; tile origin = sorted particle t*TJ
+// sorted position s <- original j = perm[s]; tile origin = the tile's middle record
 __global__ __launch_bounds__(256) void build_kernel(const double *__restrict__ in, int n,
                                                     const int *__restrict__ perm,
                                                     double *__restrict__ rec64,
@@ -54,7 +66,8 @@ __global__ __launch_bounds__(256) void build_kernel(const double *__restrict__ i
     const int s = blockIdx.x * 256 + threadIdx.x;
     if (s >= n) return;
     const int t = s / JTILE;
-    const double *p = in + (size_t)perm[t * JTILE] * 8;
+    const int c = min(JTILE, n - t * JTILE);
+    const double *p = in + (size_t)perm[t * JTILE + c / 2] * 8;
     const double *r = in + (size_t)perm[s] * 8;
     double *o = rec64 + (size_t)s * 8;
 #pragma unroll
@@ -140,17 +153,12 @@ __global__ __launch_bounds__(256) void stats_final_kernel(const double *__restri
 __global__ __launch_bounds__(256) void imorton_kernel(const double *__restrict__ ib, int stride,
                                                       int ni, double lx, double ly, double lz,
                                                       double sx, double sy, double sz,
-                                                      unsigned *__restrict__ key,
+                                                      unsigned long long *__restrict__ key,
                                                       int *__restrict__ val) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= ni) return;
-    const double x = ib[i], y = ib[(size_t)stride + i], z = ib[2 * (size_t)stride + i];
-    const double fx = (x - lx) * sx, fy = (y - ly) * sy, fz = (z - lz) * sz;
-    // comparisons in FP64 first: NaN and far-outside values clamp without overflow
-    const int qx = fx > 0.0 ? (fx < 1023.0 ? (int)fx : 1023) : 0;
-    const int qy = fy > 0.0 ? (fy < 1023.0 ? (int)fy : 1023) : 0;
-    const int qz = fz > 0.0 ? (fz < 1023.0 ? (int)fz : 1023) : 0;
-    key[i] = spread10(qx) | (spread10(qy) << 1) | (spread10(qz) << 2);
+    key[i] = morton63(ib[i], ib[(size_t)stride + i], ib[2 * (size_t)stride + i], lx, ly, lz, sx,
+                      sy, sz);
     val[i] = i;
 }
 
@@ -224,7 +232,7 @@ void jset_reserve(JSet &js, int cap) {
     js.rec32.get((size_t)cap * 2 * sizeof(float4));
     js.torg.get((size_t)ntile * 8 * sizeof(double));
     js.perm.get((size_t)cap * sizeof(int));
-    js.keys.get((size_t)cap * 2 * sizeof(unsigned));
+    js.keys.get((size_t)cap * 2 * sizeof(unsigned long long));
     js.vals.get((size_t)cap * sizeof(int));
     js.stat.get((8 + 7 * STATS_GROUPS) * sizeof(double));
     js.tbox.get((size_t)ntile * TBOX * sizeof(float));
@@ -268,19 +276,19 @@ void jset_build(JSet &js, int n, const double *box, double mmax, hipStream_t s) 
     double sc[3];
     for (int d = 0; d < 3; d++) {
         const double w = box[3 + d] - box[d];
-        sc[d] = w > 0.0 ? 1024.0 / (w * (1.0 + 1e-12)) : 0.0;
+        sc[d] = w > 0.0 ? MORTON_CELLS / (w * (1.0 + 1e-12)) : 0.0;
     }
-    unsigned *k0 = js.keys.as<unsigned>();
-    unsigned *k1 = k0 + js.cap;
+    unsigned long long *k0 = js.keys.as<unsigned long long>();
+    unsigned long long *k1 = k0 + js.cap;
     int *v0 = js.vals.as<int>();
     int *v1 = js.perm.as<int>();
     hipLaunchKernelGGL(morton_kernel, dim3((n + 255) / 256), dim3(256), 0, s, js.in64.as<double>(),
                        n, box[0], box[1], box[2], sc[0], sc[1], sc[2], k0, v0);
     NB6_HIP(hipGetLastError());
     size_t tb = 0;
-    NB6_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, n, 0, 30, s));
+    NB6_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, n, 0, 63, s));
     void *tmp = js.sort_tmp.get(tb);
-    NB6_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, v1, n, 0, 30, s));
+    NB6_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, v1, n, 0, 63, s));
     hipLaunchKernelGGL(build_kernel, dim3((n + 255) / 256), dim3(256), 0, s, js.in64.as<double>(), n,
                        js.perm.as<int>(), js.rec64.as<double>(), js.rec32.as<float4>(),
                        js.torg.as<double>());
@@ -293,19 +301,20 @@ const int *jset_sort_i(const JSet &js, int ni, int stride, const double *d_i, Re
     double sc[3];
     for (int d = 0; d < 3; d++) {
         const double wd = js.box[3 + d] - js.box[d];
-        sc[d] = wd > 0.0 ? 1024.0 / (wd * (1.0 + 1e-12)) : 0.0;
+        sc[d] = wd > 0.0 ? MORTON_CELLS / (wd * (1.0 + 1e-12)) : 0.0;
     }
-    unsigned *k0 = static_cast<unsigned *>(w.ikey.get((size_t)ni * 2 * sizeof(unsigned)));
-    unsigned *k1 = k0 + ni;
+    unsigned long long *k0 =
+        static_cast<unsigned long long *>(w.ikey.get((size_t)ni * 2 * sizeof(unsigned long long)));
+    unsigned long long *k1 = k0 + ni;
     int *v0 = static_cast<int *>(w.ival.get((size_t)ni * 2 * sizeof(int)));
     int *v1 = v0 + ni;
     hipLaunchKernelGGL(imorton_kernel, dim3((ni + 255) / 256), dim3(256), 0, s, d_i, stride, ni,
                        js.box[0], js.box[1], js.box[2], sc[0], sc[1], sc[2], k0, v0);
     NB6_HIP(hipGetLastError());
     size_t tb = 0;
-    NB6_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, ni, 0, 30, s));
+    NB6_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, ni, 0, 63, s));
     void *tmp = w.itmp.get(tb);
-    NB6_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, v1, ni, 0, 30, s));
+    NB6_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, v1, ni, 0, 63, s));
     return v1;
 }
 

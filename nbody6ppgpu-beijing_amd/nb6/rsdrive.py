@@ -99,6 +99,17 @@ class Sim:
         self.d.free()
         self.g.close()
 
+    def _step(self, a, j, cap, t):
+        """eta |a| / |adot| as a power of two in [dtmin, min(SMAX, cap)] that divides t."""
+        an = np.linalg.norm(a, axis=1)
+        jn = np.linalg.norm(j, axis=1) + 1e-300
+        d = np.minimum(self._pow2(self.eta * an / jn), cap)
+        while True:
+            bad = np.fmod(t, d) != 0.0
+            if not bad.any():
+                return d
+            d[bad] *= 0.5
+
     def _pow2(self, dt):
         dt = np.clip(dt, self.dtmin, self.smax)
         return 2.0 ** np.floor(np.log2(dt))
@@ -139,6 +150,8 @@ class Sim:
         out, _ = self._regf(self.x, self.v, np.zeros(self.n), np.zeros(self.n), self.d.tmp, 2, 1)
         self.a, self.j = out[0:3].T.copy(), out[3:6].T.copy()
         self.pot = out[6].copy()
+        # first regular steps from the force criterion, at most the fpoly0.F estimate
+        self.dt = np.minimum(self.dt, self._step(self.a, self.j, np.inf, 0.0))
         reruns = 0
         for b in range(0, self.n, self.maxthr):
             idx = np.arange(b, min(self.n, b + self.maxthr))
@@ -275,15 +288,7 @@ class Sim:
         self.a[act], self.j[act] = a1, j1
         self.pot[act] = out[6]
         self.t0[act] = t
-        # next regular step: eta |a| / |adot| as a power of two, commensurate with t
-        an = np.linalg.norm(a1, axis=1)
-        jn = np.linalg.norm(j1, axis=1) + 1e-300
-        dnew = np.minimum(self._pow2(self.eta * an / jn), 2.0 * dtr)
-        while True:
-            bad = np.fmod(t, dnew) != 0.0
-            if not bad.any():
-                break
-            dnew[bad] *= 0.5
+        dnew = self._step(a1, j1, 2.0 * dtr, t)
 
         # RS of the block's next regular step
         feats = self.features(act, xi, a1, j1, rs, dtr, nnb_new)
