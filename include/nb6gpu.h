@@ -226,6 +226,10 @@ void nb6_jp_regf_(int *ni, int addr[], double h2[], double dtr[], double acc[][3
  *     caller would have built.
  *   nb6_irr_firr_device(ti, ni, d_addr, d_out, stream): irr_simd_firr_vec_ with
  *     device addresses and d_out double[ni][8] = acc[3], jrk[3], nnbid (1-based), nnb. */
+/* Launch shape of the irregular-force gather: 16 or 64 lanes per active particle,
+ * and whether blocks of >= 2048 active particles are processed in Morton order of
+ * their positions (results unchanged).  Defaults: 64, 1. */
+int nb6_set_irr_config(int lanes_per_i, int sort_i);
 void nb6_irr_sync_jp_(int *n);
 int nb6_irr_set_lists_device(int ni, const int *d_addr, const int *d_list, int lmax, int jbase,
                              void *stream);

@@ -19,6 +19,8 @@
 // (r^2 bits, index) minimum.
 #include "nb6_internal.h"
 
+#include <hipcub/hipcub.hpp>
+
 #include <cstring>
 #include <vector>
 
@@ -47,7 +49,7 @@ struct IrrState {
     std::vector<size_t> ls_off;    // offsets (ints) of live entries in ls_log
     size_t ls_used = 0;            // ints used
     DBuf d_jp, d_ls, d_lsoff;
-    DBuf d_addr, d_out, d_bad;
+    DBuf d_addr, d_out, d_bad, d_okey, d_oval, d_otmp;
     HBuf h_addr, h_out;
     // profile
     double time_grav = 0.0;
@@ -170,17 +172,16 @@ __device__ __forceinline__ Pred predict(const IrrRec *__restrict__ rec,
     return o;
 }
 
-__device__ __forceinline__ double wsum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
+// Launch shape of the irregular-force gather: LPI lanes per active i (16: 4 i per
+// wavefront; 64: one i per wavefront), each lane keeps two neighbour gathers in
+// flight; the i are taken in the order of `order` (Morton order of their positions
+// when sorting is on) and consecutive work-groups go to the same XCD, so one XCD's
+// L2 serves one compact region of the cluster.
+struct IrrConfig {
+    int lpi = 64;
+    int sort_i = 1;
+} g_irrcfg;
 
-// One group of IRR_LPI lanes per active i (4 i per wavefront): with ~100 neighbours a
-// row keeps its lanes busy, and each lane keeps two neighbour gathers in flight.
-constexpr int IRR_LPI = 16;
-
-// pair terms in FP32, the lane's running sums in FP64 (a lane sums several pairs)
 __device__ __forceinline__ void irr_pair(const Pred &pi, const Pred &pj, int j, double &ax,
                                          double &ay, double &az, double &jx, double &jy, double &jz,
                                          unsigned long long &best) {
@@ -194,6 +195,7 @@ __device__ __forceinline__ void irr_pair(const Pred &pi, const Pred &pj, int j, 
     const float rinv2 = rinv * rinv;
     const float mr3 = pj.m * rinv * rinv2;
     const float al = -3.0f * rv * rinv2;
+    // pair terms in FP32, the lane's running sums in FP64
     ax += (double)(mr3 * dx);
     ay += (double)(mr3 * dy);
     az += (double)(mr3 * dz);
@@ -204,20 +206,33 @@ __device__ __forceinline__ void irr_pair(const Pred &pi, const Pred &pj, int j, 
     best = key < best ? key : best;
 }
 
+template <int LPI>
 __device__ __forceinline__ double gsum(double v) {
 #pragma unroll
-    for (int o = IRR_LPI / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    for (int o = LPI / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
 
+// work-group b of G -> position in the launch order such that XCD x (= b mod 8)
+// receives a contiguous range of work
+__device__ __forceinline__ int xcd_order(int b, int G) {
+    const int xcd = b & 7, k = b >> 3;
+    const int per = G >> 3, rem = G & 7;
+    return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
+}
+
+template <int LPI>
 __global__ __launch_bounds__(256) void irr_force_kernel(
-    double ti, int ni, const int *__restrict__ addr, const IrrRec *__restrict__ rec,
-    const double *__restrict__ tim, const int *__restrict__ lists, int lmax,
+    double ti, int ni, const int *__restrict__ addr, const int *__restrict__ order,
+    const IrrRec *__restrict__ rec, const double *__restrict__ tim,
+    const int *__restrict__ lists, int lmax,
     double *__restrict__ out /* [ni][8]: acc3, jrk3, nnbid, nnb */) {
-    const int w = blockIdx.x * (256 / IRR_LPI) + (threadIdx.x / IRR_LPI);
-    const int lane = threadIdx.x % IRR_LPI;
+    const int blk = xcd_order(blockIdx.x, gridDim.x);
+    const int q = blk * (256 / LPI) + (threadIdx.x / LPI);
+    const int lane = threadIdx.x % LPI;
     // the whole group leaves together; the shuffles below stay within a group
-    if (w >= ni) return;
+    if (q >= ni) return;
+    const int w = order ? order[q] : q;
     const int ia = addr[w] - 1;
     const Pred pi = predict(rec, tim, ia, ti);
     const int *row = lists + (size_t)ia * lmax;
@@ -225,8 +240,8 @@ __global__ __launch_bounds__(256) void irr_force_kernel(
     double ax = 0.0, ay = 0.0, az = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
     unsigned long long best = ~0ull;
     int k = lane;
-    for (; k + IRR_LPI < nnb; k += 2 * IRR_LPI) {
-        const int j0 = row[1 + k], j1 = row[1 + k + IRR_LPI];
+    for (; k + LPI < nnb; k += 2 * LPI) {
+        const int j0 = row[1 + k], j1 = row[1 + k + LPI];
         const Pred p0 = predict(rec, tim, j0, ti);
         const Pred p1 = predict(rec, tim, j1, ti);
         irr_pair(pi, p0, j0, ax, ay, az, jx, jy, jz, best);
@@ -236,10 +251,10 @@ __global__ __launch_bounds__(256) void irr_force_kernel(
         const int j0 = row[1 + k];
         irr_pair(pi, predict(rec, tim, j0, ti), j0, ax, ay, az, jx, jy, jz, best);
     }
-    const double sax = gsum(ax), say = gsum(ay), saz = gsum(az);
-    const double sjx = gsum(jx), sjy = gsum(jy), sjz = gsum(jz);
+    const double sax = gsum<LPI>(ax), say = gsum<LPI>(ay), saz = gsum<LPI>(az);
+    const double sjx = gsum<LPI>(jx), sjy = gsum<LPI>(jy), sjz = gsum<LPI>(jz);
 #pragma unroll
-    for (int o = IRR_LPI / 2; o > 0; o >>= 1) {
+    for (int o = LPI / 2; o > 0; o >>= 1) {
         const unsigned long long other = __shfl_xor(best, o, 64);
         best = other < best ? other : best;
     }
@@ -251,6 +266,42 @@ __global__ __launch_bounds__(256) void irr_force_kernel(
         o[7] = (double)nnb;
     }
 }
+
+// Morton key of each active i's record position (for the processing order)
+__global__ __launch_bounds__(256) void irr_key_kernel(int ni, const int *__restrict__ addr,
+                                                      const IrrRec *__restrict__ rec, double lx,
+                                                      double ly, double lz, double sx, double sy,
+                                                      double sz, unsigned *__restrict__ key,
+                                                      int *__restrict__ val) {
+    const int w = blockIdx.x * 256 + threadIdx.x;
+    if (w >= ni) return;
+    const IrrRec &r = rec[addr[w] - 1];
+    const double f[3] = {(r.x - lx) * sx, (r.y - ly) * sy, (r.z - lz) * sz};
+    unsigned q[3];
+    for (int d = 0; d < 3; d++) q[d] = f[d] > 0.0 ? (f[d] < 1023.0 ? (unsigned)f[d] : 1023u) : 0u;
+    unsigned kk = 0;
+    for (int bit = 9; bit >= 0; bit--)
+        for (int d = 2; d >= 0; d--) kk = (kk << 1) | ((q[d] >> bit) & 1u);
+    key[w] = kk;
+    val[w] = w;
+}
+
+void launch_irr(double ti, int ni, const int *d_addr, const int *order, const IrrState &g,
+                double *d_out, hipStream_t s) {
+    const int lpi = g_irrcfg.lpi;
+    const dim3 grid((ni + 256 / lpi - 1) / (256 / lpi));
+    if (lpi == 16)
+        hipLaunchKernelGGL(irr_force_kernel<16>, grid, dim3(256), 0, s, ti, ni, d_addr, order,
+                           g.rec.as<IrrRec>(), g.tim.as<double>(), g.lists.as<int>(), g.lmax, d_out);
+    else
+        hipLaunchKernelGGL(irr_force_kernel<64>, grid, dim3(256), 0, s, ti, ni, d_addr, order,
+                           g.rec.as<IrrRec>(), g.tim.as<double>(), g.lists.as<int>(), g.lmax, d_out);
+    NB6_HIP(hipGetLastError());
+}
+
+// processing order of the active i: Morton order of their record positions in the
+// box of the gpunb j-set (nullptr: as given, when sorting is off or no box is known)
+const int *irr_order(int ni, const int *d_addr, hipStream_t s);
 
 void flush_logs(hipStream_t s) {
     IrrState &g = g_irr;
@@ -301,11 +352,48 @@ void flush_logs(hipStream_t s) {
 
 }  // namespace
 
+namespace {
+const int *irr_order(int ni, const int *d_addr, hipStream_t s) {
+    IrrState &g = g_irr;
+    const JSet *js = gpunb_jset();
+    if (!g_irrcfg.sort_i || ni < 2048 || !js || js->n <= 0) return nullptr;
+    double sc[3];
+    for (int d = 0; d < 3; d++) {
+        const double w = js->box[3 + d] - js->box[d];
+        sc[d] = w > 0.0 ? 1024.0 / (w * (1.0 + 1e-12)) : 0.0;
+    }
+    unsigned *k0 = static_cast<unsigned *>(g.d_okey.get((size_t)ni * 2 * sizeof(unsigned)));
+    unsigned *k1 = k0 + ni;
+    int *v0 = static_cast<int *>(g.d_oval.get((size_t)ni * 2 * sizeof(int)));
+    int *v1 = v0 + ni;
+    hipLaunchKernelGGL(irr_key_kernel, dim3((ni + 255) / 256), dim3(256), 0, s, ni, d_addr,
+                       g.rec.as<IrrRec>(), js->box[0], js->box[1], js->box[2], sc[0], sc[1], sc[2],
+                       k0, v0);
+    NB6_HIP(hipGetLastError());
+    size_t tb = 0;
+    NB6_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, ni, 0, 30, s));
+    void *tmp = g.d_otmp.get(tb);
+    NB6_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, v1, ni, 0, 30, s));
+    return v1;
+}
+}  // namespace
+
+int irr_config(int lanes_per_i, int sort_i) {
+    if ((lanes_per_i != 16 && lanes_per_i != 64) || (sort_i != 0 && sort_i != 1)) {
+        set_error(E_ARG, "nb6_set_irr_config: lanes_per_i must be 16 or 64 and sort_i 0 or 1");
+        return E_ARG;
+    }
+    g_irrcfg.lpi = lanes_per_i;
+    g_irrcfg.sort_i = sort_i;
+    return OK;
+}
+
 void irr_release() {
     IrrState &g = g_irr;
     g.rec.release(); g.tim.release(); g.lists.release();
     g.d_jp.release(); g.d_ls.release(); g.d_lsoff.release();
     g.d_addr.release(); g.d_out.release(); g.d_bad.release();
+    g.d_okey.release(); g.d_oval.release(); g.d_otmp.release();
     g.jp_log.release(); g.ls_log.release(); g.h_addr.release(); g.h_out.release();
     g.jp_pos.clear(); g.ls_pos.clear(); g.ls_off.clear();
     g.jp_n = 0; g.ls_used = 0;
@@ -443,11 +531,10 @@ int irr_firr_vec(double ti, int ni, const int *addr, double (*acc)[3], double (*
     double *dout = static_cast<double *>(g.d_out.get((size_t)ni * 8 * sizeof(double)));
     double *hout = static_cast<double *>(g.h_out.get((size_t)ni * 8 * sizeof(double)));
     NB6_HIP(hipMemcpyAsync(da, ha, (size_t)ni * sizeof(int), hipMemcpyHostToDevice, s));
+    const int *order = irr_order(ni, da, s);
     KTimer &t = ktimer("irr");
     t.start(s);
-    hipLaunchKernelGGL(irr_force_kernel, dim3((ni + 256 / IRR_LPI - 1) / (256 / IRR_LPI)), dim3(256), 0, s, ti, ni, da,
-                       g.rec.as<IrrRec>(), g.tim.as<double>(), g.lists.as<int>(), g.lmax, dout);
-    NB6_HIP(hipGetLastError());
+    launch_irr(ti, ni, da, order, g, dout, s);
     t.stop(s);
     NB6_HIP(hipMemcpyAsync(hout, dout, (size_t)ni * 8 * sizeof(double), hipMemcpyDeviceToHost, s));
     NB6_HIP(hipStreamSynchronize(s));
@@ -512,11 +599,10 @@ int irr_firr_device(double ti, int ni, const int *d_addr, double *d_out, hipStre
     if (!g.open) { set_error(E_STATE, "nb6_irr_firr_device: irr library not open"); return E_STATE; }
     if (ni <= 0) return OK;
     flush_logs(s);
+    const int *order = irr_order(ni, d_addr, s);
     KTimer &t = ktimer("irr");
     t.start(s);
-    hipLaunchKernelGGL(irr_force_kernel, dim3((ni + 256 / IRR_LPI - 1) / (256 / IRR_LPI)), dim3(256), 0, s, ti, ni, d_addr,
-                       g.rec.as<IrrRec>(), g.tim.as<double>(), g.lists.as<int>(), g.lmax, d_out);
-    NB6_HIP(hipGetLastError());
+    launch_irr(ti, ni, d_addr, order, g, d_out, s);
     t.stop(s);
     g.num_fcall++;
     g.num_steps += ni;

@@ -180,6 +180,7 @@ int irr_set_lists_device(int ni, const int *d_addr, const int *d_list, int lmax_
                          hipStream_t s);
 int irr_sync_jp(int n, hipStream_t s);
 int irr_firr_device(double ti, int ni, const int *d_addr, double *d_out, hipStream_t s);
+int irr_config(int lanes_per_i, int sort_i);
 // the j-set of the open gpunb library (nullptr when closed)
 const JSet *gpunb_jset();
 
