@@ -376,12 +376,15 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
     }
 }
 
-// waves per SIMD the register budget is sized for: 5 at one i per lane (<= 96 VGPRs)
-template <int IPL>
-constexpr int regf_waves() { return IPL == 1 ? 5 : 2; }
+// waves per SIMD the register budget is sized for, at one i per lane: 6 (<= 80 VGPRs;
+// the few spills are set-up values reloaded once per tile) for 512-wide work-groups,
+// measured 5 % faster on the N = 2^20 step than 5; 5 (<= 96 VGPRs, no spills) for
+// narrower ones, where 6 measured slower
+template <int T, int IPL>
+constexpr int regf_waves() { return IPL == 1 ? (T >= 512 ? 6 : 5) : 2; }
 
 template <int T, int IPL, bool MFLAG>
-__global__ __launch_bounds__(T, regf_waves<IPL>()) void regf_tile_kernel(TileParams p) {
+__global__ __launch_bounds__(T, (regf_waves<T, IPL>())) void regf_tile_kernel(TileParams p) {
     __shared__ float4 sA[2][T];   // x', y', z', m
     __shared__ float4 sB[2][T];   // vx', vy', vz', 0
     const int chunk = blockIdx.y;
@@ -393,7 +396,7 @@ __global__ __launch_bounds__(T, regf_waves<IPL>()) void regf_tile_kernel(TilePar
 // Persistent form for a batch of independent clusters: each work-group takes the
 // next work item from a queue (items are ordered longest first) until none is left.
 template <int T, int IPL, bool MFLAG>
-__global__ __launch_bounds__(T, regf_waves<IPL>()) void regf_items_kernel(TileParams p,
+__global__ __launch_bounds__(T, (regf_waves<T, IPL>())) void regf_items_kernel(TileParams p,
                                                        const int4 *__restrict__ items,
                                                        int nitems, int *__restrict__ queue) {
     __shared__ float4 sA[2][T];
