@@ -299,7 +299,8 @@ void gpunb_regf_(int *ni_, double h2[], double dtr[], double xi[][3], double vi[
     Gpunb &g = g_nb;
     if (!g.open) { set_error(E_STATE, "gpunb_regf: library not open"); return; }
     const int ni = *ni_, lmax = *lmax_, nnbmax = *nnbmax_;
-    if (ni <= 0) { set_error(E_ARG, "gpunb_regf: ni must be positive"); return; }
+    if (ni < 0) { set_error(E_ARG, "gpunb_regf: ni must not be negative"); return; }
+    if (ni == 0) return;   // an empty block (the CUDA back end launches no work either)
     if (lmax < 1) { set_error(E_ARG, "gpunb_regf: lmax must be >= 1"); return; }
     if (!check_finite(h2, ni) || !check_finite(dtr, ni) || !check_finite(&xi[0][0], 3 * (size_t)ni) ||
         !check_finite(&vi[0][0], 3 * (size_t)ni)) {
@@ -450,7 +451,8 @@ int nb6_regf_device(int ni, int ni_stride, const double *d_i, int lmax, int nnbm
                     double *d_out, int *d_list, void *stream) {
     Gpunb &g = g_nb;
     if (!g.open) { set_error(E_STATE, "nb6_regf_device: library not open"); return E_STATE; }
-    if (ni <= 0 || ni_stride < ni) { set_error(E_ARG, "nb6_regf_device: bad ni / stride"); return E_ARG; }
+    if (ni < 0 || ni_stride < ni) { set_error(E_ARG, "nb6_regf_device: bad ni / stride"); return E_ARG; }
+    if (ni == 0) return OK;
     Device &d = device();
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : d.stream;
     g.num_inter += (long long)ni * g.js.n;

@@ -177,6 +177,15 @@ __device__ __forceinline__ bool lane_far(const IC &c, bool ok, const float *__re
     return fmaf(-0x1p-16f * X, X, fminf(dd, dp)) >= ths * (1.0f + 0x1p-8f);
 }
 
+typedef __attribute__((address_space(3))) void lds_void;
+typedef __attribute__((address_space(1))) void glb_void;
+
+// j record beyond the end of a range: far and massless, never a candidate, adds zero
+__device__ const float4 g_far_rec[2] = {{1.0e17f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+
+// wait for this wavefront's outstanding global loads, LDS DMA included
+__device__ __forceinline__ void lds_dma_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 struct PairBase {
     float dx, dy, dz, du, dv, dw, r2, rv;
 };
@@ -286,26 +295,25 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
 
     const int ntile = (j1 - j0 + T - 1) / T;
 
-    float4 ga, gb;
-    auto gload = [&](int t) {
+    // Tile t goes straight from HBM into LDS buffer t & 1 (global_load_lds_dwordx4:
+    // lane l of a wavefront writes 16 bytes at the wavefront's LDS base + 16 l), so the
+    // next tile's records occupy no registers while the current one is evaluated.
+    // Lanes past j1 read the far, massless dummy record.
+    const int w0 = tid & ~63;
+    auto gload = [&](int t, int b) {
         const int j = j0 + t * T + tid;
-        if (j < j1) {
-            ga = p.rec32[2 * (size_t)j];
-            gb = p.rec32[2 * (size_t)j + 1];
-        } else {   // far, massless dummy: never a candidate, contributes zero
-            ga = make_float4(1.0e17f, 0.f, 0.f, 0.f);
-            gb = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
+        const float4 *src = j < j1 ? p.rec32 + 2 * (size_t)j : g_far_rec;
+        __builtin_amdgcn_global_load_lds((glb_void *)src, (lds_void *)&sA[b][w0], 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((glb_void *)(src + 1), (lds_void *)&sB[b][w0], 16, 0, 0);
     };
 
-    gload(0);
-    sA[0][tid] = ga;
-    sB[0][tid] = gb;
+    gload(0, 0);
+    lds_dma_wait();
     __syncthreads();
 
     for (int t = 0; t < ntile; t++) {
         const int buf = t & 1;
-        if (t + 1 < ntile) gload(t + 1);
+        if (t + 1 < ntile) gload(t + 1, buf ^ 1);
         const int jt = j0 + t * T;
         IC c[IPL];
         bool allfar = false;
@@ -356,10 +364,7 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
             }
         }
 
-        if (t + 1 < ntile) {
-            sA[buf ^ 1][tid] = ga;
-            sB[buf ^ 1][tid] = gb;
-        }
+        lds_dma_wait();   // this wavefront's part of tile t + 1 is in LDS
         __syncthreads();
     }
 

@@ -268,3 +268,53 @@ def test_regf_beats_reference_precision_on_golden_inputs(gpunb, case):
     e_ours = np.abs(pot[ok] - rpot[ok]) / rpot[ok]
     e_ref = np.abs(g["pot"][ok] - rpot[ok]) / rpot[ok]
     assert e_ours.max() <= e_ref.max() and np.median(e_ours) <= np.median(e_ref)
+
+
+def test_regf_edge_cases(gpunb):
+    """ni = 0; a single j; lmax = 1 (count only: every non-empty row overflows);
+    coincident distinct particles (r2 = 0: always neighbours when h2 > 0, never
+    evaluated otherwise); i-particles far outside the j-set's box."""
+    L = lib.load()
+    m, x, v, h2, dtr = c1_inputs(n=512)
+    n = m.size
+    g = gpunb(n)
+    # a single j and an i at distance 1
+    g.send(m[:1], x[:1], v[:1])
+    acc, jrk, pot, lst = g.regf(np.array([1e-4]), np.zeros(1), x[:1] + [[1.0, 0, 0]], v[:1],
+                                10, 5)
+    ref = O.regf(m[:1], x[:1], v[:1], np.array([1e-4]), np.zeros(1), x[:1] + [[1.0, 0, 0]],
+                 v[:1], 10, 5)
+    assert np.allclose(acc, ref[0], rtol=1e-6) and np.allclose(pot, ref[2], rtol=1e-6)
+    assert lst[0, 0] == 0
+    # ni = 0 is a no-op (no error)
+    L.nb6_set_abort_on_error(0)
+    try:
+        g.send(m, x, v)
+        L.nb6_clear_error()
+        rc = L.nb6_regf_device(0, 1, None, 10, 5, 0, None, None, None)
+        assert rc == 0 and L.nb6_last_error() == 0
+    finally:
+        L.nb6_set_abort_on_error(1)
+    # coincident particles: duplicate 20 of them in the j-set
+    mx = np.r_[m, m[:20]]
+    xx = np.r_[x, x[:20]]
+    vx = np.r_[v, v[:20]]
+    g.close()
+    g = gpunb(mx.size)
+    g.send(mx, xx, vx)
+    idx = np.arange(40)
+    got = g.regf(h2[idx], dtr[idx], xx[idx], vx[idx], 200, 150)
+    ref = O.regf(mx, xx, vx, h2[idx], dtr[idx], xx[idx], vx[idx], 200, 150)
+    assert rows_as_sets(got[3]) == rows_as_sets(ref[3])
+    for k in range(20):
+        assert {k, n + k} <= set(got[3][k, 1:1 + got[3][k, 0]])
+    assert np.allclose(got[0], ref[0], rtol=1e-6) and np.allclose(got[2], ref[2], rtol=1e-6)
+    # lmax = 1: only the count, always -count for a non-empty row
+    got = g.regf(h2[idx], dtr[idx], xx[idx], vx[idx], 1, 0)
+    assert (got[3][:, 0] == -ref[3][:, 0]).all()
+    # i far outside the j box: no neighbours, the far field in FP32 to 1e-6
+    far = xx[:8] + np.array([50.0, -30.0, 20.0])
+    got = g.regf(h2[:8], dtr[:8], far, vx[:8], 200, 150)
+    ref = O.regf(mx, xx, vx, h2[:8], dtr[:8], far, vx[:8], 200, 150)
+    assert (got[3][:, 0] == 0).all()
+    assert vec_rel_err(got[0], ref[0]).max() <= 1e-6 and vec_rel_err(got[1], ref[1]).max() <= 1e-6
