@@ -297,6 +297,65 @@ int nb6_rs_rule_device(int n, int stride, const double *d_x, const double *d_v,
 int nb6_rs_kth_device(int ni, int ib_stride, const double *d_ib, const int *d_list, int lmax,
                       int k, double *d_rs, void *stream);
 
+/* ---------------------------------------------------------------------------
+ * Part 4 (extension): the regular corrector on the GPU.  Replaces the loop of
+ * util_gpu.F:150-175 that calls REGCOR_GPU (regcor_gpu.F:1-740, with FPCORR,
+ * fpcorr.F:1-148) once per particle of a regular block, on per-particle state
+ * held in device memory.  Every array is a device pointer indexed by particle
+ * number - 1; [3][nmax] arrays are component-major (row k = component k + 1).
+ * ------------------------------------------------------------------------- */
+typedef struct nb6_regstate {
+    int nmax;                         /* stride of every per-particle array */
+    int lmax;                         /* row length of list (LMAX) */
+    /* read */
+    const double *x, *xdot;           /* [3][nmax] X, XDOT predicted to the block time */
+    const double *body;               /* [nmax] BODY */
+    const double *x0, *x0dot;         /* [3][nmax] X0, X0DOT */
+    const double *f, *fdot;           /* [3][nmax] F = a/2, FDOT = adot/6 (FPCORR) */
+    const double *t0;                 /* [nmax] T0 */
+    const double *h;                  /* [nmax - N] H of KS pairs (c.m. bodies I > N); may
+                                         be null when no active particle exceeds N */
+    /* read and written */
+    double *step, *timenw;            /* [nmax] STEP, TIMENW */
+    double *fi, *fidot, *fr, *frdot;  /* [3][nmax] FI, FIDOT, FR, FRDOT */
+    double *d2, *d3;                  /* [3][nmax] D2, D3 (FPCORR) */
+    double *t0r, *stepr, *rs;         /* [nmax] T0R, STEPR, RS */
+    int *list;                        /* int[nmax][lmax] LIST: count, then 1-based ascending */
+    /* written */
+    double *xn, *xndot;               /* [3][nmax] XN, XNDOT */
+    double *d0, *d0r, *d1r, *d2r, *d3r;  /* [3][nmax] D0, D0R, D1R, D2R, D3R */
+} nb6_regstate;
+
+typedef struct nb6_regcor_params {
+    double time;                      /* TIME of the block */
+    double dtmin, smin, smax, etar;   /* DTMIN, SMIN (= 2 DTMIN, adjust.F:192), SMAX, ETAR */
+    double rscale, rc2, rsfac, znbmin, bodym;  /* RSCALE, RC^2, RSFAC, ZNBMIN, BODYM */
+    double tidal4;                    /* TIDAL(4) when KZ(14) <= 2, else 0 */
+    double rdens[3];                  /* RDENS */
+    int n, ifirst;                    /* N (single bodies), IFIRST */
+    int nnbopt, nnbmax;               /* NNBOPT, NNBMAX */
+    int nb_flag, m_flag;              /* from KZ(39) (regcor_gpu.F:20-21) */
+    int kz38;                         /* KZ(38): 0 none, 1 always, 2 if DF2 > 1e-4 FR2, 3 losses */
+    int pad;
+} nb6_regcor_params;
+
+/* REGCOR_GPU for the ni particles d_addr[ni] (1-based I) of one block:
+ *   d_force  double[7][force_stride] = the block's nb6_regf_device output (FREG, FDR);
+ *   d_nlist  int[ni][lmax] = the block's nb6_regf_device rows (count, 0-based
+ *            positions in the sent j-set of particles IFIRST.., self included), turned
+ *            in place into the final new list (count, 1-based ascending, no self) as
+ *            util_gpu.F:101-110 and regcor_gpu.F:298-350 leave it; lmax must equal
+ *            s->lmax and exceed nnbmax + 1;
+ *   d_counters int[5] accumulated: NBVOID, NBFLUX, NBSMIN, NBCORR, unsupported
+ *            (particles skipped: a negative or too long row, an index outside
+ *            [1, nmax], I > N without h; FPCORR terms left out for KS components
+ *            J < IFIRST).
+ * Not included: external tides beyond TIDAL(4), FCLOUD, CHECKL (KZ(37)).  Particles
+ * of one block must be distinct.  Asynchronous on `stream`. */
+int nb6_regcor_device(const nb6_regcor_params *p, const nb6_regstate *s, int ni,
+                      const int *d_addr, const double *d_force, int force_stride,
+                      int *d_nlist, int *d_counters, void *stream);
+
 /* Device allocation / copies for callers without another GPU runtime
  * (tests, benchmarks). */
 void *nb6_dev_alloc(unsigned long long bytes);

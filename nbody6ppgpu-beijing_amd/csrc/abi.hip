@@ -630,6 +630,14 @@ int nb6_irr_firr_device(double ti, int ni, const int *d_addr, double *d_out, voi
     return irr_firr_device(ti, ni, d_addr, d_out, pick(stream));
 }
 
+int nb6_regcor_device(const nb6_regcor_params *p, const nb6_regstate *s, int ni,
+                      const int *d_addr, const double *d_force, int force_stride,
+                      int *d_nlist, int *d_counters, void *stream) {
+    if (!device().ready) return E_NODEV;
+    return regcor_device(p, s, ni, d_addr, d_force, force_stride, d_nlist, d_counters,
+                         pick(stream));
+}
+
 int nb6_rsnet_nparam(void) { return rsnet_nparam(); }
 int nb6_rsnet_open(unsigned long long seed) { return rsnet_open(seed); }
 int nb6_rsnet_close(void) { return rsnet_close(); }

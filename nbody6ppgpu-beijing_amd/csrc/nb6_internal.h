@@ -180,6 +180,15 @@ int irr_set_lists_device(int ni, const int *d_addr, const int *d_list, int lmax_
                          hipStream_t s);
 int irr_sync_jp(int n, hipStream_t s);
 int irr_firr_device(double ti, int ni, const int *d_addr, double *d_out, hipStream_t s);
+
+// ---------------------------------------------------------------- regular corrector
+}  // namespace nb6
+struct nb6_regcor_params;
+struct nb6_regstate;
+namespace nb6 {
+int regcor_device(const ::nb6_regcor_params *p, const ::nb6_regstate *s, int ni,
+                  const int *d_addr, const double *d_force, int force_stride, int *d_nlist,
+                  int *d_counters, hipStream_t stream);
 int irr_config(int lanes_per_i, int sort_i);
 // the j-set of the open gpunb library (nullptr when closed)
 const JSet *gpunb_jset();

@@ -93,8 +93,88 @@ def load():
     L.nb6_rs_rule_device.argtypes = [i, i, _vp, _vp, _vp, _vp, _vp, _vp, i, _vp, _vp, i, _vp, i, i,
                                      _vp, _vp]
     L.nb6_rs_kth_device.argtypes = [i, i, _vp, _vp, i, i, _vp, _vp]
+    L.nb6_regcor_device.argtypes = [ctypes.POINTER(RegcorParams), ctypes.POINTER(RegState), i,
+                                    _vp, _vp, i, _vp, _vp, _vp]
     _L = L
     return L
+
+
+# --------------------------------------------------------------- regular corrector
+# Field order of nb6_regstate / nb6_regcor_params (include/nb6gpu.h).
+REG3 = ("x", "xdot", "x0", "x0dot", "f", "fdot", "fi", "fidot", "fr", "frdot", "d2", "d3",
+        "xn", "xndot", "d0", "d0r", "d1r", "d2r", "d3r")      # [3][nmax]
+REG1 = ("body", "t0", "step", "timenw", "t0r", "stepr", "rs")  # [nmax]
+_STATE_PTRS = ("x", "xdot", "body", "x0", "x0dot", "f", "fdot", "t0", "h", "step", "timenw",
+               "fi", "fidot", "fr", "frdot", "d2", "d3", "t0r", "stepr", "rs", "list", "xn",
+               "xndot", "d0", "d0r", "d1r", "d2r", "d3r")
+_PARAM_D = ("time", "dtmin", "smin", "smax", "etar", "rscale", "rc2", "rsfac", "znbmin",
+            "bodym", "tidal4")
+_PARAM_I = ("n", "ifirst", "nnbopt", "nnbmax", "nb_flag", "m_flag", "kz38", "pad")
+
+
+class RegState(ctypes.Structure):
+    _fields_ = ([("nmax", ctypes.c_int), ("lmax", ctypes.c_int)]
+                + [(k, ctypes.c_void_p) for k in _STATE_PTRS])
+
+
+class RegcorParams(ctypes.Structure):
+    _fields_ = ([(k, ctypes.c_double) for k in _PARAM_D] + [("rdens", ctypes.c_double * 3)]
+                + [(k, ctypes.c_int) for k in _PARAM_I])
+
+
+def regcor_params(P: dict) -> RegcorParams:
+    p = RegcorParams()
+    for k in _PARAM_D:
+        setattr(p, k, float(P[k]))
+    for q in range(3):
+        p.rdens[q] = float(P["rdens"][q])
+    for k in _PARAM_I:
+        setattr(p, k, int(P.get(k, 0)))
+    return p
+
+
+class Regcor:
+    """Device copy of the per-particle state of the regular corrector and the call
+    nb6_regcor_device (REGCOR_GPU over one block, regcor_gpu.F:1-740).  S holds numpy
+    arrays named as in nb6_regstate: [3][nmax] for REG3, [nmax] for REG1, list
+    int32 [nmax][lmax], h [npairs] (optional)."""
+
+    def __init__(self, S: dict):
+        self.L = load()
+        self.nmax = int(len(S["body"]))
+        self.lmax = int(S["list"].shape[1])
+        self.dev = {}
+        for k in REG3:
+            self.dev[k] = DeviceArray.from_host(c64(S[k]).reshape(3, self.nmax))
+        for k in REG1:
+            self.dev[k] = DeviceArray.from_host(c64(S[k]).reshape(self.nmax))
+        self.dev["list"] = DeviceArray.from_host(np.ascontiguousarray(S["list"], np.int32))
+        h = S.get("h")
+        self.dev["h"] = DeviceArray.from_host(c64(h)) if h is not None and len(h) else None
+        self.st = RegState(nmax=self.nmax, lmax=self.lmax)
+        for k in _STATE_PTRS:
+            setattr(self.st, k, self.dev[k].ptr if self.dev.get(k) is not None else None)
+
+    def correct(self, P: dict, ni: int, d_addr, d_force, force_stride: int, d_nlist,
+                d_counters, stream=None):
+        p = regcor_params(P)
+        rc = self.L.nb6_regcor_device(ctypes.byref(p), ctypes.byref(self.st), int(ni),
+                                      d_addr.ptr, d_force.ptr, int(force_stride), d_nlist.ptr,
+                                      d_counters.ptr, stream)
+        if rc:
+            check_error()
+            raise RuntimeError(f"nb6_regcor_device returned {rc}")
+
+    def download(self) -> dict:
+        out = {k: self.dev[k].to_host(np.float64, (3, self.nmax)) for k in REG3}
+        out.update({k: self.dev[k].to_host(np.float64, (self.nmax,)) for k in REG1})
+        out["list"] = self.dev["list"].to_host(np.int32, (self.nmax, self.lmax))
+        return out
+
+    def free(self):
+        for v in self.dev.values():
+            if v is not None:
+                v.free()
 
 
 def header_symbols() -> list[str]:

@@ -58,3 +58,126 @@ def rows_as_sets(lst):
         c = int(row[0])
         out.append(tuple(row[1:1 + c]) if c >= 0 else ("ovf", -c))
     return out
+
+
+# ------------------------------------------------------------------ regular corrector
+def _nb_lists(x, v, rs2, dtr, idx, exclude_self):
+    """FP64 velocity-criterion neighbour sets min(r^2, |dx + dtr dv|^2) < rs2 of the
+    particles idx (0-based), ascending 0-based indices."""
+    out = []
+    for k, i in enumerate(idx):
+        dx = x - x[i]
+        dv = v - v[i]
+        r2 = (dx * dx).sum(1)
+        p = dx + dtr[k] * dv
+        rp2 = (p * p).sum(1)
+        sel = np.minimum(r2, rp2) < rs2[k]
+        if exclude_self:
+            sel[i] = False
+        out.append(np.flatnonzero(sel))
+    return out
+
+
+def pair_forces(x, v, m, i, js):
+    """FP64 force and jerk of bodies js on body i (the pair terms of regcor_gpu.F:334-353)."""
+    if len(js) == 0:
+        return np.zeros(3), np.zeros(3)
+    a = x[js] - x[i]
+    dv = v[js] - v[i]
+    r2 = (a * a).sum(1)
+    dr3i = m[js] / (r2 * np.sqrt(r2))
+    drdp = 3.0 * (a * dv).sum(1) / r2
+    f = (a * dr3i[:, None]).sum(0)
+    j = ((dv - a * drdp[:, None]) * dr3i[:, None]).sum(0)
+    return f, j
+
+
+def regcor_case(seed=7, n=800, ni=300, lmax=160, nnbopt=20, ncm=0, kz38=1, nb_flag=1,
+                m_flag=0, tidal4=0.0, frac_void=0.04, frac_empty_old=0.05,
+                frac_small_step=0.25, frac_few=0.04, frac_same=0.3):
+    """A regular block for the corrector: state S (oracle layout), parameters P,
+    1-based addresses, the regf output force [7][ni] (FP64 regular force and jerk over
+    the non-neighbours) and the regf rows [ni][lmax] (0-based, self included).
+    Particles n-ncm.. are c.m. bodies with binding energies h.  Steps are powers of
+    two commensurate with TIME = 1, so every branch of the step rule is reachable."""
+    rng = np.random.default_rng(seed)
+    m, x, v = ics.plummer(n, seed)
+    nmax = n
+    time = 1.0
+    dtmin = 2.0 ** -14
+    addr0 = np.sort(rng.choice(n, ni, replace=False))
+    kdt = rng.integers(3, 8, n)
+    dtr_all = 2.0 ** (-kdt.astype(float))
+    step = dtr_all / 2.0 ** rng.integers(0, 4, n)
+    small = rng.random(n) < frac_small_step
+    step[small] = dtmin                    # STEP < SMIN = 2 DTMIN
+    rs = ics.rs_for_counts(x, np.arange(n), np.full(n, nnbopt))
+    few = rng.random(n) < frac_few
+    rs[few] = ics.rs_for_counts(x, np.flatnonzero(few), np.full(few.sum(), 3))
+    nnbmax = min(n // 2, lmax - 50)
+    # old lists at a perturbed radius (the previous regular step)
+    rs_old = rs * rng.uniform(0.85, 1.15, n)
+    x_prev = x - dtr_all[:, None] * v                 # positions one regular step back
+    old = _nb_lists(x_prev, v, rs_old ** 2, dtr_all, np.arange(n), exclude_self=True)
+    lst = np.zeros((nmax, lmax), np.int32)
+    empty_old = rng.random(n) < frac_empty_old
+    for i in range(n):
+        js = old[i][:nnbmax] if not empty_old[i] else old[i][:0]
+        lst[i, 0] = len(js)
+        lst[i, 1:1 + len(js)] = js + 1
+    # new rows in regf form: 0-based positions, self included; void rows hold self only
+    dtr = dtr_all[addr0]
+    new = _nb_lists(x, v, rs[addr0] ** 2, dtr, addr0, exclude_self=False)
+    rows = np.zeros((ni, lmax), np.int32)
+    void = rng.random(ni) < frac_void
+    force = np.zeros((7, ni))
+    for k, i in enumerate(addr0):
+        js = np.array([i]) if void[k] else new[k][:nnbmax]
+        rows[k, 0] = len(js)
+        rows[k, 1:1 + len(js)] = js
+        non = np.setdiff1d(np.arange(n), js)
+        f, j = pair_forces(x, v, m, i, non)
+        force[0:3, k] = f
+        force[3:6, k] = j
+    # a fraction of the block keeps its list (no neighbour change)
+    same = rng.random(ni) < frac_same
+    for k, i in enumerate(addr0):
+        if same[k] and not void[k]:
+            js = rows[k, 1:1 + rows[k, 0]]
+            js = js[js != i]
+            lst[i, :] = 0
+            lst[i, 0] = len(js)
+            lst[i, 1:1 + len(js)] = js + 1
+    # state of the previous regular / irregular steps (arbitrary but smooth values)
+    fi_old = np.zeros((3, n))
+    fidot_old = np.zeros((3, n))
+    for i in range(n):
+        fi_old[:, i], fidot_old[:, i] = pair_forces(x, v, m, i, lst[i, 1:1 + lst[i, 0]] - 1)
+    S = dict(
+        x=x.T.copy(), xdot=v.T.copy(), body=m.copy(),
+        x0=(x - 0.01 * v).T.copy(), x0dot=(v * 0.999).T.copy(),
+        f=0.5 * rng.normal(size=(3, n)), fdot=rng.normal(size=(3, n)) / 6.0,
+        t0=time - step, h=-rng.uniform(0.5, 5.0, max(ncm, 1)),
+        step=step, timenw=time + step, fi=fi_old, fidot=fidot_old,
+        fr=rng.normal(size=(3, n)) * 0.3, frdot=rng.normal(size=(3, n)) * 0.3,
+        d2=rng.normal(size=(3, n)), d3=rng.normal(size=(3, n)),
+        t0r=time - dtr_all, stepr=dtr_all.copy(), rs=rs.copy(), list=lst,
+        xn=np.zeros((3, n)), xndot=np.zeros((3, n)), d0=np.zeros((3, n)),
+        d0r=np.zeros((3, n)), d1r=np.zeros((3, n)), d2r=np.zeros((3, n)),
+        d3r=np.zeros((3, n)))
+    # regular force history of the block: from smooth (the force one step back along
+    # its derivative) to unrelated, so that the new step can grow, stay or shrink
+    eps = 10.0 ** rng.uniform(-6, 0, ni)
+    for k, i in enumerate(addr0):
+        fr = force[0:3, k] - dtr[k] * force[3:6, k]
+        S["fr"][:, i] = fr + eps[k] * np.abs(fr).max() * rng.normal(size=3)
+        S["frdot"][:, i] = force[3:6, k] * (1.0 + eps[k] * rng.normal(size=3))
+    P = dict(time=time, dtmin=dtmin, smin=2.0 * dtmin, smax=0.125, etar=0.02, rscale=0.8,
+             rc2=0.05, rsfac=12.0, znbmin=0.2 * nnbopt, bodym=1.0 / n, tidal4=tidal4,
+             rdens=[0.01, -0.02, 0.005], n=n - ncm, ifirst=1, nnbopt=nnbopt, nnbmax=nnbmax,
+             nb_flag=nb_flag, m_flag=m_flag, kz38=kz38)
+    return S, P, addr0 + 1, force, rows
+
+
+def copy_state(S):
+    return {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in S.items()}
