@@ -471,6 +471,154 @@ def bench_irr(steps, warmup, no_cpu):
             "check_vs_fp64": check}
 
 
+def bench_regcor(steps, warmup, no_cpu):
+    """Regular corrector (nb6_regcor_device, REGCOR_GPU + FPCORR) of the ni = 131072
+    block of the N = 1M regf workload: new rows and force from the device regf at RS,
+    old lists from a device regf at radii and steps of the previous block; KZ(38) = 1.
+    Each step restores the mutated state by device copies outside the timed kernels."""
+    m, x, v, idx, h2, dtr = make_workload()
+    L = lib.load()
+    n, ni = m.size, idx.size
+    rng = np.random.Generator(np.random.PCG64(SEED + 90))
+    time_ = 1.0
+    dtr_q = 2.0 ** np.round(np.log2(dtr))
+    rs = np.sqrt(h2)
+    counts0 = ics.lognormal_counts(ni, SEED + 4, median=80.0)
+    fac = np.minimum(rng.uniform(0.85, 1.15, ni), (400.0 / counts0) ** (1.0 / 3.0))
+    g = lib.Gpunb(n)
+    g.send(m, x, v)
+
+    def regf(rad, dt):
+        ib = np.ascontiguousarray(np.concatenate([x[idx].T, v[idx].T, (rad * rad)[None], dt[None]]))
+        di = lib.DeviceArray.from_host(ib)
+        do = lib.DeviceArray(7 * ni * 8)
+        dl = lib.DeviceArray(ni * LMAX * 4)
+        if L.nb6_regf_device(ni, ni, di.ptr, LMAX, NNBMAX, 0, do.ptr, dl.ptr, None) != 0:
+            lib.check_error()
+        di.free()
+        return do, dl
+
+    do_old, dl_old = regf(rs * fac, dtr_q * 2.0 ** rng.integers(-1, 2, ni))
+    old_rows = dl_old.to_host(np.int32, (ni, LMAX))
+    do_old.free()
+    dl_old.free()
+    d_force, d_rows0 = regf(rs, dtr_q)
+    rows = d_rows0.to_host(np.int32, (ni, LMAX))
+    g.close()
+    lst = np.zeros((n, LMAX), np.int32)
+    nnb0 = np.zeros(ni, np.int64)
+    for k, i in enumerate(idx):
+        c = old_rows[k, 0]
+        js = old_rows[k, 1:1 + max(c, 0)]
+        js = js[js != i] + 1
+        lst[i, 0] = len(js)
+        lst[i, 1:1 + len(js)] = js
+        nnb0[k] = len(js)
+    dtr_all = np.full(n, 2.0 ** -6)
+    dtr_all[idx] = dtr_q
+    step = dtr_all / 2.0 ** rng.integers(0, 4, n)
+    step[rng.random(n) < 0.05] = 2.0 ** -14
+    z3 = np.zeros((3, n))
+    S = dict(x=x.T.copy(), xdot=v.T.copy(), body=m, x0=(x - 1e-3 * v).T.copy(), x0dot=v.T.copy(),
+             f=rng.normal(0, 0.1, (3, n)), fdot=rng.normal(0, 0.1, (3, n)), t0=time_ - step,
+             h=None, step=step, timenw=time_ + step, fi=rng.normal(0, 0.1, (3, n)),
+             fidot=rng.normal(0, 0.1, (3, n)), fr=rng.normal(0, 0.1, (3, n)),
+             frdot=rng.normal(0, 0.1, (3, n)), d2=z3.copy(), d3=z3.copy(), t0r=time_ - dtr_all,
+             stepr=dtr_all.copy(), rs=np.full(n, 0.01), list=lst, xn=z3.copy(), xndot=z3.copy(),
+             d0=z3.copy(), d0r=z3.copy(), d1r=z3.copy(), d2r=z3.copy(), d3r=z3.copy())
+    S["rs"][idx] = rs
+    P = dict(time=time_, dtmin=2.0 ** -14, smin=2.0 ** -13, smax=0.125, etar=0.02, rscale=0.5,
+             rc2=0.01, rsfac=12.0, znbmin=16.0, bodym=1.0 / n, tidal4=0.0, rdens=[0.0, 0.0, 0.0],
+             n=n, ifirst=1, nnbopt=80, nnbmax=NNBMAX, nb_flag=1, m_flag=0, kz38=1)
+    rc = lib.Regcor(S)
+    del lst
+    # pristine copies of everything the corrector reads and writes
+    keep = ("fi", "fidot", "fr", "frdot", "d2", "d3", "t0r", "stepr", "rs", "step", "timenw",
+            "list")
+    saved = {}
+    for k in keep:
+        d = rc.dev[k]
+        saved[k] = lib.DeviceArray(d.nbytes)
+        L.nb6_device_copy(saved[k].ptr, d.ptr, d.nbytes, None)
+    d_rows = lib.DeviceArray(d_rows0.nbytes)
+    d_addr = lib.DeviceArray.from_host((idx + 1).astype(np.int32))
+    d_cnt = lib.DeviceArray(5 * 4)
+
+    def restore():
+        for k in keep:
+            L.nb6_device_copy(rc.dev[k].ptr, saved[k].ptr, saved[k].nbytes, None)
+        L.nb6_device_copy(d_rows.ptr, d_rows0.ptr, d_rows0.nbytes, None)
+        L.nb6_memcpy_htod(d_cnt.ptr, np.zeros(5, np.int32).ctypes.data, 20)
+
+    for _ in range(warmup):
+        restore()
+        rc.correct(P, ni, d_addr, d_force, ni, d_rows, d_cnt)
+    L.nb6_sync()
+    kms = []
+    for _ in range(steps):
+        restore()
+        rc.correct(P, ni, d_addr, d_force, ni, d_rows, d_cnt)
+        kms.append(L.nb6_kernel_ms(b"regcor"))
+    L.nb6_sync()
+    cnt = d_cnt.to_host(np.int32, (5,))
+    kt = float(np.mean(kms)) * 1e-3
+    out = rc.download()
+    rows_g = d_rows.to_host(np.int32, (ni, LMAX))
+    # algorithmic bytes of one call: per particle 276 B read (address, FREG/FDR, X/XDOT,
+    # X0/X0DOT, FI/FIDOT/FR/FRDOT, T0R/STEPR/RS/STEP) + 288 B written (XN/XNDOT,
+    # FI/FIDOT/FR/FRDOT, D0/D0R/D1R/D2R/D3R, T0R/STEPR/RS); list rows: regf row read,
+    # converted row written, old list read, new list written where it changed; per
+    # changed neighbour 136 B (X/XDOT/BODY, and T0/F/FDOT/X0DOT for FPCORR) + 8 B STEP;
+    # per FPCORR call 144 B (F/FDOT of I, D2/D3 read and written)
+    c = np.maximum(rows[:, 0], 0).astype(np.int64)
+    nnb = rows_g[:, 0].astype(np.int64)
+    changed = np.array([rows_g[k, 0] != nnb0[k] or not np.array_equal(
+        rows_g[k, 1:1 + rows_g[k, 0]], S["list"][idx[k], 1:1 + nnb0[k]]) for k in range(ni)])
+    alg = (ni * (276 + 288) + 4 * ((1 + c).sum() + (1 + nnb).sum() + (1 + nnb0).sum()
+                                    + (1 + nnb[changed]).sum())
+           + (int(cnt[1]) + int(cnt[2])) * 144 + int(cnt[3]) * 144)
+    check = None
+    cpu = None
+    if not no_cpu:
+        import regcor as RC
+        sub = np.sort(rng.choice(ni, 256, replace=False))
+        So = {k: (np.array(v_) if isinstance(v_, np.ndarray) else v_) for k, v_ in S.items()}
+        RC.regcor_block(So, P, idx[sub] + 1, rc_force_host(d_force, ni)[:, sub], rows[sub].copy())
+        ia = idx[sub]
+        same = all(np.array_equal(out[k][..., ia], So[k][..., ia]) for k in lib.REG3 + lib.REG1)
+        same = same and np.array_equal(out["list"][ia], So["list"][ia])
+        check = {"sample_i": int(sub.size), "bit_identical": bool(same), "counters": cnt.tolist()}
+        # the oracle (pure Python, one core) on a bounded sample of the same block
+        k_cpu = 16384
+        sub = np.sort(rng.choice(ni, k_cpu, replace=False))
+        So = {k: (np.array(v_) if isinstance(v_, np.ndarray) else v_) for k, v_ in S.items()}
+        t0 = time.perf_counter()
+        RC.regcor_block(So, P, idx[sub] + 1, rc_force_host(d_force, ni)[:, sub], rows[sub].copy())
+        tc = time.perf_counter() - t0
+        cpu = {"value": k_cpu / tc, "unit": "particles/s", "cores": 1, "kind": "port",
+               "sample": f"oracle/regcor.py (Python restatement of regcor_gpu.F + fpcorr.F), "
+                         f"{k_cpu} of the {ni} particles of the block, {tc:.1f} s"}
+    for d in list(saved.values()) + [d_rows, d_rows0, d_addr, d_cnt, d_force]:
+        d.free()
+    rc.free()
+    return {"metric": "regular corrector: particles/s (King W0=6 N=1M, block ni=131072)",
+            "value": ni / kt, "unit": "particles/s", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": kt * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp64 (REAL*4 where the reference is)",
+            "data": "synthetic (seeded King W0=6, 50% binaries, Kroupa masses)",
+            "config": {"workload": "regcor", "n": n, "n_i": ni, "lmax": LMAX, "kz38": 1,
+                       "mean_nnb": float(nnb.mean()), "changed_fraction": float(changed.mean())},
+            "roofline": {"bound": "hbm", "achieved": round(alg / kt / 1e9, 1), "peak": 8000.0,
+                         "unit": "GB/s", "frac": round(alg / kt / 8e12, 4),
+                         "algorithmic_bytes": int(alg), "kernel": "regcor_kernel",
+                         "avg_launch_ms": kt * 1e3},
+            "cpu_baseline": cpu, "check_vs_oracle": check}
+
+
+def rc_force_host(d_force, ni):
+    return d_force.to_host(np.float64, (7, ni))
+
+
 def _rs_accuracy(s, k=128):
     """Regular force of k particles with the policy's current RS against FP64 (checker)."""
     import oracle as O
@@ -494,7 +642,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--workload", default="regf",
-                    choices=["regf", "ensemble", "learned-rs", "precision", "irr"],
+                    choices=["regf", "ensemble", "learned-rs", "precision", "irr", "regcor"],
                     help="regf: the headline N=1M step; ensemble: 256-cluster batch vs loop; "
                          "learned-rs: overflow retries of the RS network vs the reference rule")
     ap.add_argument("--rs-train-samples", type=int, default=10_000_000)
@@ -521,6 +669,9 @@ def main():
         return 0
     if args.workload == "irr":
         print(json.dumps(bench_irr(args.steps, args.warmup, args.no_cpu)), flush=True)
+        return
+    if args.workload == "regcor":
+        print(json.dumps(bench_regcor(args.steps, args.warmup, args.no_cpu)), flush=True)
         return 0
     if args.workload == "precision":
         print(json.dumps(bench_precision(args.no_cpu)), flush=True)

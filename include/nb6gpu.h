@@ -302,28 +302,29 @@ int nb6_rs_kth_device(int ni, int ib_stride, const double *d_ib, const int *d_li
  * util_gpu.F:150-175 that calls REGCOR_GPU (regcor_gpu.F:1-740, with FPCORR,
  * fpcorr.F:1-148) once per particle of a regular block, on per-particle state
  * held in device memory.  Every array is a device pointer indexed by particle
- * number - 1; [3][nmax] arrays are component-major (row k = component k + 1).
+ * number - 1; [nmax][3] arrays are laid out as the Fortran X(3,NMAX) (the three
+ * components of a particle adjacent), so COMMON arrays can be copied as they are.
  * ------------------------------------------------------------------------- */
 typedef struct nb6_regstate {
     int nmax;                         /* stride of every per-particle array */
     int lmax;                         /* row length of list (LMAX) */
     /* read */
-    const double *x, *xdot;           /* [3][nmax] X, XDOT predicted to the block time */
+    const double *x, *xdot;           /* [nmax][3] X, XDOT predicted to the block time */
     const double *body;               /* [nmax] BODY */
-    const double *x0, *x0dot;         /* [3][nmax] X0, X0DOT */
-    const double *f, *fdot;           /* [3][nmax] F = a/2, FDOT = adot/6 (FPCORR) */
+    const double *x0, *x0dot;         /* [nmax][3] X0, X0DOT */
+    const double *f, *fdot;           /* [nmax][3] F = a/2, FDOT = adot/6 (FPCORR) */
     const double *t0;                 /* [nmax] T0 */
     const double *h;                  /* [nmax - N] H of KS pairs (c.m. bodies I > N); may
                                          be null when no active particle exceeds N */
     /* read and written */
     double *step, *timenw;            /* [nmax] STEP, TIMENW */
-    double *fi, *fidot, *fr, *frdot;  /* [3][nmax] FI, FIDOT, FR, FRDOT */
-    double *d2, *d3;                  /* [3][nmax] D2, D3 (FPCORR) */
+    double *fi, *fidot, *fr, *frdot;  /* [nmax][3] FI, FIDOT, FR, FRDOT */
+    double *d2, *d3;                  /* [nmax][3] D2, D3 (FPCORR) */
     double *t0r, *stepr, *rs;         /* [nmax] T0R, STEPR, RS */
     int *list;                        /* int[nmax][lmax] LIST: count, then 1-based ascending */
     /* written */
-    double *xn, *xndot;               /* [3][nmax] XN, XNDOT */
-    double *d0, *d0r, *d1r, *d2r, *d3r;  /* [3][nmax] D0, D0R, D1R, D2R, D3R */
+    double *xn, *xndot;               /* [nmax][3] XN, XNDOT */
+    double *d0, *d0r, *d1r, *d2r, *d3r;  /* [nmax][3] D0, D0R, D1R, D2R, D3R */
 } nb6_regstate;
 
 typedef struct nb6_regcor_params {

@@ -2,14 +2,17 @@
 // FPCORR (fpcorr.F:1-148), called by the reference once per particle after each
 // regular force evaluation (util_gpu.F:150-175).
 //
-// One thread per active particle I.  Its work is O(old + new list length): the regf
+// One wavefront per active particle I.  Its work is O(old + new list length): the regf
 // row is turned into the 1-based list without self (util_gpu.F:101-110), RS is
 // updated (regcor_gpu.F:95-238), the old and new lists are walked in ascending
 // order to find losses and gains (regcor_gpu.F:252-296), lost neighbours with
 // STEP <= SMIN inside 2 RS are kept (regcor_gpu.F:298-350), the force of each lost
 // (gained) neighbour moves between the irregular and regular parts
 // (regcor_gpu.F:355-415), then the Hermite corrector (:417-530), FPCORR, the list
-// copy and the new regular step (:600-740).
+// copy and the new regular step (:600-740).  Lists are read coalesced into LDS, losses
+// and gains are found by binary search there, and the moved force terms are formed 64
+// at a time by the lanes and summed in list order by twelve lanes, so every sum has the
+// reference's order and rounding.
 //
 // Arithmetic follows the reference operation by operation: FP64 without FMA
 // contraction in Fortran's left-to-right order, and the REAL*4 sub-expressions of the
@@ -19,7 +22,7 @@
 #include "nb6_internal.h"
 #include "../../include/nb6gpu.h"
 
-#include <climits>
+#include <algorithm>
 #include <cmath>
 
 namespace nb6 {
@@ -35,12 +38,11 @@ enum { C_VOID, C_FLUX, C_SMIN, C_CORR, C_UNS, NCOUNT };
 __device__ __forceinline__ void pair_fj(const nb6_regstate &s, int j, const double *xi,
                                         const double *vi, double *fa, double *fj) {
 #pragma clang fp contract(off)
-    const size_t M = s.nmax;
     double a[3], dv[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        a[k] = s.x[k * M + j] - xi[k];
-        dv[k] = s.xdot[k * M + j] - vi[k];
+        a[k] = s.x[3 * (size_t)j + k] - xi[k];
+        dv[k] = s.xdot[3 * (size_t)j + k] - vi[k];
     }
     const double rij2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
     const double dr2i = 1.0 / rij2;
@@ -59,17 +61,16 @@ __device__ __forceinline__ void fp_terms(const nb6_regstate &s, double time, int
                                          const double *xi, const double *vi, double *f2,
                                          double *f3) {
 #pragma clang fp contract(off)
-    const size_t M = s.nmax;
     const double sj = time - s.t0[j];
     const double s3 = 3.0 * sj;
     double A[12];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        const double fj = s.f[k * M + j], fdj = s.fdot[k * M + j];
-        A[k] = s.x[k * M + j] - xi[k];
-        A[k + 3] = (fdj * s3 + 2.0 * fj) * sj + s.x0dot[k * M + j] - vi[k];
-        A[k + 6] = 2.0 * (fdj * s3 + fj - s.f[k * M + i]);
-        A[k + 9] = 6.0 * (fdj - s.fdot[k * M + i]);
+        const double fj = s.f[3 * (size_t)j + k], fdj = s.fdot[3 * (size_t)j + k];
+        A[k] = s.x[3 * (size_t)j + k] - xi[k];
+        A[k + 3] = (fdj * s3 + 2.0 * fj) * sj + s.x0dot[3 * (size_t)j + k] - vi[k];
+        A[k + 6] = 2.0 * (fdj * s3 + fj - s.f[3 * (size_t)i + k]);
+        A[k + 9] = 6.0 * (fdj - s.fdot[3 * (size_t)i + k]);
     }
     const double a13 = 1.0 / (A[0] * A[0] + A[1] * A[1] + A[2] * A[2]);
     const double a14 = s.body[j] * a13 * sqrt(a13);
@@ -93,124 +94,193 @@ __device__ __forceinline__ void fp_terms(const nb6_regstate &s, double time, int
     }
 }
 
-__device__ __forceinline__ int at_or_max(const int *r, int l, int n) { return l <= n ? r[l] : INT_MAX; }
+// ------------------------------------------------------------------ one wavefront per particle
+constexpr int WAVE = 64;
+constexpr int NTERM = 12;   // moved force terms: DFIRR[3], DFD[3], F2DOT[3], F3DOT[3]
 
-__device__ void regcor_one(const nb6_regcor_params &P, const nb6_regstate &s, int I,
-                           const double *fg, int fstride, int *row, int *cnt,
-                           unsigned char *shrink) {
+// LDS of one wavefront: one chunk of terms (double [NTERM][64]), the new and old lists
+// (int [lmax + 1] each) and two bits per old-list entry: lost, and candidate for the
+// SMIN retention
+__host__ __device__ constexpr int loss_words(int lmax) { return 2 * ((lmax + 64) / 64); }
+__host__ __device__ constexpr size_t wave_lds_bytes(int lmax) {
+    return ((size_t)NTERM * WAVE * 8 + (size_t)2 * (lmax + 1) * 4 +
+            (size_t)2 * loss_words(lmax) * 4 + 15) & ~(size_t)15;
+}
+
+__device__ __forceinline__ bool lds_contains(const int *a, int n, int v) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo < n && a[lo] == v;
+}
+
+// number of set bits of m in the lanes below this one
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+__device__ __forceinline__ double pick3(const double *a, int q) {
+    return q == 0 ? a[0] : q == 1 ? a[1] : a[2];
+}
+
+// NNB = 0: force of a mass BODYM/100 at the origin, regcor_gpu.F:108-116
+__device__ __forceinline__ void void_terms(const nb6_regcor_params &P, const double *xi,
+                                           const double *vi, double *dfirr, double *dfd) {
 #pragma clang fp contract(off)
-    const size_t M = s.nmax;
+    const double r2 = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
+    const double fij = (double)0.01f * P.bodym / (r2 * sqrt(r2));
+    const double rdot = 3.0 * (xi[0] * vi[0] + xi[1] * vi[1] + xi[2] * vi[2]) / r2;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        dfirr[k] = -(fij * xi[k]);
+        dfd[k] = -((vi[k] - rdot * xi[k]) * fij);
+    }
+}
+
+// RS after the step, regcor_gpu.F:95-238.  nl: the new list before SMIN retention
+// (1-based, ascending; read for NNB <= 3 only).
+__device__ double rs_update(const nb6_regcor_params &P, const nb6_regstate &s, int I, int i,
+                            const double *xi, const double *vi, int nnb, int nnb0, double dtr,
+                            double ri2, double rh2, double rs, const int *nl) {
+#pragma clang fp contract(off)
+    const double rscale = P.rscale;
+    if (nnb == 0) {
+        const double r2 = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
+        const double rdot = 3.0 * (xi[0] * vi[0] + xi[1] * vi[1] + xi[2] * vi[2]) / r2;
+        return rdot > 0.0 ? fmax(0.75 * rs, (double)0.1f * rscale) : (double)1.1f * rs;
+    }
+    const float fnb = (float)nnb;
+    const double fac = (double)(1.0f + (0.1f * ((float)P.nnbopt - fnb)) / fnb);
+    double a3 = (double)fnb * fac;
+    const int nbave = min(P.nnbopt + 150, max(P.nnbmax - 50, (int)P.znbmin));
+    a3 = fmin(a3, (double)(0.9f * (float)nbave));
+    const int nbp = (int)a3;
+    if (ri2 > rh2 && P.nb_flag == 1)
+        a3 = a3 * fmax(rscale / sqrt(ri2), (double)0.1f);
+    else
+        a3 = fmax(a3, P.znbmin);
+    double a4 = a3 / (double)fnb;
+    if ((a3 - (double)nnb0) * (a3 - (double)nnb) < 0.0) a4 = sqrt(a4);
+    const double ridot = (xi[0] - P.rdens[0]) * vi[0] + (xi[1] - P.rdens[1]) * vi[1] +
+                         (xi[2] - P.rdens[2]) * vi[2];
+    if (ri2 > P.rc2 && P.nb_flag == 1 && ri2 < 100.0 * rh2) a4 = a4 * (1.0 + ridot * dtr / ri2);
+    if (I > P.n && ri2 <= rh2) {                      // c.m. body, regcor_gpu.F:169-181
+        const double a2 = 100.0 * s.body[i] / fabs(s.h[I - P.n - 1]);
+        if (a2 > rs) {
+            a3 = fmax((double)(1.0f - fnb / (float)nbave), 0.0);
+            a4 = 1.0 + 0.5 * a3;
+        }
+    }
+    const double a5 = fmin(P.rsfac * dtr, 0.25);
+    a4 = a4 - 1.0;
+    a4 = a4 > a5 ? a5 : fmax(a4, -a5);
+    a3 = ONE3 * a4;
+    double a1 = 1.0 + a3 - a3 * a3;
+    if (nnb < P.znbmin && nbp > P.znbmin && a1 < 1.0) a1 = (double)1.05f;
+    if (fabs(a1 - 1.0) > (double)0.003f) {
+        if (rs * rs > ri2 && rs * rs < 2.0 * ri2 && nnb > 5) a1 = sqrt(a1);
+        rs = a1 * rs;
+    }
+    if (nnb > nbave && ridot < 0.0) {
+        const double drs = (double)((fnb - (float)nbave) / (3.0f * fnb));
+        rs = rs * (1.0 - drs);
+    }
+    if (P.m_flag == 0 && nnb <= 3 && ri2 < 100.0 * rh2) {   // regcor_gpu.F:222-238
+        a1 = 2.0 * rs;
+        for (int l = 0; l < nnb; l++) {
+            const int j = nl[l] - 1;
+            const double d0 = xi[0] - s.x[3 * (size_t)j], d1 = xi[1] - s.x[3 * (size_t)j + 1], d2 = xi[2] - s.x[3 * (size_t)j + 2];
+            const double rij = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+            const double rsdot = (d0 * (vi[0] - s.xdot[3 * (size_t)j]) + d1 * (vi[1] - s.xdot[3 * (size_t)j + 1]) +
+                                  d2 * (vi[2] - s.xdot[3 * (size_t)j + 2])) / rij;
+            a1 = fmin(a1, rij + rsdot * dtr);
+        }
+        rs = fmax(a1, (double)1.1f * rs);
+    }
+    return rs;
+}
+
+// Hand-over from the list kernel to the finishing kernel, SoA over the block:
+// doubles DFIRR, DFD, F2DOT and F3DOT sums, FREG, FDR (after retention); ints flags
+// (1: corrected, 2: FPCORR applies), NNB before retention, NNB0 and, for NNB <= 3,
+// the new list.
+enum { H_DFIRR = 0, H_DFD = 3, H_SV2 = 6, H_SV3 = 9, H_FREG = 12, H_FDR = 15, NHD = 18 };
+enum { HI_FLAGS = 0, HI_NNB = 1, HI_NNB0 = 2, HI_J3 = 3, NHI = 6 };
+
+// Lists of particle I by one wavefront (regcor_gpu.F:252-415 and the list copy): values
+// every lane computes from the same inputs are uniform, list work is spread over the
+// lanes, lane 0 alone runs the rare SMIN retention.  hd, hi: this particle's column of
+// the hand-over (stride ni).
+__device__ void regcor_wave(const nb6_regcor_params &P, const nb6_regstate &s, int I,
+                            const double *fg, int fstride, int *row, int *cnt, double *T,
+                            int *nl, int *ol, unsigned *lb, unsigned *cb, int lane, double *hd,
+                            int *hi, int ni) {
+#pragma clang fp contract(off)
     const int lmax = s.lmax;
     const double time = P.time;
     if (I < 1 || I > s.nmax) { cnt[C_UNS]++; return; }
     const int i = I - 1;
+    if (I > P.n && !s.h) { cnt[C_UNS]++; return; }
 
-    // regf row -> 1-based list without self, util_gpu.F:101-110
+    // regf row -> 1-based list without self, util_gpu.F:101-110 (ballot compaction)
     const int c = row[0];
     if (c < 0 || c > lmax - 1) { cnt[C_UNS]++; return; }
     int nnb = 0;
-    for (int l = 1; l <= c; l++) {
-        const int t = row[l] + P.ifirst;
-        if (t < 1 || t > s.nmax) { cnt[C_UNS]++; return; }
-        if (t != I) row[++nnb] = t;
+    bool bad = false;
+    for (int b = 0; b < c; b += WAVE) {
+        const int l = b + lane;
+        int t = 0;
+        bool keep = false;
+        if (l < c) {
+            t = row[1 + l] + P.ifirst;
+            bad = bad || t < 1 || t > s.nmax;
+            keep = t != I;
+        }
+        const unsigned long long m = __ballot(keep);
+        if (keep) nl[nnb + lanes_below(m)] = t;
+        nnb += __popcll(m);
     }
-    row[0] = nnb;
-
     int *orow = s.list + (size_t)i * lmax;
     const int nnb0 = orow[0];
     if (nnb0 < 0 || nnb0 > lmax - 1) { cnt[C_UNS]++; return; }
-    for (int l = 1; l <= nnb0; l++)
-        if (orow[l] < 1 || orow[l] > s.nmax) { cnt[C_UNS]++; return; }
+    for (int b = 0; b < nnb0; b += WAVE) {
+        const int l = b + lane;
+        if (l < nnb0) {
+            const int t = orow[1 + l];
+            bad = bad || t < 1 || t > s.nmax;
+            ol[l] = t;
+        }
+    }
+    if (__ballot(bad)) { cnt[C_UNS]++; return; }
+    __builtin_amdgcn_wave_barrier();
 
     double xi[3], vi[3], freg[3], fdr[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        xi[k] = s.x[k * M + i];
-        vi[k] = s.xdot[k * M + i];
+        xi[k] = s.x[3 * (size_t)i + k];
+        vi[k] = s.xdot[3 * (size_t)i + k];
         freg[k] = fg[(size_t)k * fstride];
         fdr[k] = fg[(size_t)(3 + k) * fstride];
     }
-    const double dtr = time - s.t0r[i];
-    const double rscale = P.rscale;
-    double ri2, rh2;
-    if (P.nb_flag == 1) {
-        const double e0 = xi[0] - P.rdens[0], e1 = xi[1] - P.rdens[1], e2 = xi[2] - P.rdens[2];
-        ri2 = e0 * e0 + e1 * e1 + e2 * e2;
-        rh2 = rscale * rscale;
-    } else {
-        ri2 = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
-        rh2 = 9.0 * (rscale * rscale);
-    }
-    double rs = s.rs[i];
-    const double rs2 = rs * rs;
+    const double rs_old = s.rs[i];
+    const double rs2 = rs_old * rs_old;
     double dfirr[3] = {0.0, 0.0, 0.0}, dfd[3] = {0.0, 0.0, 0.0};
-
-    if (nnb == 0) {                                   // regcor_gpu.F:95-127
-        const double r2 = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
-        const double fij = (double)0.01f * P.bodym / (r2 * sqrt(r2));
-        const double rdot = 3.0 * (xi[0] * vi[0] + xi[1] * vi[1] + xi[2] * vi[2]) / r2;
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            dfirr[k] = -(fij * xi[k]);
-            dfd[k] = -((vi[k] - rdot * xi[k]) * fij);
-        }
-        rs = rdot > 0.0 ? fmax(0.75 * rs, (double)0.1f * rscale) : (double)1.1f * rs;
+    if (nnb == 0) {                                   // regcor_gpu.F:108-116
+        void_terms(P, xi, vi, dfirr, dfd);
         cnt[C_VOID]++;
-    } else {                                          // regcor_gpu.F:139-238
-        const float fnb = (float)nnb;
-        const double fac = (double)(1.0f + (0.1f * ((float)P.nnbopt - fnb)) / fnb);
-        double a3 = (double)fnb * fac;
-        const int nbave = min(P.nnbopt + 150, max(P.nnbmax - 50, (int)P.znbmin));
-        a3 = fmin(a3, (double)(0.9f * (float)nbave));
-        const int nbp = (int)a3;
-        if (ri2 > rh2 && P.nb_flag == 1)
-            a3 = a3 * fmax(rscale / sqrt(ri2), (double)0.1f);
-        else
-            a3 = fmax(a3, P.znbmin);
-        double a4 = a3 / (double)fnb;
-        if ((a3 - (double)nnb0) * (a3 - (double)nnb) < 0.0) a4 = sqrt(a4);
-        const double ridot = (xi[0] - P.rdens[0]) * vi[0] + (xi[1] - P.rdens[1]) * vi[1] +
-                             (xi[2] - P.rdens[2]) * vi[2];
-        if (ri2 > P.rc2 && P.nb_flag == 1 && ri2 < 100.0 * rh2) a4 = a4 * (1.0 + ridot * dtr / ri2);
-        if (I > P.n && ri2 <= rh2) {                  // c.m. body, regcor_gpu.F:169-181
-            if (!s.h) { cnt[C_UNS]++; return; }
-            const double a2 = 100.0 * s.body[i] / fabs(s.h[I - P.n - 1]);
-            if (a2 > rs) {
-                a3 = fmax((double)(1.0f - fnb / (float)nbave), 0.0);
-                a4 = 1.0 + 0.5 * a3;
-            }
-        }
-        const double a5 = fmin(P.rsfac * dtr, 0.25);
-        a4 = a4 - 1.0;
-        a4 = a4 > a5 ? a5 : fmax(a4, -a5);
-        a3 = ONE3 * a4;
-        double a1 = 1.0 + a3 - a3 * a3;
-        if (nnb < P.znbmin && nbp > P.znbmin && a1 < 1.0) a1 = (double)1.05f;
-        if (fabs(a1 - 1.0) > (double)0.003f) {
-            if (rs * rs > ri2 && rs * rs < 2.0 * ri2 && nnb > 5) a1 = sqrt(a1);
-            rs = a1 * rs;
-        }
-        if (nnb > nbave && ridot < 0.0) {
-            const double drs = (double)((fnb - (float)nbave) / (3.0f * fnb));
-            rs = rs * (1.0 - drs);
-        }
-        if (P.m_flag == 0 && nnb <= 3 && ri2 < 100.0 * rh2) {   // regcor_gpu.F:222-238
-            a1 = 2.0 * rs;
-            for (int l = 1; l <= nnb; l++) {
-                const int j = row[l] - 1;
-                const double d0 = xi[0] - s.x[j], d1 = xi[1] - s.x[M + j], d2 = xi[2] - s.x[2 * M + j];
-                const double rij = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-                const double rsdot = (d0 * (vi[0] - s.xdot[j]) + d1 * (vi[1] - s.xdot[M + j]) +
-                                      d2 * (vi[2] - s.xdot[2 * M + j])) / rij;
-                a1 = fmin(a1, rij + rsdot * dtr);
-            }
-            rs = fmax(a1, (double)1.1f * rs);
-        }
     }
-    s.rs[i] = rs;
+    const int nnb_pre = nnb;
+    if (lane < 3 && lane < nnb) hi[(HI_J3 + lane) * ni] = nl[lane];
 
-    // losses and gains, regcor_gpu.F:252-350
-    int nbloss = 0, nbgain = 0;
+    // losses (ascending), gains and JMIN, regcor_gpu.F:252-296
+    int nloss = 0, nbgain = 0;
+    bool jmin = false;
     double df2, fr2;
     if (nnb0 == 0) {
         nbgain = nnb;
@@ -221,132 +291,239 @@ __device__ void regcor_one(const nb6_regcor_params &P, const nb6_regstate &s, in
         fr2 = 0.0;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            const double e = freg[k] - s.fr[k * M + i];
+            const double e = freg[k] - s.fr[3 * (size_t)i + k];
             df2 = df2 + e * e;
             fr2 = fr2 + freg[k] * freg[k];
         }
-        bool jmin = false;
-        for (int a = 1, b = 1; a <= nnb0 || b <= nnb;) {
-            const int vo = at_or_max(orow, a, nnb0), vn = at_or_max(row, b, nnb);
-            if (vo < vn) {
-                nbloss++;
-                if (s.step[vo - 1] < P.smin) jmin = true;
-                a++;
-            } else if (vn < vo) {
-                nbgain++;
-                b++;
-            } else {
-                a++;
-                b++;
+        // a lost J with STEP(J) < SMIN enables the retention (JMIN); a candidate has
+        // STEP(J) <= SMIN, IFIRST <= J <= N and |x_J - x_I| <= 2 RS (regcor_gpu.F:305-313)
+        bool jm = false;
+        for (int b = 0; b < nnb0; b += WAVE) {
+            const int l = b + lane;
+            bool lost = false, cand = false;
+            if (l < nnb0) {
+                const int t = ol[l];
+                lost = !lds_contains(nl, nnb, t);
+                if (lost) {
+                    const double st = s.step[t - 1];
+                    jm = jm || st < P.smin;
+                    if (!(st > P.smin || t < P.ifirst || t > P.n)) {
+                        const size_t j3 = 3 * (size_t)(t - 1);
+                        const double e0 = xi[0] - s.x[j3], e1 = xi[1] - s.x[j3 + 1],
+                                     e2 = xi[2] - s.x[j3 + 2];
+                        cand = !(e0 * e0 + e1 * e1 + e2 * e2 > 4.0 * rs2);
+                    }
+                }
             }
+            const unsigned long long m = __ballot(lost), mc = __ballot(cand);
+            if (lane == 0) {
+                lb[b / 32] = (unsigned)m;
+                lb[b / 32 + 1] = (unsigned)(m >> 32);
+                cb[b / 32] = (unsigned)mc;
+                cb[b / 32 + 1] = (unsigned)(mc >> 32);
+            }
+            nloss += __popcll(m);
         }
-        // keep lost neighbours with small steps inside 2 RS, in ascending order, while
-        // NNB <= NNBMAX (regcor_gpu.F:298-350)
-        if (jmin) {
-            for (int a = 1, b = 1; a <= nnb0 || b <= nnb;) {
-                if (nnb > P.nnbmax || I > P.n) break;
-                const int vo = at_or_max(orow, a, nnb0), vn = at_or_max(row, b, nnb);
-                if (vo < vn) {
-                    const int j = vo - 1;
-                    bool keep = !(s.step[j] > P.smin || vo < P.ifirst || vo > P.n);
-                    if (keep) {
-                        const double e0 = xi[0] - s.x[j], e1 = xi[1] - s.x[M + j],
-                                     e2 = xi[2] - s.x[2 * M + j];
-                        keep = !(e0 * e0 + e1 * e1 + e2 * e2 > 4.0 * rs2);
-                    }
-                    if (keep) {
-                        for (int l = nnb; l >= b; l--) row[l + 1] = row[l];
-                        row[b] = vo;
-                        nnb++;
-                        nbloss--;
-                        cnt[C_SMIN]++;
-                        double fa[3], fj[3];
-                        pair_fj(s, j, xi, vi, fa, fj);
+        jmin = __ballot(jm) != 0;
+        for (int b = 0; b < nnb; b += WAVE) {
+            const int l = b + lane;
+            const bool gain = l < nnb && !lds_contains(ol, nnb0, nl[l]);
+            nbgain += __popcll(__ballot(gain));
+        }
+    }
+    int nbloss = nloss;
+    __builtin_amdgcn_wave_barrier();
+
+    // keep the candidates, ascending, while NNB <= NNBMAX (regcor_gpu.F:298-350): NNB
+    // only grows at a kept candidate, so testing the bound before each candidate is the
+    // reference's test before each lost neighbour.  All lanes run this loop with the same
+    // values; each insertion shifts the tail of the new list by one, 64 entries at a time
+    // from the top.  A kept neighbour loses its loss bit.
+    if (jmin && I <= P.n) {
+        bool full = false;
+        for (int wd = 0; wd < 2 * ((nnb0 + WAVE - 1) / WAVE) && !full; wd++) {
+            unsigned bits = lb[wd] & cb[wd];
+            while (bits) {
+                if (nnb > P.nnbmax) {
+                    full = true;
+                    break;
+                }
+                const int q = wd * 32 + __builtin_ctz(bits);
+                bits &= bits - 1;
+                const int vo = ol[q];
+                int lo = 0, hi_ = nnb;
+                while (lo < hi_) {
+                    const int mid = (lo + hi_) >> 1;
+                    if (nl[mid] < vo)
+                        lo = mid + 1;
+                    else
+                        hi_ = mid;
+                }
+                for (int c0 = nnb - 1; c0 >= lo; c0 -= WAVE) {
+                    const int idx = c0 - lane;
+                    const bool act = idx >= lo;
+                    int v = 0;
+                    if (act) v = nl[idx];
+                    __builtin_amdgcn_wave_barrier();
+                    if (act) nl[idx + 1] = v;
+                    __builtin_amdgcn_wave_barrier();
+                }
+                if (lane == 0) {
+                    nl[lo] = vo;
+                    lb[wd] &= ~(1u << (q & 31));
+                }
+                __builtin_amdgcn_wave_barrier();
+                nnb++;
+                nbloss--;
+                cnt[C_SMIN]++;
+                double fa[3], fj[3];
+                pair_fj(s, vo - 1, xi, vi, fa, fj);
 #pragma unroll
-                        for (int k = 0; k < 3; k++) {
-                            freg[k] = freg[k] - fa[k];
-                            fdr[k] = fdr[k] - fj[k];
-                        }
-                        a++;
-                        b++;
-                    } else {
-                        a++;
-                    }
-                } else if (vn < vo) {
-                    b++;
-                } else {
-                    a++;
-                    b++;
+                for (int k = 0; k < 3; k++) {
+                    freg[k] = freg[k] - fa[k];
+                    fdr[k] = fdr[k] - fj[k];
                 }
             }
         }
     }
 
     // FPCORR decision (regcor_gpu.F:600-612); KZ(38) = 3 drops the gains
-    bool fp = false;
-    if (P.kz38 == 1)
-        fp = true;
-    else if (P.kz38 == 2 && df2 > (double)0.0001f * fr2)
-        fp = true;
-    else if (P.kz38 == 3 && nbloss > 0)
-        fp = true;
+    bool fp = P.kz38 == 1 || (P.kz38 == 2 && df2 > (double)0.0001f * fr2) ||
+              (P.kz38 == 3 && nbloss > 0);
     const bool fp_gains = fp && P.kz38 != 3;
     fp = fp && (nbloss + (fp_gains ? nbgain : 0)) > 0;
 
-    // force differences of lost then gained neighbours (regcor_gpu.F:355-415) and the
-    // FPCORR sums in the same order (fpcorr.F:19-92)
-    double sv2[3] = {0.0, 0.0, 0.0}, sv3[3] = {0.0, 0.0, 0.0};
+    // force moved by lost then gained neighbours (regcor_gpu.F:355-415) and the FPCORR
+    // sums (fpcorr.F:19-92): up to 64 terms at a time, compacted in list order into LDS,
+    // then lane q < 12 adds term component q over them in that order, as the
+    // reference's serial loop does
+    double acc = lane < 3 ? pick3(dfirr, lane) : lane < 6 ? pick3(dfd, lane - 3) : 0.0;
     for (int pass = 0; pass < 2; pass++) {
-        if (pass == 0 && nbloss == 0) continue;
-        if (pass == 1 && nbgain == 0) continue;
+        if (pass == 0 ? nbloss == 0 : nbgain == 0) continue;
+        const int np = pass == 0 ? nnb0 : nnb;
         const bool fps = fp && (pass == 0 || fp_gains);
-        for (int a = 1, b = 1; a <= nnb0 || b <= nnb;) {
-            const int vo = at_or_max(orow, a, nnb0), vn = at_or_max(row, b, nnb);
+        for (int b = 0; b < np; b += WAVE) {
+            const int l = b + lane;
             int j1 = 0;
-            if (vo < vn) {
-                if (pass == 0) j1 = vo;
-                a++;
-            } else if (vn < vo) {
-                if (pass == 1) j1 = vn;
-                b++;
-            } else {
-                a++;
-                b++;
-            }
-            if (!j1) continue;
-            const int j = j1 - 1;
-            double fa[3], fj[3];
-            pair_fj(s, j, xi, vi, fa, fj);
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
+            if (l < np) {
                 if (pass == 0) {
-                    dfirr[k] = dfirr[k] - fa[k];
-                    dfd[k] = dfd[k] - fj[k];
+                    if ((lb[l >> 5] >> (l & 31)) & 1u) j1 = ol[l];
                 } else {
-                    dfirr[k] = dfirr[k] + fa[k];
-                    dfd[k] = dfd[k] + fj[k];
+                    const int t = nl[l];
+                    if (nnb0 == 0 || !lds_contains(ol, nnb0, t)) j1 = t;
                 }
             }
-            if (fps) {
-                if (j1 < P.ifirst) {
-                    cnt[C_UNS]++;
-                } else {
-                    double f2[3], f3[3];
-                    fp_terms(s, time, i, j, xi, vi, f2, f3);
+            const unsigned long long m = __ballot(j1 != 0);
+            if (!m) continue;
+            bool ks = false;
+            if (j1) {
+                const int pos = lanes_below(m);
+                double fa[3], fj[3];
+                pair_fj(s, j1 - 1, xi, vi, fa, fj);
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    T[k * WAVE + pos] = fa[k];
+                    T[(3 + k) * WAVE + pos] = fj[k];
+                }
+                if (fps) {
+                    double f2[3] = {0.0, 0.0, 0.0}, f3[3] = {0.0, 0.0, 0.0};
+                    if (j1 < P.ifirst)
+                        ks = true;         // KS component: not supported, term left at zero
+                    else
+                        fp_terms(s, time, i, j1 - 1, xi, vi, f2, f3);
 #pragma unroll
                     for (int k = 0; k < 3; k++) {
-                        if (pass == 0) {
-                            f2[k] = -f2[k];
-                            f3[k] = -f3[k];
-                        }
-                        sv2[k] = sv2[k] + f2[k];
-                        sv3[k] = sv3[k] + f3[k];
+                        T[(6 + k) * WAVE + pos] = f2[k];
+                        T[(9 + k) * WAVE + pos] = f3[k];
                     }
                 }
             }
+            cnt[C_UNS] += __popcll(__ballot(ks));
+            __builtin_amdgcn_wave_barrier();
+            const int nt = __popcll(m);
+            if (lane < (fps ? NTERM : 6)) {
+                const double *Tq = T + lane * WAVE;
+                if (pass == 0)
+                    for (int t = 0; t < nt; t++) acc = acc - Tq[t];
+                else
+                    for (int t = 0; t < nt; t++) acc = acc + Tq[t];
+            }
+            __builtin_amdgcn_wave_barrier();
         }
     }
+    double sv2[3], sv3[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        dfirr[k] = __shfl(acc, k);
+        dfd[k] = __shfl(acc, 3 + k);
+        sv2[k] = __shfl(acc, 6 + k);
+        sv3[k] = __shfl(acc, 9 + k);
+    }
     if (P.kz38 == 3 && nbloss > 0) nbgain = 0;
+    const bool copy = nbloss + nbgain > 0;
+    if (copy) cnt[C_FLUX] += nbloss + nbgain;
+    if (fp) cnt[C_CORR]++;
+
+    // final list to the regf row and, when it changed, to LIST (regcor_gpu.F:614-622)
+    for (int b = 0; b <= nnb; b += WAVE) {
+        const int l = b + lane;
+        if (l <= nnb) {
+            const int v = l == 0 ? nnb : nl[l - 1];
+            row[l] = v;
+            if (copy) orow[l] = v;
+        }
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            hd[(H_DFIRR + k) * ni] = dfirr[k];
+            hd[(H_DFD + k) * ni] = dfd[k];
+            hd[(H_SV2 + k) * ni] = sv2[k];
+            hd[(H_SV3 + k) * ni] = sv3[k];
+            hd[(H_FREG + k) * ni] = freg[k];
+            hd[(H_FDR + k) * ni] = fdr[k];
+        }
+        hi[HI_NNB * ni] = nnb_pre;
+        hi[HI_NNB0 * ni] = nnb0;
+        hi[HI_FLAGS * ni] = 1 | (fp ? 2 : 0);
+    }
+}
+
+// RS, corrector, FPCORR and step of particle I (regcor_gpu.F:95-238, :417-740) from
+// the list kernel's hand-over, one thread per particle.
+__device__ void regcor_finish(const nb6_regcor_params &P, const nb6_regstate &s, int I,
+                              const double *hd, const int *hi, int ni) {
+#pragma clang fp contract(off)
+    const double time = P.time;
+    const int i = I - 1;
+    const int flags = hi[HI_FLAGS * ni];
+    const int nnb = hi[HI_NNB * ni], nnb0 = hi[HI_NNB0 * ni];
+    const bool fp = (flags & 2) != 0;
+    double xi[3], vi[3], freg[3], fdr[3], dfirr[3], dfd[3], sv2[3], sv3[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        xi[k] = s.x[3 * (size_t)i + k];
+        vi[k] = s.xdot[3 * (size_t)i + k];
+        dfirr[k] = hd[(H_DFIRR + k) * ni];
+        dfd[k] = hd[(H_DFD + k) * ni];
+        sv2[k] = hd[(H_SV2 + k) * ni];
+        sv3[k] = hd[(H_SV3 + k) * ni];
+        freg[k] = hd[(H_FREG + k) * ni];
+        fdr[k] = hd[(H_FDR + k) * ni];
+    }
+    const double dtr = time - s.t0r[i];
+    double ri2, rh2;
+    if (P.nb_flag == 1) {
+        const double e0 = xi[0] - P.rdens[0], e1 = xi[1] - P.rdens[1], e2 = xi[2] - P.rdens[2];
+        ri2 = e0 * e0 + e1 * e1 + e2 * e2;
+        rh2 = P.rscale * P.rscale;
+    } else {
+        ri2 = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
+        rh2 = 9.0 * (P.rscale * P.rscale);
+    }
+    int j3[3] = {0, 0, 0};
+    for (int q = 0; q < 3 && q < nnb; q++) j3[q] = hi[(HI_J3 + q) * ni];
+    s.rs[i] = rs_update(P, s, I, i, xi, vi, nnb, nnb0, dtr, ri2, rh2, s.rs[i], j3);
 
     // corrector, regcor_gpu.F:417-530
     const double dtsq = dtr * dtr;
@@ -363,53 +540,45 @@ __device__ void regcor_one(const nb6_regcor_params &P, const nb6_regstate &s, in
     double xnd[3], d2r[3], d3r[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        const double dfr = s.fr[k * M + i] - freg[k] - dfirr[k];
+        const double dfr = s.fr[3 * (size_t)i + k] - freg[k] - dfirr[k];
         const double fdr0 = fdr[k] + dfd[k];
-        const double frd = s.frdot[k * M + i];
+        const double frd = s.frdot[3 * (size_t)i + k];
         const double sm = frd + fdr0;
         const double at3 = 2.0 * dfr + dtr * sm;
         const double bt2 = -3.0 * dfr - dtr * (sm + frd);
-        s.xn[k * M + i] = s.x0[k * M + i] + ((double)0.6f * at3 + bt2) * dtsq12;
-        xnd[k] = s.x0dot[k * M + i] + (0.75 * at3 + bt2) * dtr13;
-        s.xndot[k * M + i] = xnd[k];
-        const double fik = s.fi[k * M + i] + dfirr[k];
-        s.fi[k * M + i] = fik;
-        s.fr[k * M + i] = freg[k];
-        s.fidot[k * M + i] = s.fidot[k * M + i] + dfd[k];
-        s.frdot[k * M + i] = fdr[k];
-        if (k == 1) s.fidot[i] = s.fidot[i] - P.tidal4 * dfr;
-        if (k == 0) s.fidot[M + i] = s.fidot[M + i] + P.tidal4 * dfr;
-        s.d0[k * M + i] = fik;
-        s.d0r[k * M + i] = freg[k];
-        s.d1r[k * M + i] = fdr[k];
+        s.xn[3 * (size_t)i + k] = s.x0[3 * (size_t)i + k] + ((double)0.6f * at3 + bt2) * dtsq12;
+        xnd[k] = s.x0dot[3 * (size_t)i + k] + (0.75 * at3 + bt2) * dtr13;
+        s.xndot[3 * (size_t)i + k] = xnd[k];
+        const double fik = s.fi[3 * (size_t)i + k] + dfirr[k];
+        s.fi[3 * (size_t)i + k] = fik;
+        s.fr[3 * (size_t)i + k] = freg[k];
+        s.fidot[3 * (size_t)i + k] = s.fidot[3 * (size_t)i + k] + dfd[k];
+        s.frdot[3 * (size_t)i + k] = fdr[k];
+        if (k == 1) s.fidot[3 * (size_t)i] = s.fidot[3 * (size_t)i] - P.tidal4 * dfr;
+        if (k == 0) s.fidot[3 * (size_t)i + 1] = s.fidot[3 * (size_t)i + 1] + P.tidal4 * dfr;
+        s.d0[3 * (size_t)i + k] = fik;
+        s.d0r[3 * (size_t)i + k] = freg[k];
+        s.d1r[3 * (size_t)i + k] = fdr[k];
         d3r[k] = at3 * dt6;
         d2r[k] = (3.0 * at3 + bt2) * dt2;
     }
     if (fp) {                                         // fpcorr.F:93-97
-        cnt[C_CORR]++;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            s.d2[k * M + i] = s.d2[k * M + i] + sv2[k];
-            s.d3[k * M + i] = s.d3[k * M + i] + sv3[k];
+            s.d2[3 * (size_t)i + k] = s.d2[3 * (size_t)i + k] + sv2[k];
+            s.d3[3 * (size_t)i + k] = s.d3[3 * (size_t)i + k] + sv3[k];
             d2r[k] = d2r[k] - sv2[k];
             d3r[k] = d3r[k] - sv3[k];
         }
     }
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        s.d2r[k * M + i] = d2r[k];
-        s.d3r[k * M + i] = d3r[k];
-    }
-
-    // list copy, regcor_gpu.F:614-622
-    row[0] = nnb;
-    if (nbloss + nbgain > 0) {
-        for (int l = 0; l <= nnb; l++) orow[l] = row[l];
-        cnt[C_FLUX] += nbloss + nbgain;
+        s.d2r[3 * (size_t)i + k] = d2r[k];
+        s.d3r[3 * (size_t)i + k] = d3r[k];
     }
 
     // new regular step, regcor_gpu.F:624-740
-    fr2 = freg[0] * freg[0] + freg[1] * freg[1] + freg[2] * freg[2];
+    const double fr2 = freg[0] * freg[0] + freg[1] * freg[1] + freg[2] * freg[2];
     const double w1 = fdr[0] * fdr[0] + fdr[1] * fdr[1] + fdr[2] * fdr[2];
     const double w2 = d2r[0] * d2r[0] + d2r[1] * d2r[1] + d2r[2] * d2r[2];
     const double w3 = d3r[0] * d3r[0] + d3r[1] * d3r[1] + d3r[2] * d3r[2];
@@ -440,50 +609,61 @@ __device__ void regcor_one(const nb6_regcor_params &P, const nb6_regstate &s, in
             const double e = vi[k] - xnd[k];
             dv2 = dv2 + e * e;
         }
-        const double q = dv2 != 0.0 ? 1.0e-4 * (dtr * dtr) * fr2 / dv2 : INFINITY;
+        const double q = 1.0e-4 * (dtr * dtr) * fr2 / dv2;
         const double dtj = dtr * pow(q, (double)0.1f);
         if (dtj < ttmp) ttmp = dtr;
     }
     ttmp = fmin(ttmp, smax);
     s.stepr[i] = ttmp;
-    if (nnb0 == 0) {        // STEP itself is scaled by regcor_step_kernel after the block
-        s.timenw[i] = s.t0[i] + 0.125 * stp;
-        *shrink = 1;
+    if (nnb0 == 0) {
+        s.step[i] = 0.125 * stp;
+        s.timenw[i] = s.t0[i] + s.step[i];
     }
 }
 
-__global__ __launch_bounds__(256) void regcor_kernel(nb6_regcor_params P, nb6_regstate s, int ni,
-                                                     const int *__restrict__ addr,
-                                                     const double *__restrict__ force, int fstride,
-                                                     int *__restrict__ nlist, int *__restrict__ counters,
-                                                     unsigned char *__restrict__ shrink) {
+__global__ __launch_bounds__(256) void regcor_lists_kernel(nb6_regcor_params P, nb6_regstate s,
+                                                           int ni, const int *__restrict__ addr,
+                                                           const double *__restrict__ force,
+                                                           int fstride, int *__restrict__ nlist,
+                                                           int *__restrict__ counters,
+                                                           double *__restrict__ hd,
+                                                           int *__restrict__ hi) {
+    extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int red[NCOUNT];
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
     if (threadIdx.x < NCOUNT) red[threadIdx.x] = 0;
     __syncthreads();
-    const int k = blockIdx.x * 256 + threadIdx.x;
+    const int k = blockIdx.x * (blockDim.x / WAVE) + w;
     int cnt[NCOUNT] = {0, 0, 0, 0, 0};
     if (k < ni) {
-        shrink[k] = 0;
-        regcor_one(P, s, addr[k], force + k, fstride, nlist + (size_t)k * s.lmax, cnt, shrink + k);
+        unsigned char *base = lds + (size_t)w * wave_lds_bytes(s.lmax);
+        double *T = reinterpret_cast<double *>(base);
+        int *nl = reinterpret_cast<int *>(base + (size_t)NTERM * WAVE * 8);
+        int *ol = nl + (s.lmax + 1);
+        unsigned *lb = reinterpret_cast<unsigned *>(ol + (s.lmax + 1));
+        unsigned *cb = lb + loss_words(s.lmax);
+        if (lane == 0) hi[HI_FLAGS * ni + k] = 0;
+        regcor_wave(P, s, addr[k], force + k, fstride, nlist + (size_t)k * s.lmax, cnt, T, nl, ol,
+                    lb, cb, lane, hd + k, hi + k, ni);
     }
+    if (lane == 0) {
 #pragma unroll
-    for (int q = 0; q < NCOUNT; q++)
-        if (cnt[q]) atomicAdd(&red[q], cnt[q]);
+        for (int q = 0; q < NCOUNT; q++)
+            if (cnt[q]) atomicAdd(&red[q], cnt[q]);
+    }
     __syncthreads();
     if (threadIdx.x < NCOUNT && red[threadIdx.x]) atomicAdd(&counters[threadIdx.x], red[threadIdx.x]);
 }
 
-// STEP(I) = STEP(I) / 8 for particles whose old list was empty (regcor_gpu.F:736-739),
-// applied after the whole block so that every retention test of the block reads the
-// steps the block started with
-__global__ __launch_bounds__(256) void regcor_step_kernel(int ni, const int *__restrict__ addr,
-                                                          const unsigned char *__restrict__ shrink,
-                                                          double *__restrict__ step) {
+__global__ __launch_bounds__(256) void regcor_finish_kernel(nb6_regcor_params P, nb6_regstate s,
+                                                            int ni, const int *__restrict__ addr,
+                                                            const double *__restrict__ hd,
+                                                            const int *__restrict__ hi) {
     const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k < ni && shrink[k]) step[addr[k] - 1] = 0.125 * step[addr[k] - 1];
+    if (k < ni && (hi[HI_FLAGS * ni + k] & 1)) regcor_finish(P, s, addr[k], hd + k, hi + k, ni);
 }
 
-DBuf g_shrink;
+DBuf g_hand;
 
 }  // namespace
 
@@ -512,12 +692,25 @@ int regcor_device(const nb6_regcor_params *p, const nb6_regstate *s, int ni, con
     if (!d.ready) return E_NODEV;
     KTimer &kt = ktimer("regcor");
     kt.start(stream);
-    auto *shrink = static_cast<unsigned char *>(g_shrink.get((size_t)ni));
-    hipLaunchKernelGGL(regcor_kernel, dim3((ni + 255) / 256), dim3(256), 0, stream, *p, *s, ni,
-                       d_addr, d_force, force_stride, d_nlist, d_counters, shrink);
+    // one wavefront per particle, up to four per work-group within the LDS of a CU
+    const size_t wl = wave_lds_bytes(s->lmax);
+    const int wpb = (int)std::min<size_t>(4, (160 * 1024) / wl);
+    if (wpb < 1) { set_error(E_ARG, "nb6_regcor_device: lmax too large for LDS"); return E_ARG; }
+    const size_t lds = wl * wpb;
+    static size_t lds_set = 0;
+    if (lds > lds_set) {
+        NB6_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(regcor_lists_kernel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        lds_set = lds;
+    }
+    char *h = static_cast<char *>(g_hand.get((size_t)ni * (NHD * 8 + NHI * 4)));
+    double *hd = reinterpret_cast<double *>(h);
+    int *hi = reinterpret_cast<int *>(h + (size_t)ni * NHD * 8);
+    hipLaunchKernelGGL(regcor_lists_kernel, dim3((ni + wpb - 1) / wpb), dim3(WAVE * wpb), lds,
+                       stream, *p, *s, ni, d_addr, d_force, force_stride, d_nlist, d_counters, hd, hi);
     NB6_HIP(hipGetLastError());
-    hipLaunchKernelGGL(regcor_step_kernel, dim3((ni + 255) / 256), dim3(256), 0, stream, ni,
-                       d_addr, shrink, s->step);
+    hipLaunchKernelGGL(regcor_finish_kernel, dim3((ni + 255) / 256), dim3(256), 0, stream, *p, *s,
+                       ni, d_addr, hd, hi);
     NB6_HIP(hipGetLastError());
     kt.stop(stream);
     return OK;

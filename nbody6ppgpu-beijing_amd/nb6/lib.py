@@ -102,7 +102,7 @@ def load():
 # --------------------------------------------------------------- regular corrector
 # Field order of nb6_regstate / nb6_regcor_params (include/nb6gpu.h).
 REG3 = ("x", "xdot", "x0", "x0dot", "f", "fdot", "fi", "fidot", "fr", "frdot", "d2", "d3",
-        "xn", "xndot", "d0", "d0r", "d1r", "d2r", "d3r")      # [3][nmax]
+        "xn", "xndot", "d0", "d0r", "d1r", "d2r", "d3r")      # [nmax][3] on the device
 REG1 = ("body", "t0", "step", "timenw", "t0r", "stepr", "rs")  # [nmax]
 _STATE_PTRS = ("x", "xdot", "body", "x0", "x0dot", "f", "fdot", "t0", "h", "step", "timenw",
                "fi", "fidot", "fr", "frdot", "d2", "d3", "t0r", "stepr", "rs", "list", "xn",
@@ -136,8 +136,9 @@ def regcor_params(P: dict) -> RegcorParams:
 class Regcor:
     """Device copy of the per-particle state of the regular corrector and the call
     nb6_regcor_device (REGCOR_GPU over one block, regcor_gpu.F:1-740).  S holds numpy
-    arrays named as in nb6_regstate: [3][nmax] for REG3, [nmax] for REG1, list
-    int32 [nmax][lmax], h [npairs] (optional)."""
+    arrays named as in nb6_regstate: [3][nmax] for REG3 (stored on the device as
+    [nmax][3], the Fortran X(3,NMAX) layout), [nmax] for REG1, list int32
+    [nmax][lmax], h [nmax - N] (optional)."""
 
     def __init__(self, S: dict):
         self.L = load()
@@ -145,7 +146,7 @@ class Regcor:
         self.lmax = int(S["list"].shape[1])
         self.dev = {}
         for k in REG3:
-            self.dev[k] = DeviceArray.from_host(c64(S[k]).reshape(3, self.nmax))
+            self.dev[k] = DeviceArray.from_host(c64(np.asarray(S[k]).reshape(3, self.nmax).T))
         for k in REG1:
             self.dev[k] = DeviceArray.from_host(c64(S[k]).reshape(self.nmax))
         self.dev["list"] = DeviceArray.from_host(np.ascontiguousarray(S["list"], np.int32))
@@ -166,7 +167,7 @@ class Regcor:
             raise RuntimeError(f"nb6_regcor_device returned {rc}")
 
     def download(self) -> dict:
-        out = {k: self.dev[k].to_host(np.float64, (3, self.nmax)) for k in REG3}
+        out = {k: self.dev[k].to_host(np.float64, (self.nmax, 3)).T.copy() for k in REG3}
         out.update({k: self.dev[k].to_host(np.float64, (self.nmax,)) for k in REG1})
         out["list"] = self.dev["list"].to_host(np.int32, (self.nmax, self.lmax))
         return out
