@@ -615,6 +615,87 @@ def bench_regcor(steps, warmup, no_cpu):
             "cpu_baseline": cpu, "check_vs_oracle": check}
 
 
+def bench_energy(steps, warmup, no_cpu):
+    """Energy path of the GPU build (energy.F:28-62: GPUPOT over all bodies, PHICOR,
+    POT/ZKIN sums) through nb6_energy_device on the N = 1M King W0 = 6 system with its
+    binaries as single bodies.  The dominant kernel is pot_tile_kernel: N^2 pair terms,
+    counted at 20 flops per pair (float-float difference 9, r^2 5, rsqrt 1, m rinv 1,
+    compensated add 4) against the FP32 vector peak."""
+    import ctypes
+    m, x, v, idx, h2, dtr = make_workload()
+    n = m.size
+    L = lib.load()
+    d = {"x": lib.DeviceArray.from_host(np.ascontiguousarray(x)),
+         "xdot": lib.DeviceArray.from_host(np.ascontiguousarray(v)),
+         "body": lib.DeviceArray.from_host(np.ascontiguousarray(m)),
+         "list": lib.DeviceArray.from_host(np.zeros(n, np.int32)),
+         "phidbl": lib.DeviceArray(n * 8)}
+    e = lib.EnergyIn(nmax=n, lmax=1, n=n, ntot=n, npairs=0, cmsep2_rmin2=0.0)
+    for k, a in d.items():
+        setattr(e, k, a.ptr)
+    out = (ctypes.c_double * 3)()
+
+    def call():
+        if L.nb6_energy_device(ctypes.byref(e), ctypes.cast(out, ctypes.c_void_p), None) != 0:
+            lib.check_error()
+    for _ in range(warmup):
+        call()
+    kms, pms = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        call()
+        kms.append(L.nb6_kernel_ms(b"energy"))
+        pms.append(L.nb6_kernel_ms(b"pot"))
+    wall = (time.perf_counter() - t0) / steps
+    te, tp = float(np.mean(kms)) * 1e-3, float(np.mean(pms)) * 1e-3
+    pairs = float(n) * n
+    phi = d["phidbl"].to_host(np.float64, (n,))
+    zkin, pot = out[0], out[1]
+    for a in d.values():
+        a.free()
+    check = cpu = None
+    if not no_cpu:
+        import oracle as O
+        k = 256
+        ref = O.pot(m, x, 1, k)
+        err = np.abs(phi[:k] - ref) / ref
+        check = {"sample_i": k, "phi_max_rel": float(err.max()),
+                 "pot_vs_sample_mean": float(np.mean(m[:k] * (phi[:k] - ref)) / np.mean(m[:k] * ref)),
+                 "zkin": zkin, "pot": pot}
+        if O.ref_available():
+            R = O._ref("pot")
+            ns = 262144                      # about 5 s on 16 host cores
+            n8 = (n + 7) // 8 * 8 + 8
+            mp = np.zeros(n8)
+            xp = np.zeros((n8, 3))
+            mp[:n], xp[:n] = m, x
+            po = np.zeros(n8)
+            tc = time.perf_counter()
+            R.gpupot_(ctypes.byref(ctypes.c_int(0)), ctypes.byref(ctypes.c_int(1)),
+                      ctypes.byref(ctypes.c_int(ns)), ctypes.byref(ctypes.c_int(n)),
+                      mp.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                      xp.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                      po.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+            tc = time.perf_counter() - tc
+            cpu = {"value": ns * float(n) / tc, "unit": "pair-interactions/s",
+                   "cores": O.num_threads(), "kind": "reference",
+                   "sample": f"src/Main/pot.avx.cpp gpupot_ for {ns} of the {n} bodies against all "
+                             f"{n}, {tc:.1f} s, OMP_NUM_THREADS={O.num_threads()}"}
+    achieved = 20.0 * pairs / tp / 1e12
+    return {"metric": "energy path (GPUPOT + PHICOR + sums): pair-interactions/s at N=1M",
+            "value": pairs / te, "unit": "pair-interactions/s", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": wall * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None,
+            "dtype": "fp32 pair math, fp64 separations and sums",
+            "data": "synthetic (seeded King W0=6, 50% binaries, Kroupa masses)",
+            "config": {"workload": "energy", "n": n, "npairs": 0},
+            "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                         "kernel": "pot_tile_kernel", "avg_launch_ms": tp * 1e3,
+                         "energy_call_ms": te * 1e3, "flops_per_pair": 20},
+            "cpu_baseline": cpu, "check_vs_fp64": check}
+
+
 def rc_force_host(d_force, ni):
     return d_force.to_host(np.float64, (7, ni))
 
@@ -642,7 +723,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--workload", default="regf",
-                    choices=["regf", "ensemble", "learned-rs", "precision", "irr", "regcor"],
+                    choices=["regf", "ensemble", "learned-rs", "precision", "irr", "regcor",
+                             "energy"],
                     help="regf: the headline N=1M step; ensemble: 256-cluster batch vs loop; "
                          "learned-rs: overflow retries of the RS network vs the reference rule")
     ap.add_argument("--rs-train-samples", type=int, default=10_000_000)
@@ -669,9 +751,12 @@ def main():
         return 0
     if args.workload == "irr":
         print(json.dumps(bench_irr(args.steps, args.warmup, args.no_cpu)), flush=True)
-        return
+        return 0
     if args.workload == "regcor":
         print(json.dumps(bench_regcor(args.steps, args.warmup, args.no_cpu)), flush=True)
+        return 0
+    if args.workload == "energy":
+        print(json.dumps(bench_energy(args.steps, args.warmup, args.no_cpu)), flush=True)
         return 0
     if args.workload == "precision":
         print(json.dumps(bench_precision(args.no_cpu)), flush=True)

@@ -357,6 +357,28 @@ int nb6_regcor_device(const nb6_regcor_params *p, const nb6_regstate *s, int ni,
                       const int *d_addr, const double *d_force, int force_stride,
                       int *d_nlist, int *d_counters, void *stream);
 
+/* ---------------------------------------------------------------------------
+ * Part 5 (extension): total-energy terms on the GPU.  Replaces the GPU branch of
+ * ENERGY (energy.F:28-62 and :119-124: GPUPOT over IFIRST..NTOT, the c.m. potential
+ * copied to the KS components, PHICOR of phicor.f:1-76, the POT and ZKIN sums) and
+ * the EBIN sum of energy.F:13-24 given each pair's HT.  Device arrays as in Part 4.
+ * ------------------------------------------------------------------------- */
+typedef struct nb6_energy_in {
+    int nmax, lmax;
+    int n, ntot, npairs;              /* N, NTOT = N + NPAIRS, NPAIRS (IFIRST = 2 NPAIRS + 1) */
+    double cmsep2_rmin2;              /* CMSEP2 * RMIN2 (PHICOR's c.m. separation test) */
+    const double *x, *xdot;           /* [nmax][3] X (components resolved), XDOT */
+    const double *body;               /* [nmax] BODY */
+    const int *list;                  /* int[nmax][lmax] LIST (1-based); rows of the first
+                                         KS components hold their perturber lists */
+    const double *ht;                 /* [npairs] HT of each pair (may be null if npairs = 0) */
+    double *phidbl;                   /* [nmax] out: PHIDBL (positive potential) */
+} nb6_energy_in;
+
+/* h_out (host double[3]) = ZKIN, POT, EBIN; the caller's total energy is
+ * ZKIN - POT + EBIN (+ EPL, ESUB, EMERGE, ECOLL, chain terms).  Synchronises. */
+int nb6_energy_device(const nb6_energy_in *e, double *h_out, void *stream);
+
 /* Device allocation / copies for callers without another GPU runtime
  * (tests, benchmarks). */
 void *nb6_dev_alloc(unsigned long long bytes);

@@ -185,7 +185,9 @@ int irr_firr_device(double ti, int ni, const int *d_addr, double *d_out, hipStre
 }  // namespace nb6
 struct nb6_regcor_params;
 struct nb6_regstate;
+struct nb6_energy_in;
 namespace nb6 {
+int energy_device(const ::nb6_energy_in *e, double *h_out, hipStream_t s);
 int regcor_device(const ::nb6_regcor_params *p, const ::nb6_regstate *s, int ni,
                   const int *d_addr, const double *d_force, int force_stride, int *d_nlist,
                   int *d_counters, hipStream_t stream);

@@ -95,6 +95,7 @@ def load():
     L.nb6_rs_kth_device.argtypes = [i, i, _vp, _vp, i, i, _vp, _vp]
     L.nb6_regcor_device.argtypes = [ctypes.POINTER(RegcorParams), ctypes.POINTER(RegState), i,
                                     _vp, _vp, i, _vp, _vp, _vp]
+    L.nb6_energy_device.argtypes = [ctypes.POINTER(EnergyIn), _vp, _vp]
     _L = L
     return L
 
@@ -176,6 +177,42 @@ class Regcor:
         for v in self.dev.values():
             if v is not None:
                 v.free()
+
+
+# --------------------------------------------------------------- energy
+class EnergyIn(ctypes.Structure):
+    _fields_ = [("nmax", ctypes.c_int), ("lmax", ctypes.c_int), ("n", ctypes.c_int),
+                ("ntot", ctypes.c_int), ("npairs", ctypes.c_int),
+                ("cmsep2_rmin2", ctypes.c_double)] + [
+        (k, ctypes.c_void_p) for k in ("x", "xdot", "body", "list", "ht", "phidbl")]
+
+
+def energy(x, xdot, body, lst, n, npairs, cmsep2_rmin2, ht=None):
+    """ENERGY's GPU branch (energy.F:28-62, :119-124, phicor.f) through
+    nb6_energy_device: returns (ZKIN, POT, EBIN, PHIDBL).  x, xdot [nmax][3], body
+    [nmax], lst int [nmax][lmax] (count, 1-based), ht [npairs]."""
+    L = load()
+    nmax, lmax = len(body), lst.shape[1]
+    d = {k: DeviceArray.from_host(c64(a)) for k, a in (("x", x), ("xdot", xdot), ("body", body))}
+    d["list"] = DeviceArray.from_host(np.ascontiguousarray(lst, np.int32))
+    d["ht"] = DeviceArray.from_host(c64(ht)) if npairs else None
+    d["phidbl"] = DeviceArray(nmax * 8)
+    e = EnergyIn(nmax=nmax, lmax=lmax, n=n, ntot=n + npairs, npairs=npairs,
+                 cmsep2_rmin2=cmsep2_rmin2)
+    for k, v in d.items():
+        setattr(e, k, v.ptr if v is not None else None)
+    out = (ctypes.c_double * 3)()
+    try:
+        rc = L.nb6_energy_device(ctypes.byref(e), ctypes.cast(out, ctypes.c_void_p), None)
+        if rc:
+            check_error()
+            raise RuntimeError(f"nb6_energy_device returned {rc}")
+        phi = d["phidbl"].to_host(np.float64, (nmax,))
+    finally:
+        for v in d.values():
+            if v is not None:
+                v.free()
+    return out[0], out[1], out[2], phi
 
 
 def header_symbols() -> list[str]:

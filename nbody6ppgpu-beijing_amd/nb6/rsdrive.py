@@ -365,12 +365,12 @@ class Sim:
         return self.stats
 
     def energy(self):
-        """Kinetic + potential energy at the common time of the last full block."""
+        """ZKIN - POT at the common time of the last full block, through the energy path
+        of the reference's GPU build (energy.F:28-62: GPUPOT, PHICOR, sums), here
+        nb6_energy_device with single bodies only."""
         xp, vp = self.predict(self.t)
-        self._send(xp, vp)
-        out, _ = self._regf(xp, vp, np.zeros(self.n), np.zeros(self.n), self.d.tmp, 2, 1)
-        ekin = 0.5 * (self.m[:, None] * vp ** 2).sum()
-        return ekin - 0.5 * (self.m * out[6]).sum()
+        zkin, pot, _, _ = lib.energy(xp, vp, self.m, np.zeros((self.n, 1), np.int32), self.n, 0, 0.0)
+        return zkin - pot
 
 
 def _overflow(rs, nnb, nnbopt, nnbmax):

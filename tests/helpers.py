@@ -181,3 +181,54 @@ def regcor_case(seed=7, n=800, ni=300, lmax=160, nnbopt=20, ncm=0, kz38=1, nb_fl
 
 def copy_state(S):
     return {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in S.items()}
+
+
+# ------------------------------------------------------------------ energy with KS pairs
+def energy_case(seed=31, nsing=600, npairs=40, lmax=200, rs=0.25, rpert=0.5, full=False):
+    """Bodies in common6.h order: KS components 1..2P, singles 2P+1..N, centres of mass
+    N+1..N+P (masses m1 + m2, positions and velocities the mass-weighted means).  LIST
+    rows: singles list singles and c.m. bodies within rs (ascending, so c.m. bodies come
+    last); each pair's first component lists its perturbers (singles and other c.m.
+    bodies) within rpert.  full=True lists every body instead.  Returns
+    (x [nmax][3], xdot, body, lst, n, npairs, ht)."""
+    rng = np.random.default_rng(seed)
+    nb = nsing + npairs
+    m, x, v = ics.plummer(nb, seed)
+    P = npairs
+    n = 2 * P + nsing
+    nmax = n + P
+    X = np.zeros((nmax, 3))
+    V = np.zeros((nmax, 3))
+    M = np.zeros(nmax)
+    X[2 * P:n], V[2 * P:n], M[2 * P:n] = x[:nsing], v[:nsing], m[:nsing]
+    X[n:], V[n:], M[n:] = x[nsing:], v[nsing:], m[nsing:]
+    for p in range(P):
+        q = rng.uniform(0.2, 0.8)
+        mc = M[n + p]
+        m1, m2 = q * mc, (1 - q) * mc
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        a = 10 ** rng.uniform(-3.5, -2)
+        w = rng.normal(size=3) * 0.3
+        X[2 * p] = X[n + p] + (m2 / mc) * a * u
+        X[2 * p + 1] = X[n + p] - (m1 / mc) * a * u
+        V[2 * p] = V[n + p] + (m2 / mc) * w
+        V[2 * p + 1] = V[n + p] - (m1 / mc) * w
+        M[2 * p], M[2 * p + 1] = m1, m2
+    lst = np.zeros((nmax, lmax), np.int32)
+    bodies = np.arange(2 * P + 1, nmax + 1)          # singles and c.m. bodies, 1-based
+    for I in range(2 * P + 1, n + 1):
+        d = X[bodies - 1] - X[I - 1]
+        sel = bodies[((d * d).sum(1) < rs * rs) | full]
+        sel = sel[sel != I][:lmax - 1]
+        lst[I - 1, 0] = len(sel)
+        lst[I - 1, 1:1 + len(sel)] = sel
+    for p in range(P):
+        I = n + p + 1
+        d = X[bodies - 1] - X[I - 1]
+        sel = bodies[((d * d).sum(1) < rpert * rpert) | full]
+        sel = sel[sel != I][:lmax - 1]
+        lst[2 * p, 0] = len(sel)
+        lst[2 * p, 1:1 + len(sel)] = sel
+    ht = -rng.uniform(1.0, 50.0, P)
+    return X, V, M, lst, n, P, ht
