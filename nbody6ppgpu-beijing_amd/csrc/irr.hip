@@ -223,7 +223,8 @@ __device__ __forceinline__ int xcd_order(int b, int G) {
 
 template <int LPI>
 __global__ __launch_bounds__(256) void irr_force_kernel(
-    double ti, int ni, const int *__restrict__ addr, const int *__restrict__ order,
+    double ti, int ni, const int *__restrict__ addr /* in processing order */,
+    const int *__restrict__ order /* processing slot -> output row; nullptr: the same */,
     const IrrRec *__restrict__ rec, const double *__restrict__ tim,
     const int *__restrict__ lists, int lmax,
     double *__restrict__ out /* [ni][8]: acc3, jrk3, nnbid, nnb */) {
@@ -233,24 +234,30 @@ __global__ __launch_bounds__(256) void irr_force_kernel(
     // the whole group leaves together; the shuffles below stay within a group
     if (q >= ni) return;
     const int w = order ? order[q] : q;
-    const int ia = addr[w] - 1;
+    const int ia = addr[q] - 1;
     const Pred pi = predict(rec, tim, ia, ti);
     const int *row = lists + (size_t)ia * lmax;
+    // the lane's first two entries are read with the count, and each loop trip reads the
+    // next two while the current records are in flight (entries past the count are
+    // never used)
+    int k = lane;
+    int j0 = 1 + k < lmax ? row[1 + k] : 0;
+    int j1 = 1 + k + LPI < lmax ? row[1 + k + LPI] : 0;
     const int nnb = row[0];
     double ax = 0.0, ay = 0.0, az = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
     unsigned long long best = ~0ull;
-    int k = lane;
     for (; k + LPI < nnb; k += 2 * LPI) {
-        const int j0 = row[1 + k], j1 = row[1 + k + LPI];
         const Pred p0 = predict(rec, tim, j0, ti);
         const Pred p1 = predict(rec, tim, j1, ti);
+        const int kn = k + 2 * LPI;
+        const int n0 = 1 + kn < lmax ? row[1 + kn] : 0;
+        const int n1 = 1 + kn + LPI < lmax ? row[1 + kn + LPI] : 0;
         irr_pair(pi, p0, j0, ax, ay, az, jx, jy, jz, best);
         irr_pair(pi, p1, j1, ax, ay, az, jx, jy, jz, best);
+        j0 = n0;
+        j1 = n1;
     }
-    if (k < nnb) {
-        const int j0 = row[1 + k];
-        irr_pair(pi, predict(rec, tim, j0, ti), j0, ax, ay, az, jx, jy, jz, best);
-    }
+    if (k < nnb) irr_pair(pi, predict(rec, tim, j0, ti), j0, ax, ay, az, jx, jy, jz, best);
     const double sax = gsum<LPI>(ax), say = gsum<LPI>(ay), saz = gsum<LPI>(az);
     const double sjx = gsum<LPI>(jx), sjy = gsum<LPI>(jy), sjz = gsum<LPI>(jz);
 #pragma unroll
@@ -300,8 +307,9 @@ void launch_irr(double ti, int ni, const int *d_addr, const int *order, const Ir
 }
 
 // processing order of the active i: Morton order of their record positions in the
-// box of the gpunb j-set (nullptr: as given, when sorting is off or no box is known)
-const int *irr_order(int ni, const int *d_addr, hipStream_t s);
+// box of the gpunb j-set (nullptr: as given, when sorting is off or no box is known);
+// *oaddr receives the addresses in processing order
+const int *irr_order(int ni, const int *d_addr, const int **oaddr, hipStream_t s);
 
 void flush_logs(hipStream_t s) {
     IrrState &g = g_irr;
@@ -353,9 +361,17 @@ void flush_logs(hipStream_t s) {
 }  // namespace
 
 namespace {
-const int *irr_order(int ni, const int *d_addr, hipStream_t s) {
+__global__ __launch_bounds__(256) void irr_oaddr_kernel(int ni, const int *__restrict__ addr,
+                                                        const int *__restrict__ order,
+                                                        int *__restrict__ oaddr) {
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q < ni) oaddr[q] = addr[order[q]];
+}
+
+const int *irr_order(int ni, const int *d_addr, const int **oaddr, hipStream_t s) {
     IrrState &g = g_irr;
     const JSet *js = gpunb_jset();
+    *oaddr = d_addr;
     if (!g_irrcfg.sort_i || ni < 2048 || !js || js->n <= 0) return nullptr;
     double sc[3];
     for (int d = 0; d < 3; d++) {
@@ -374,6 +390,11 @@ const int *irr_order(int ni, const int *d_addr, hipStream_t s) {
     NB6_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, ni, 0, 30, s));
     void *tmp = g.d_otmp.get(tb);
     NB6_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, v1, ni, 0, 30, s));
+    int *oa = reinterpret_cast<int *>(k0);     // the unsorted keys are no longer needed
+    hipLaunchKernelGGL(irr_oaddr_kernel, dim3((ni + 255) / 256), dim3(256), 0, s, ni, d_addr, v1,
+                       oa);
+    NB6_HIP(hipGetLastError());
+    *oaddr = oa;
     return v1;
 }
 }  // namespace
@@ -531,10 +552,11 @@ int irr_firr_vec(double ti, int ni, const int *addr, double (*acc)[3], double (*
     double *dout = static_cast<double *>(g.d_out.get((size_t)ni * 8 * sizeof(double)));
     double *hout = static_cast<double *>(g.h_out.get((size_t)ni * 8 * sizeof(double)));
     NB6_HIP(hipMemcpyAsync(da, ha, (size_t)ni * sizeof(int), hipMemcpyHostToDevice, s));
-    const int *order = irr_order(ni, da, s);
+    const int *oaddr = nullptr;
+    const int *order = irr_order(ni, da, &oaddr, s);
     KTimer &t = ktimer("irr");
     t.start(s);
-    launch_irr(ti, ni, da, order, g, dout, s);
+    launch_irr(ti, ni, oaddr, order, g, dout, s);
     t.stop(s);
     NB6_HIP(hipMemcpyAsync(hout, dout, (size_t)ni * 8 * sizeof(double), hipMemcpyDeviceToHost, s));
     NB6_HIP(hipStreamSynchronize(s));
@@ -599,10 +621,11 @@ int irr_firr_device(double ti, int ni, const int *d_addr, double *d_out, hipStre
     if (!g.open) { set_error(E_STATE, "nb6_irr_firr_device: irr library not open"); return E_STATE; }
     if (ni <= 0) return OK;
     flush_logs(s);
-    const int *order = irr_order(ni, d_addr, s);
+    const int *oaddr = nullptr;
+    const int *order = irr_order(ni, d_addr, &oaddr, s);
     KTimer &t = ktimer("irr");
     t.start(s);
-    launch_irr(ti, ni, d_addr, order, g, d_out, s);
+    launch_irr(ti, ni, oaddr, order, g, d_out, s);
     t.stop(s);
     g.num_fcall++;
     g.num_steps += ni;
