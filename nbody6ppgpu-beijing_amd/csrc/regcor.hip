@@ -228,9 +228,28 @@ __device__ void regcor_wave(const nb6_regcor_params &P, const nb6_regstate &s, i
     const int i = I - 1;
     if (I > P.n && !s.h) { cnt[C_UNS]++; return; }
 
-    // regf row -> 1-based list without self, util_gpu.F:101-110 (ballot compaction)
+    // every load that depends only on I is issued before any is used: the regf row's
+    // count and first 128 entries, the same of LIST(.,I), and the particle's own state
+    int *orow = s.list + (size_t)i * lmax;
     const int c = row[0];
-    if (c < 0 || c > lmax - 1) { cnt[C_UNS]++; return; }
+    const int nnb0 = orow[0];
+    int rpre0 = 1 + lane < lmax ? row[1 + lane] : 0;
+    int rpre1 = 1 + WAVE + lane < lmax ? row[1 + WAVE + lane] : 0;
+    int opre0 = 1 + lane < lmax ? orow[1 + lane] : 0;
+    int opre1 = 1 + WAVE + lane < lmax ? orow[1 + WAVE + lane] : 0;
+    double xi[3], vi[3], freg[3], fdr[3], fr_old[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        xi[k] = s.x[3 * (size_t)i + k];
+        vi[k] = s.xdot[3 * (size_t)i + k];
+        freg[k] = fg[(size_t)k * fstride];
+        fdr[k] = fg[(size_t)(3 + k) * fstride];
+        fr_old[k] = s.fr[3 * (size_t)i + k];
+    }
+    const double rs_old = s.rs[i];
+    if (c < 0 || c > lmax - 1 || nnb0 < 0 || nnb0 > lmax - 1) { cnt[C_UNS]++; return; }
+
+    // regf row -> 1-based list without self, util_gpu.F:101-110 (ballot compaction)
     int nnb = 0;
     bool bad = false;
     for (int b = 0; b < c; b += WAVE) {
@@ -238,7 +257,7 @@ __device__ void regcor_wave(const nb6_regcor_params &P, const nb6_regstate &s, i
         int t = 0;
         bool keep = false;
         if (l < c) {
-            t = row[1 + l] + P.ifirst;
+            t = (b == 0 ? rpre0 : b == WAVE ? rpre1 : row[1 + l]) + P.ifirst;
             bad = bad || t < 1 || t > s.nmax;
             keep = t != I;
         }
@@ -246,13 +265,10 @@ __device__ void regcor_wave(const nb6_regcor_params &P, const nb6_regstate &s, i
         if (keep) nl[nnb + lanes_below(m)] = t;
         nnb += __popcll(m);
     }
-    int *orow = s.list + (size_t)i * lmax;
-    const int nnb0 = orow[0];
-    if (nnb0 < 0 || nnb0 > lmax - 1) { cnt[C_UNS]++; return; }
     for (int b = 0; b < nnb0; b += WAVE) {
         const int l = b + lane;
         if (l < nnb0) {
-            const int t = orow[1 + l];
+            const int t = b == 0 ? opre0 : b == WAVE ? opre1 : orow[1 + l];
             bad = bad || t < 1 || t > s.nmax;
             ol[l] = t;
         }
@@ -260,15 +276,6 @@ __device__ void regcor_wave(const nb6_regcor_params &P, const nb6_regstate &s, i
     if (__ballot(bad)) { cnt[C_UNS]++; return; }
     __builtin_amdgcn_wave_barrier();
 
-    double xi[3], vi[3], freg[3], fdr[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        xi[k] = s.x[3 * (size_t)i + k];
-        vi[k] = s.xdot[3 * (size_t)i + k];
-        freg[k] = fg[(size_t)k * fstride];
-        fdr[k] = fg[(size_t)(3 + k) * fstride];
-    }
-    const double rs_old = s.rs[i];
     const double rs2 = rs_old * rs_old;
     double dfirr[3] = {0.0, 0.0, 0.0}, dfd[3] = {0.0, 0.0, 0.0};
     if (nnb == 0) {                                   // regcor_gpu.F:108-116
@@ -291,7 +298,7 @@ __device__ void regcor_wave(const nb6_regcor_params &P, const nb6_regstate &s, i
         fr2 = 0.0;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            const double e = freg[k] - s.fr[3 * (size_t)i + k];
+            const double e = freg[k] - fr_old[k];
             df2 = df2 + e * e;
             fr2 = fr2 + freg[k] * freg[k];
         }
