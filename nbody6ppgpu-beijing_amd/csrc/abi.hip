@@ -631,16 +631,14 @@ int nb6_irr_firr_device(double ti, int ni, const int *d_addr, double *d_out, voi
 }
 
 int nb6_energy_device(const nb6_energy_in *e, double *h_out, void *stream) {
-    if (!device().ready) return E_NODEV;
-    return energy_device(e, h_out, pick(stream));
+    return energy_device(e, h_out, static_cast<hipStream_t>(stream));   // validates first
 }
 
 int nb6_regcor_device(const nb6_regcor_params *p, const nb6_regstate *s, int ni,
                       const int *d_addr, const double *d_force, int force_stride,
                       int *d_nlist, int *d_counters, void *stream) {
-    if (!device().ready) return E_NODEV;
     return regcor_device(p, s, ni, d_addr, d_force, force_stride, d_nlist, d_counters,
-                         pick(stream));
+                         static_cast<hipStream_t>(stream));   // validates first
 }
 
 int nb6_rsnet_nparam(void) { return rsnet_nparam(); }

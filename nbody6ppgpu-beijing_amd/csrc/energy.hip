@@ -166,6 +166,7 @@ int energy_device(const nb6_energy_in *e, double *h_out, hipStream_t s) {
     }
     Device &d = device();
     if (!d.ready) return E_NODEV;
+    if (!s) s = d.stream;
     double *xs = static_cast<double *>(g_pack.get((size_t)4 * nnt * sizeof(double)));
     double *phi = static_cast<double *>(g_phi.get((size_t)nnt * sizeof(double)));
     const int nb = std::max(nnt, e->npairs);

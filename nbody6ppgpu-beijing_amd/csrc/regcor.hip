@@ -697,6 +697,7 @@ int regcor_device(const nb6_regcor_params *p, const nb6_regstate *s, int ni, con
         if (!q) { set_error(E_ARG, "nb6_regcor_device: null device array"); return E_ARG; }
     Device &d = device();
     if (!d.ready) return E_NODEV;
+    if (!stream) stream = d.stream;
     KTimer &kt = ktimer("regcor");
     kt.start(stream);
     // one wavefront per particle, up to four per work-group within the LDS of a CU
