@@ -97,13 +97,14 @@ __device__ __forceinline__ void fp_terms(const nb6_regstate &s, double time, int
 // ------------------------------------------------------------------ one wavefront per particle
 constexpr int WAVE = 64;
 constexpr int NTERM = 12;   // moved force terms: DFIRR[3], DFD[3], F2DOT[3], F3DOT[3]
+constexpr int TCH = 32;     // terms held in LDS at a time
 
-// LDS of one wavefront: one chunk of terms (double [NTERM][64]), the new and old lists
-// (int [lmax + 1] each) and two bits per old-list entry: lost, and candidate for the
-// SMIN retention
+// LDS of one wavefront: half a chunk of terms (double [NTERM][32]), the new and old
+// lists (int [lmax + 1] each) and two bits per old-list entry: lost, and candidate for
+// the SMIN retention
 __host__ __device__ constexpr int loss_words(int lmax) { return 2 * ((lmax + 64) / 64); }
 __host__ __device__ constexpr size_t wave_lds_bytes(int lmax) {
-    return ((size_t)NTERM * WAVE * 8 + (size_t)2 * (lmax + 1) * 4 +
+    return ((size_t)NTERM * TCH * 8 + (size_t)2 * (lmax + 1) * 4 +
             (size_t)2 * loss_words(lmax) * 4 + 15) & ~(size_t)15;
 }
 
@@ -423,39 +424,39 @@ __device__ void regcor_wave(const nb6_regcor_params &P, const nb6_regstate &s, i
             const unsigned long long m = __ballot(j1 != 0);
             if (!m) continue;
             bool ks = false;
+            double tv[NTERM];
+            const int pos = lanes_below(m);
             if (j1) {
-                const int pos = lanes_below(m);
-                double fa[3], fj[3];
-                pair_fj(s, j1 - 1, xi, vi, fa, fj);
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    T[k * WAVE + pos] = fa[k];
-                    T[(3 + k) * WAVE + pos] = fj[k];
-                }
+                pair_fj(s, j1 - 1, xi, vi, tv, tv + 3);
                 if (fps) {
-                    double f2[3] = {0.0, 0.0, 0.0}, f3[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int k = 6; k < NTERM; k++) tv[k] = 0.0;
                     if (j1 < P.ifirst)
                         ks = true;         // KS component: not supported, term left at zero
                     else
-                        fp_terms(s, time, i, j1 - 1, xi, vi, f2, f3);
-#pragma unroll
-                    for (int k = 0; k < 3; k++) {
-                        T[(6 + k) * WAVE + pos] = f2[k];
-                        T[(9 + k) * WAVE + pos] = f3[k];
-                    }
+                        fp_terms(s, time, i, j1 - 1, xi, vi, tv + 6, tv + 9);
                 }
             }
             cnt[C_UNS] += __popcll(__ballot(ks));
-            __builtin_amdgcn_wave_barrier();
+            // the compacted terms pass through LDS 32 at a time, in list order
             const int nt = __popcll(m);
-            if (lane < (fps ? NTERM : 6)) {
-                const double *Tq = T + lane * WAVE;
-                if (pass == 0)
-                    for (int t = 0; t < nt; t++) acc = acc - Tq[t];
-                else
-                    for (int t = 0; t < nt; t++) acc = acc + Tq[t];
+            for (int h = 0; h < nt; h += TCH) {
+                if (j1 && pos >= h && pos < h + TCH) {
+#pragma unroll
+                    for (int k = 0; k < NTERM; k++)
+                        if (k < 6 || fps) T[k * TCH + (pos - h)] = tv[k];
+                }
+                __builtin_amdgcn_wave_barrier();
+                const int nh = min(TCH, nt - h);
+                if (lane < (fps ? NTERM : 6)) {
+                    const double *Tq = T + lane * TCH;
+                    if (pass == 0)
+                        for (int t = 0; t < nh; t++) acc = acc - Tq[t];
+                    else
+                        for (int t = 0; t < nh; t++) acc = acc + Tq[t];
+                }
+                __builtin_amdgcn_wave_barrier();
             }
-            __builtin_amdgcn_wave_barrier();
         }
     }
     double sv2[3], sv3[3];
@@ -645,7 +646,7 @@ __global__ __launch_bounds__(256) void regcor_lists_kernel(nb6_regcor_params P, 
     if (k < ni) {
         unsigned char *base = lds + (size_t)w * wave_lds_bytes(s.lmax);
         double *T = reinterpret_cast<double *>(base);
-        int *nl = reinterpret_cast<int *>(base + (size_t)NTERM * WAVE * 8);
+        int *nl = reinterpret_cast<int *>(base + (size_t)NTERM * TCH * 8);
         int *ol = nl + (s.lmax + 1);
         unsigned *lb = reinterpret_cast<unsigned *>(ol + (s.lmax + 1));
         unsigned *cb = lb + loss_words(s.lmax);
