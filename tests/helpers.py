@@ -94,18 +94,21 @@ def pair_forces(x, v, m, i, js):
 
 def regcor_case(seed=7, n=800, ni=300, lmax=160, nnbopt=20, ncm=0, kz38=1, nb_flag=1,
                 m_flag=0, tidal4=0.0, frac_void=0.04, frac_empty_old=0.05,
-                frac_small_step=0.25, frac_few=0.04, frac_same=0.3):
+                frac_small_step=0.25, frac_few=0.04, frac_same=0.3, ifirst=1):
     """A regular block for the corrector: state S (oracle layout), parameters P,
     1-based addresses, the regf output force [7][ni] (FP64 regular force and jerk over
     the non-neighbours) and the regf rows [ni][lmax] (0-based, self included).
-    Particles n-ncm.. are c.m. bodies with binding energies h.  Steps are powers of
-    two commensurate with TIME = 1, so every branch of the step rule is reachable."""
+    Particles n-ncm.. are c.m. bodies with binding energies h.  With ifirst > 1 the
+    bodies 1..ifirst-1 stand for KS components: they are outside the j-set (regf rows
+    are positions in IFIRST..NTOT), in no list and in no block.  Steps are powers of two
+    commensurate with TIME = 1, so every branch of the step rule is reachable."""
     rng = np.random.default_rng(seed)
     m, x, v = ics.plummer(n, seed)
     nmax = n
     time = 1.0
     dtmin = 2.0 ** -14
-    addr0 = np.sort(rng.choice(n, ni, replace=False))
+    f0 = ifirst - 1                                   # first j-set body, 0-based
+    addr0 = np.sort(f0 + rng.choice(n - f0, ni, replace=False))
     kdt = rng.integers(3, 8, n)
     dtr_all = 2.0 ** (-kdt.astype(float))
     step = dtr_all / 2.0 ** rng.integers(0, 4, n)
@@ -119,6 +122,7 @@ def regcor_case(seed=7, n=800, ni=300, lmax=160, nnbopt=20, ncm=0, kz38=1, nb_fl
     rs_old = rs * rng.uniform(0.85, 1.15, n)
     x_prev = x - dtr_all[:, None] * v                 # positions one regular step back
     old = _nb_lists(x_prev, v, rs_old ** 2, dtr_all, np.arange(n), exclude_self=True)
+    old = [o[o >= f0] for o in old]
     lst = np.zeros((nmax, lmax), np.int32)
     empty_old = rng.random(n) < frac_empty_old
     for i in range(n):
@@ -128,13 +132,14 @@ def regcor_case(seed=7, n=800, ni=300, lmax=160, nnbopt=20, ncm=0, kz38=1, nb_fl
     # new rows in regf form: 0-based positions, self included; void rows hold self only
     dtr = dtr_all[addr0]
     new = _nb_lists(x, v, rs[addr0] ** 2, dtr, addr0, exclude_self=False)
+    new = [o[o >= f0] for o in new]
     rows = np.zeros((ni, lmax), np.int32)
     void = rng.random(ni) < frac_void
     force = np.zeros((7, ni))
     for k, i in enumerate(addr0):
         js = np.array([i]) if void[k] else new[k][:nnbmax]
         rows[k, 0] = len(js)
-        rows[k, 1:1 + len(js)] = js
+        rows[k, 1:1 + len(js)] = js - f0             # regf rows: positions in the j-set
         non = np.setdiff1d(np.arange(n), js)
         f, j = pair_forces(x, v, m, i, non)
         force[0:3, k] = f
@@ -143,7 +148,7 @@ def regcor_case(seed=7, n=800, ni=300, lmax=160, nnbopt=20, ncm=0, kz38=1, nb_fl
     same = rng.random(ni) < frac_same
     for k, i in enumerate(addr0):
         if same[k] and not void[k]:
-            js = rows[k, 1:1 + rows[k, 0]]
+            js = rows[k, 1:1 + rows[k, 0]] + f0
             js = js[js != i]
             lst[i, :] = 0
             lst[i, 0] = len(js)
@@ -174,7 +179,7 @@ def regcor_case(seed=7, n=800, ni=300, lmax=160, nnbopt=20, ncm=0, kz38=1, nb_fl
         S["frdot"][:, i] = force[3:6, k] * (1.0 + eps[k] * rng.normal(size=3))
     P = dict(time=time, dtmin=dtmin, smin=2.0 * dtmin, smax=0.125, etar=0.02, rscale=0.8,
              rc2=0.05, rsfac=12.0, znbmin=0.2 * nnbopt, bodym=1.0 / n, tidal4=tidal4,
-             rdens=[0.01, -0.02, 0.005], n=n - ncm, ifirst=1, nnbopt=nnbopt, nnbmax=nnbmax,
+             rdens=[0.01, -0.02, 0.005], n=n - ncm, ifirst=ifirst, nnbopt=nnbopt, nnbmax=nnbmax,
              nb_flag=nb_flag, m_flag=m_flag, kz38=kz38)
     return S, P, addr0 + 1, force, rows
 

@@ -58,6 +58,7 @@ CASES = [
     dict(seed=11, kz38=1, m_flag=1),
     dict(seed=12, kz38=1, ncm=60, ni=400),
     dict(seed=13, kz38=2, frac_void=0.3, frac_empty_old=0.3, frac_few=0.2),
+    dict(seed=14, kz38=1, ncm=50, ifirst=101, ni=350),
 ]
 
 
