@@ -466,3 +466,42 @@ class RSNet:
 
     def close(self):
         self.L.nb6_rsnet_close()
+
+
+def input_setup(src, dat10=None):
+    """Masses, positions, velocities and driver settings from a NAMELIST parameter file
+    (input.F:18-60; path or text).  Initial conditions: unit 10 in NBODY format for
+    KZ(22) = 2, 6, 9, 10 or TREE format for KZ(22) = 3, 7 (data.F:36-66, `dat10` path or
+    text); otherwise an equal-mass Plummer sphere from this library's generator seeded
+    with NRAND (the reference's own generators are not restated).  The regular-step
+    parameter is sqrt(ETAR): regcor_gpu.F:641-643 takes the square root of ETAR times a
+    squared time scale.  LMAX follows configure.ac (400 up to N = 16k, 500 up to 64k,
+    600 above)."""
+    from pathlib import Path
+
+    from . import ics, io
+    text = Path(src).read_text() if isinstance(src, (str, Path)) and Path(str(src)).exists() \
+        else str(src)
+    p = io.run_parameters(io.read_namelist(text))
+    n = p["n"]
+    if p["kz22"] in (2, 6, 9, 10):
+        if dat10 is None:
+            raise ValueError(f"KZ(22) = {p['kz22']} reads the initial conditions from unit 10")
+        m, x, v = io.read_dat10(dat10, n)
+    elif p["kz22"] in (3, 7):
+        if dat10 is None:
+            raise ValueError(f"KZ(22) = {p['kz22']} reads the TREE format from unit 10")
+        m, x, v = io.read_tree(dat10)
+    else:
+        m, x, v = ics.plummer(n, p["nrand"])
+    lmax = 400 if n <= 16384 else 500 if n <= 65536 else 600
+    kw = dict(nnbopt=p["nnbopt"], lmax=lmax, smax=p["smax"], dtmin=p["dtmin"],
+              eta=math.sqrt(p["etar"]), rs0=p["rs0"])
+    return m, x, v, kw, p
+
+
+def sim_from_input(src, dat10=None, **override):
+    """A Sim set up from a parameter file (and unit 10), see input_setup."""
+    m, x, v, kw, _ = input_setup(src, dat10)
+    kw.update(override)
+    return Sim(m, x, v, **kw)

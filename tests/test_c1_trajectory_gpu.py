@@ -70,3 +70,42 @@ def test_c1_every_regular_call_over_ten_time_units():
     # N = 256); the trajectory only supplies evolving inputs to the regular calls,
     # each of which is checked against the FP64 oracle on its own inputs above.
     assert np.isfinite(e1) and np.isfinite(e10)
+
+
+def test_run_from_the_reference_example_input_file():
+    """examples/example_1k/1k.inp (N = 1000, NNBOPT = 80, ETAR = 0.01, RS0 = 0.15,
+    SMAX = 1, DTMIN = 2.5e-6, KZ(22) = 0) read by nb6.io, set up by input_setup and
+    advanced to t = 0.25 with every regular call checked against the FP64 oracle."""
+    import time
+    from pathlib import Path
+    from nb6.rsdrive import input_setup
+    inp = Path(__file__).resolve().parent / "golden" / "inputs" / "1k.inp"
+    m, x, v, kw, p = input_setup(inp)
+    s = Sim(m, x, v, **kw)
+    stat = {"calls": 0, "lists_equal": True, "acc": [], "pot": 0.0}
+
+    def hook(jx, jv, xi, vi, h2, dtr, out, rows):
+        racc, rjrk, rpot, rlst = O.regf(m, jx, jv, h2, dtr, xi, vi, s.lmax, s.nnbmax)
+        stat["calls"] += 1
+        stat["lists_equal"] &= rows_as_sets(rows) == rows_as_sets(rlst)
+        stat["pot"] = max(stat["pot"], float((np.abs(out[6] - rpot) / rpot).max()))
+        stat["acc"].append(vec_rel_err(out[0:3].T, racc))
+
+    t_wall = time.time()
+    try:
+        s.start()
+        e0 = s.energy()
+        s.hook = hook
+        while s.t < 0.25 and time.time() - t_wall < 90.0:
+            s.block_step("rule")
+        s.hook = None
+        t_end = s.t
+    finally:
+        s.close()
+    ea = np.concatenate(stat["acc"])
+    print(f"1k.inp: N={m.size} NNBOPT={kw['nnbopt']} LMAX={kw['lmax']} to t={t_end:.4f}: "
+          f"{stat['calls']} regular calls, acc p99.9 {np.percentile(ea, 99.9):.2e}, "
+          f"pot max {stat['pot']:.2e}, E0 {e0:.6f}", flush=True)
+    assert stat["calls"] > 0 and stat["lists_equal"]
+    assert np.percentile(ea, 99.9) <= 1e-6 and stat["pot"] <= 1e-6
+    assert abs(e0 + 0.25) < 0.02

@@ -90,3 +90,21 @@ def test_tree_format():
     m, x, v = io.read_tree(txt)
     assert np.array_equal(m, [0.5, 0.25, 0.25]) and np.array_equal(x, np.eye(3))
     assert np.array_equal(v, np.eye(3)[::-1])
+
+
+def test_input_setup_from_reference_file_and_dat10(tmp_path):
+    from nb6 import ics
+    from nb6.rsdrive import input_setup
+    m, x, v, kw, p = input_setup(INP / "1k.inp")
+    assert m.size == 1000 and kw["nnbopt"] == 80 and kw["lmax"] == 400
+    assert kw["dtmin"] == 2.5e-6 and kw["smax"] == 1.0 and kw["rs0"] == 0.15
+    assert np.isclose(kw["eta"], 0.1)
+    assert np.array_equal(m, ics.plummer(1000, p["nrand"])[0])
+    # the same file with KZ(22) = 2 reads the bodies from unit 10
+    text = (INP / "1k.inp").read_text().replace("KZ(21:30)=1 0 2", "KZ(21:30)=1 2 2")
+    mm, xx, vv = ics.plummer(1000, 5)
+    io.write_dat10(tmp_path / "dat.10", mm, xx, vv)
+    m2, x2, v2, kw2, p2 = input_setup(text, tmp_path / "dat.10")
+    assert p2["kz22"] == 2 and np.array_equal(x2, xx) and np.array_equal(m2, mm)
+    with pytest.raises(ValueError):
+        input_setup(text)
