@@ -230,6 +230,11 @@ void nb6_jp_regf_(int *ni, int addr[], double h2[], double dtr[], double acc[][3
  * and whether blocks of >= 2048 active particles are processed in Morton order of
  * their positions (results unchanged).  Defaults: 64, 1. */
 int nb6_set_irr_config(int lanes_per_i, int sort_i);
+/* Prediction of the irregular-force records: 0 = each neighbour record is predicted
+ * where it is gathered (a record and its time per neighbour), 2 = every record is
+ * predicted to ti once per call and the gather reads one 64-byte predicted record per
+ * neighbour, 1 (default) = 2 when ni * 8 >= nmax.  Results are the same bits. */
+int nb6_set_irr_predict(int mode);
 void nb6_irr_sync_jp_(int *n);
 int nb6_irr_set_lists_device(int ni, const int *d_addr, const int *d_list, int lmax, int jbase,
                              void *stream);

@@ -625,6 +625,7 @@ int nb6_irr_set_lists_device(int ni, const int *d_addr, const int *d_list, int l
     return irr_set_lists_device(ni, d_addr, d_list, lmax, jbase, pick(stream));
 }
 int nb6_set_irr_config(int lanes_per_i, int sort_i) { return irr_config(lanes_per_i, sort_i); }
+int nb6_set_irr_predict(int mode) { return irr_predict_mode(mode); }
 int nb6_irr_firr_device(double ti, int ni, const int *d_addr, double *d_out, void *stream) {
     if (!device().ready) return E_NODEV;
     return irr_firr_device(ti, ni, d_addr, d_out, pick(stream));

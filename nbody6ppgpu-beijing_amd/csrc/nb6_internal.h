@@ -192,6 +192,7 @@ int regcor_device(const ::nb6_regcor_params *p, const ::nb6_regstate *s, int ni,
                   const int *d_addr, const double *d_force, int force_stride, int *d_nlist,
                   int *d_counters, hipStream_t stream);
 int irr_config(int lanes_per_i, int sort_i);
+int irr_predict_mode(int mode);
 // the j-set of the open gpunb library (nullptr when closed)
 const JSet *gpunb_jset();
 

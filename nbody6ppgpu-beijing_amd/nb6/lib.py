@@ -79,6 +79,7 @@ def load():
     L.nb6_irr_set_lists_device.argtypes = [i, _vp, _vp, i, i, _vp]
     L.nb6_irr_firr_device.argtypes = [d, i, _vp, _vp, _vp]
     L.nb6_set_irr_config.argtypes = [i, i]
+    L.nb6_set_irr_predict.argtypes = [i]
     L.nb6_regf_autotune_device.argtypes = [i, i, _vp, i, i, i, _vp, _vp, _vp, _vp]
     L.nb6_ens_autotune.argtypes = [_vp, i, i, i, i, _vp, _vp, _vp, _vp]
     L.nb6_rsnet_open.argtypes = [u]

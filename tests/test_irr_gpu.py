@@ -148,3 +148,34 @@ def test_firr_fed_from_device_equals_host_path():
         g.close()
     assert np.array_equal(out[:, 0:3], ha) and np.array_equal(out[:, 3:6], hj)
     assert np.array_equal(out[:, 6].astype(int), hn)
+
+
+@pytest.mark.parametrize("lpi", [64, 16])
+def test_firr_predict_modes_give_the_same_bits(lpi):
+    """Records predicted per gathered neighbour (mode 0) and all records predicted once
+    per call (mode 2) give identical results, and both match FP64."""
+    m, x, v, a2, j6, t0, lists = irr_state(n=2048, seed=12)
+    n = m.size
+    ti = 64.0 / 1024.0
+    L = lib.load()
+    irr = lib.IrrSimd(n, 400)
+    outs = []
+    try:
+        for i in range(n):
+            irr.set_jp(i + 1, x[i], v[i], a2[i], j6[i], m[i], t0[i])
+            irr.set_list(i + 1, lists[i, :1 + lists[i, 0]])
+        addr = np.arange(1, n + 1, 3, dtype=np.int32)
+        assert L.nb6_set_irr_config(lpi, 1) == 0
+        for mode in (0, 2, 1):
+            assert L.nb6_set_irr_predict(mode) == 0
+            outs.append(irr.firr_vec(ti, addr))
+    finally:
+        L.nb6_set_irr_predict(1)
+        L.nb6_set_irr_config(64, 1)
+        irr.close()
+    for o in outs[1:]:
+        for a, b in zip(o, outs[0]):
+            assert np.array_equal(a, b)
+    racc, rjrk, rnn = O.firr(ti, addr, x, v, a2, j6, m, t0, lists, 400)
+    err = np.linalg.norm(outs[0][0] - racc, axis=1) / np.linalg.norm(racc, axis=1)
+    assert err.max() <= REL_TOL and np.array_equal(outs[0][2], rnn)
