@@ -466,7 +466,7 @@ def bench_irr(steps, warmup, no_cpu):
             "config": {"workload": "irr", "n": n, "n_i": ni, "mean_nnb": nnb_tot / ni, "lmax": LMAX},
             "roofline": {"bound": "hbm", "achieved": round(alg / kt / 1e9, 1), "peak": 8000.0,
                          "unit": "GB/s", "frac": round(alg / kt / 8e12, 4),
-                         "algorithmic_bytes": alg, "kernel": "irr_force_kernel",
+                         "algorithmic_bytes": alg, "kernel": "irr_predict_all + irr_force_kernel",
                          "avg_launch_ms": kt * 1e3},
             "check_vs_fp64": check}
 
