@@ -716,6 +716,38 @@ def _rs_accuracy(s, k=128):
             "counts_identical": bool(np.array_equal(rows[:, 0], rlst[:, 0]))}
 
 
+def all_i_vs_fp64_kernel(L, ni, di, do, dl, out, lst):
+    """Every i of the step against the FP64 kernel (nb6_set_regf_precision(2), which
+    matches the CPU oracle to 1e-14): per-particle vector relative errors of acc, jerk
+    and pot, and whether every row (count and neighbours) is identical.  Runs after
+    the timed region."""
+    L.nb6_set_regf_precision(2)
+    try:
+        if L.nb6_regf_device(ni, ni, di.ptr, LMAX, NNBMAX, 0, do.ptr, dl.ptr, None) != 0:
+            lib.check_error()
+        L.nb6_sync()
+    finally:
+        L.nb6_set_regf_precision(0)
+    ref = do.to_host(np.float64, (7, ni))
+    rl = dl.to_host(np.int32, (ni, LMAX))
+
+    def rel(a, b):
+        return np.linalg.norm(np.atleast_2d(a - b), axis=0) / np.linalg.norm(np.atleast_2d(b), axis=0)
+    ea, ej, ep = rel(out[0:3], ref[0:3]), rel(out[3:6], ref[3:6]), rel(out[6], ref[6])
+    same_counts = bool(np.array_equal(lst[:, 0], rl[:, 0]))
+    keep = np.arange(LMAX)[None, :] <= np.maximum(lst[:, :1], 0)
+    same_rows = same_counts and bool(np.array_equal(np.where(keep, lst, 0), np.where(keep, rl, 0)))
+    ovf = lst[:, 0] < 0
+
+    def stats(e):
+        return {"max": float(e.max()), "p99_9": float(np.percentile(e, 99.9)),
+                "p99": float(np.percentile(e, 99)), "median": float(np.median(e)),
+                "n_above_1e-6": int((e > 1e-6).sum())}
+    return {"n_i": int(ni), "acc": stats(ea), "jerk": stats(ej), "pot": stats(ep),
+            "lists_identical": same_rows, "overflow_rows": int(ovf.sum()),
+            "acc_max_overflow_rows": float(ea[ovf].max()) if ovf.any() else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -802,6 +834,7 @@ def main():
 
     out = do.to_host(np.float64, (7, ni))
     lst = dl.to_host(np.int32, (ni, LMAX))
+    full = all_i_vs_fp64_kernel(L, ni, di, do, dl, out, lst)
     for d in (dj, di, do, dl):
         d.free()
     g.close()
@@ -832,7 +865,8 @@ def main():
                       "n_j": nj, "n_i": ni, "lmax": LMAX, "nnbmax": NNBMAX,
                       "pct_fp32_peak": round(100.0 * value * FLOPS_PER_PAIR / 1e12 /
                                              FP32_PEAK_TFLOPS, 2)},
-           "roofline": roof, "cpu_baseline": cpu, "accuracy_vs_fp64": acc}
+           "roofline": roof, "cpu_baseline": cpu, "accuracy_vs_fp64": acc,
+           "all_i_vs_fp64_kernel": full}
     print(json.dumps(rec), flush=True)
     return 0
 

@@ -25,6 +25,12 @@
 //    FP32 sum).  The near field is where the 1/r^3-weighted jerk terms amplify
 //    FP32 rounding.  The test is two-stage: a cheap, conservative FP32 bound
 //    (r2p >= r2 + 2 dtr min(rv, 0)) guards a rarely-taken branch with the full test.
+//    A row whose hard slots fill up overflows; past that point a hard candidate
+//    below the band's lower edge is a neighbour for certain (no force, counted),
+//    and one inside the band takes a soft slot marked ~j so that the FP64 test
+//    still decides it.  Overflowing rows thus keep their exact count and sums, as
+//    the reference computes them; if even the soft slots run out the row's
+//    forces are NaN.
 //
 //  * regf_finalize_kernel: one wavefront per i.  Sums the FP64 chunk partials,
 //    re-tests every hard candidate in FP64 with the reference formula (no FMA
@@ -42,6 +48,7 @@ namespace {
 
 constexpr int RF_FLUSH = 64;          // j per FP32 partial sum
 constexpr int CAP_SOFT = 1024;        // soft (near-field) slots per (i, chunk)
+constexpr int BAND_SLOTS = 16;        // of which kept for band pairs of overflowing rows
 constexpr int NB_SORT_MAX = 2048;     // largest neighbour row the finalize sorts
 
 // rounding-error margins (u = 2^-24)
@@ -80,6 +87,7 @@ struct ILane {
     double x, y, z, u, v, w;         // FP64 position / velocity
     float dt1, dt2;                  // dtr, 2 dtr
     float h2H, h2S;                  // (1+slack) h2 and F_NEAR^2 (1+slack) h2 (MFLAG: slopes)
+    float h2L;                       // (1-slack) h2 (MFLAG: slope)
     float rsH, rsS;                  // radii used in the absolute error bands
     bool ok;
 };
@@ -89,6 +97,8 @@ struct IC {
     float x, y, z, u, v, w;          // i relative to the tile origin
     float dt1, dt2;
     float hH, dH, hS, dS, hQ, dQ;    // thresholds (MFLAG: slopes h*, offsets d*)
+    float hL;                        // below it a pair is an FP64 neighbour for certain
+                                     // (MFLAG: slope, offset -2 dH)
 };
 
 template <bool MFLAG>
@@ -107,6 +117,7 @@ __device__ __forceinline__ ILane make_lane(const double *__restrict__ ib, int is
     const double hk = hh * (1.0 + H_REL) * (1.0 + 4.0 * U24);
     L.h2H = (float)hk;
     L.h2S = (float)(hk * (double)(F_NEAR * F_NEAR));
+    L.h2L = (float)(hh * (1.0 - H_REL) * (1.0 - 12.0 * U24));
     const double rs = MFLAG ? sqrt(hh * mmax) : sqrt(hh);
     L.rsH = (float)(rs * (1.0 + 4.0 * U24));
     L.rsS = F_NEAR * L.rsH;
@@ -131,13 +142,15 @@ __device__ __forceinline__ IC tile_ic(const ILane &L, const double *__restrict__
         c.hH = L.h2H; c.dH = dH;
         c.hS = L.h2S; c.dS = dS;
         c.hQ = L.h2S * (1.0f + Q_REL); c.dQ = dS * (1.0f + Q_REL);
+        c.hL = L.h2L;
     } else {
         c.hH = L.h2H + dH; c.dH = 0.f;
         c.hS = L.h2S + dS; c.dS = 0.f;
         c.hQ = c.hS * (1.0f + Q_REL); c.dQ = 0.f;
+        c.hL = L.h2L - 2.0f * dH;
     }
     if (!L.ok) {
-        c.hH = c.hS = c.hQ = -INFINITY;
+        c.hH = c.hS = c.hQ = c.hL = -INFINITY;
         c.dH = c.dS = c.dQ = -INFINITY;
     }
     return c;
@@ -235,10 +248,23 @@ __device__ __forceinline__ void pair_step(const TileParams &p, const IC &ck, con
                 const float thh = MFLAG ? fmaf(a.w, ck.hH, ck.dH) : ck.hH;
                 const float ths = MFLAG ? fmaf(a.w, ck.hS, ck.dS) : ck.hS;
                 if (mn < thh && jg < j1) {
-                    if (nhk < p.caph) hpk[nhk] = jg;
+                    if (nhk < p.caph) {
+                        hpk[nhk] = jg;
+                    } else {
+                        // hard slots full (the row overflows): a pair below the lower
+                        // band is a neighbour for certain and stays out of every sum;
+                        // one inside the band goes to a soft slot marked ~j, where the
+                        // FP64 test decides it (the last BAND_SLOTS slots are kept for
+                        // these; the count runs on past caps if even they fill up)
+                        const float thl = MFLAG ? fmaf(a.w, ck.hL, -2.0f * ck.dH) : ck.hL;
+                        if (!(fmaf(KAPPA_P, q.r2, mn) < thl)) {
+                            if (nsk < p.caps) spk[nsk] = ~jg;
+                            nsk++;
+                        }
+                    }
                     nhk++;
                     rinv = 0.f;
-                } else if (mn < ths && jg < j1 && nsk < p.caps) {
+                } else if (mn < ths && jg < j1 && nsk < p.caps - BAND_SLOTS) {
                     spk[nsk] = jg;
                     nsk++;
                     rinv = 0.f;
@@ -479,7 +505,8 @@ __global__ __launch_bounds__(256) void regf_var_kernel(TileParams p) {
     const double h2 = p.ib[6 * is + rr], dtr = p.ib[7 * is + rr];
     const size_t row = (size_t)chunk * p.ni + (in ? i : 0);
     int *hp = p.hslot + row * p.caph;
-    int nh = 0;
+    int *sp = p.sslot + row * p.caps;
+    int nh = 0, ns = 0;
     double a64[7] = {0, 0, 0, 0, 0, 0, 0};
     ff f[10];
 #pragma unroll
@@ -547,7 +574,13 @@ __global__ __launch_bounds__(256) void regf_var_kernel(TileParams p) {
                 const double thr = MFLAG ? a.w * h2 : h2;
                 // float-float keeps ~2^-44 relative: 2^-30 covers the decision
                 if (rmin < thr * (1.0 + 0x1p-30) + 0x1p-120 && jg < j1) {
-                    if (nh < p.caph) hp[nh] = jg;
+                    if (nh < p.caph) {
+                        hp[nh] = jg;
+                    } else if (!(rmin < thr * (1.0 - 0x1p-30) - 0x1p-120)) {
+                        // overflowing row, pair inside the band: the FP64 test decides
+                        if (ns < p.caps) sp[ns] = ~jg;
+                        ns++;
+                    }
                     nh++;
                 } else if (d2 > 0.0) {
                     // rinv: v_rsq_f32 refined by one float-float Newton-Raphson step
@@ -590,7 +623,7 @@ __global__ __launch_bounds__(256) void regf_var_kernel(TileParams p) {
 #pragma unroll
         for (int q = 0; q < 7; q++) pp[(size_t)q * p.ni] = a64[q];
         p.cnt[((size_t)chunk * 2 + 0) * p.ni + i] = nh;
-        p.cnt[((size_t)chunk * 2 + 1) * p.ni + i] = 0;
+        p.cnt[((size_t)chunk * 2 + 1) * p.ni + i] = ns;
     }
 }
 
@@ -705,11 +738,15 @@ __global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
 
     int nnb = 0;          // exact neighbours found (wave-uniform)
     int dropped = 0;      // hard candidates beyond the slot capacity
+    int unres = 0;        // band pairs of an overflowing row left without a slot
+    bool dropped_any = false;
     for (int cb = 0; cb < f.nchunk; cb += 64) {
         const int c = cb + lane;
         const int nh = c < f.nchunk ? f.cnt[((size_t)c * 2 + 0) * f.ni + i] : 0;
         const int ns = c < f.nchunk ? f.cnt[((size_t)c * 2 + 1) * f.ni + i] : 0;
-        dropped += wave_sum_i(nh > f.caph ? nh - f.caph : 0);
+        const int over = wave_sum_i(nh > f.caph ? nh - f.caph : 0);
+        dropped += over;
+        dropped_any = dropped_any || over > 0;
         unsigned long long mask = __ballot(nh > 0 || ns > 0);
         while (mask) {
             const int cl = __builtin_ctzll(mask);
@@ -734,11 +771,23 @@ __global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
                 if (isnb && pos < NB_SORT_MAX) buf[pos] = jo;
                 nnb += __popcll(b);
             }
+            // soft slots: near-field pairs (force only) and, marked ~j, band pairs
+            // beyond the hard slots of an overflowing row; those were counted in
+            // dropped and are re-counted by the FP64 test here
             const int *ss = f.sslot + rowc * f.caps;
-            for (int k = lane; k < nsc; k += 64) {
-                const int sj = ss[k];
-                if ((unsigned)sj < (unsigned)f.nj)
-                    classify<MFLAG>(f.rec64 + (size_t)sj * 8, I, false, s);
+            const int nsr = min(nsc, f.caps);
+            unres += nsc - nsr;
+            for (int kb = 0; kb < nsr; kb += 64) {
+                const int k = kb + lane;
+                bool band = false, isnb = false;
+                if (k < nsr) {
+                    const int e = ss[k];
+                    band = e < 0;
+                    const int sj = band ? ~e : e;
+                    if ((unsigned)sj < (unsigned)f.nj)
+                        isnb = classify<MFLAG>(f.rec64 + (size_t)sj * 8, I, band, s);
+                }
+                dropped += __popcll(__ballot(isnb)) - __popcll(__ballot(band));
             }
         }
     }
@@ -747,15 +796,24 @@ __global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
     for (int q = 0; q < 7; q++) s[q] = wave_sum(s[q]);
     int *row = f.list + (size_t)r * f.lmax;
     const int tot = nnb + dropped;
+    // a row whose hard slots overflowed reports -count, as any overflow does, even in
+    // the (never seen) case where band pairs brought the exact count back under nnbmax:
+    // its neighbours past the slots were not kept.  Band pairs left without a slot
+    // would leave the sums incomplete: such a row's forces are NaN.
+    const bool ovf = tot > f.nnbmax || dropped_any;
+    if (unres > 0) {
+#pragma unroll
+        for (int q = 0; q < 7; q++) s[q] = __builtin_nan("");
+    }
     if (lane == 0) {
         double *o = f.out + r;
         const int os = f.out_stride;
         o[0 * os] = s[0]; o[1 * os] = s[1]; o[2 * os] = s[2];
         o[3 * os] = s[3]; o[4 * os] = s[4]; o[5 * os] = s[5];
         o[6 * os] = s[6];
-        row[0] = tot > f.nnbmax ? -tot : tot;
+        row[0] = ovf ? -tot : tot;
     }
-    if (tot <= f.nnbmax && nnb > 0) {
+    if (!ovf && nnb > 0) {
         // ascending row: the rank of each (distinct) index among the row
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();

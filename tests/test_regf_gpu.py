@@ -191,6 +191,85 @@ def test_regf_overflow_exact_negative_counts(gpunb):
     assert (got[3][:, 0] == ref[3][:, 0]).all()
     ok = got[3][:, 0] >= 0
     assert rows_as_sets(got[3][ok]) == rows_as_sets(ref[3][ok])
+    # the reference sums every non-neighbour whatever the list does: so do overflowing rows
+    assert vec_rel_err(got[0], ref[0]).max() <= REL_TOL
+    assert vec_rel_err(got[2][:, None], ref[2][:, None]).max() <= REL_TOL
+
+
+def _crowded_row(seed, n_bg, n_in, n_band, n_shell, rs=0.05, eps=2.0e-7):
+    """Background particles, then around the point c: n_in inside 0.9 RS, n_band at
+    RS (1 +- eps) (alternately inside and outside, all within the FP32 kernel's band
+    around the threshold), n_shell between RS and 2 RS.  dtr = 0, so the FP64
+    decision is r2 < RS^2 and is far from ambiguous at eps = 2e-7."""
+    rng = np.random.default_rng(seed)
+    c = np.array([0.3, 0.2, -0.1])
+
+    def dirs(k):
+        u = rng.normal(size=(k, 3))
+        return u / np.linalg.norm(u, axis=1)[:, None]
+    xb = rng.normal(size=(n_bg, 3)) * 0.6
+    xb = xb[np.linalg.norm(xb - c, axis=1) > 2.5 * rs]
+    xin = c + dirs(n_in) * (0.9 * rs * rng.random(n_in) ** (1 / 3))[:, None]
+    sgn = np.where(np.arange(n_band) % 2 == 0, -1.0, 1.0)
+    xband = c + dirs(n_band) * (rs * (1.0 + sgn * eps))[:, None]
+    xsh = c + dirs(n_shell) * (rs * (1.05 + 0.9 * rng.random(n_shell)))[:, None]
+    x = np.concatenate([xb, xin, xband, xsh])
+    n = len(x)
+    m = np.full(n, 1.0 / n) * rng.uniform(0.5, 1.5, n)
+    v = rng.normal(size=(n, 3)) * 0.3
+    xi = np.stack([c, c + [0.4, -0.3, 0.2]])          # the crowded row and a plain one
+    vi = np.stack([[0.1, 0.0, -0.2], [0.0, 0.1, 0.0]])
+    return m, x, v, xi, vi, np.array([rs * rs, 0.01]), np.zeros(2)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_regf_overflow_band_pairs_past_the_slots(gpunb, mode):
+    """A row with 4x more neighbours than hard slots and 24 pairs inside the FP32
+    kernel's rounding band around RS, most of them met after the slots are full: the
+    band pairs are decided in FP64 (forces of the non-neighbours added, the count
+    exact), as the oracle and the reference, which sum every non-neighbour."""
+    m, x, v, xi, vi, h2, dtr = _crowded_row(31, 3000, 200, 24, 0)
+    lmax, nnbmax = 60, 50
+    g = gpunb(len(m))
+    g.send(m, x, v)
+    L = lib.load()
+    L.nb6_set_regf_config(256, 1, 1)      # one j-chunk: every neighbour meets one slot set
+    L.nb6_set_regf_precision(mode)
+    try:
+        got = g.regf(h2, dtr, xi, vi, lmax, nnbmax)
+    finally:
+        L.nb6_set_regf_precision(0)
+        L.nb6_set_regf_config(0, 1, 2048)
+    assert lib.load().nb6_last_regf_chunks() == 1
+    ref = O.regf(m, x, v, h2, dtr, xi, vi, lmax, nnbmax)
+    assert ref[3][0, 0] < -200 and ref[3][1, 0] >= 0
+    assert (got[3][:, 0] == ref[3][:, 0]).all(), (got[3][:, 0], ref[3][:, 0])
+    tol = REL_TOL if mode == 0 else 1e-12
+    assert vec_rel_err(got[0], ref[0]).max() <= tol
+    assert vec_rel_err(got[2][:, None], ref[2][:, None]).max() <= tol
+    assert vec_rel_err(got[1], ref[1]).max() <= JRK_MAX
+
+
+def test_regf_overflow_band_pairs_without_slots_give_nan(gpunb):
+    """When band pairs of an overflowing row find neither a hard nor a soft slot
+    (the near field has already taken the soft slots), the row cannot be summed
+    exactly: its forces come back NaN and its count stays negative, at least the
+    exact count in magnitude.  The other row is untouched."""
+    m, x, v, xi, vi, h2, dtr = _crowded_row(32, 1000, 200, 400, 3000)
+    lmax, nnbmax = 60, 50
+    g = gpunb(len(m))
+    g.send(m, x, v)
+    L = lib.load()
+    L.nb6_set_regf_config(256, 1, 1)
+    try:
+        got = g.regf(h2, dtr, xi, vi, lmax, nnbmax)
+    finally:
+        L.nb6_set_regf_config(0, 1, 2048)
+    ref = O.regf(m, x, v, h2, dtr, xi, vi, lmax, nnbmax)
+    assert np.isnan(got[0][0]).all() and np.isnan(got[2][0])
+    assert got[3][0, 0] <= ref[3][0, 0] < 0
+    assert got[3][1, 0] == ref[3][1, 0]
+    assert vec_rel_err(got[0][1:], ref[0][1:]).max() <= REL_TOL
 
 
 def test_regf_split_j_many_chunks(gpunb):
