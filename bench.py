@@ -84,14 +84,14 @@ def cpu_baseline(m, x, v, idx, h2, dtr, target_s=15.0):
     rate = probe * m.size / max(time.time() - t, 1e-6)
     ns = int(min(idx.size, max(probe, rate * target_s / m.size)))
     t = time.time()
-    R.regf(h2[:ns], dtr[:ns], x[idx[:ns]], v[idx[:ns]], LMAX, NNBMAX)
+    res = R.regf(h2[:ns], dtr[:ns], x[idx[:ns]], v[idx[:ns]], LMAX, NNBMAX)
     dt = time.time() - t
     R.close()
     return {"value": ns * m.size / dt, "unit": "pair-interactions/s", "cores": threads,
             "kind": "reference",
             "sample": f"src/Main/reg.avx.cpp (g++ -O3 -fopenmp -mavx2 -mfma), {ns} of the "
                       f"{idx.size} i-particles of one step against all {m.size} j, "
-                      f"{dt:.1f} s, OMP_NUM_THREADS={threads}"}
+                      f"{dt:.1f} s, OMP_NUM_THREADS={threads}"}, res
 
 
 def accuracy(m, x, v, idx, h2, dtr, out, lst, k=256):
@@ -716,6 +716,36 @@ def _rs_accuracy(s, k=128):
             "counts_identical": bool(np.array_equal(rows[:, 0], rlst[:, 0]))}
 
 
+def err_stats(e):
+    return {"max": float(e.max()), "p99_9": float(np.percentile(e, 99.9)),
+            "p99": float(np.percentile(e, 99)), "median": float(np.median(e)),
+            "n_above_1e-6": int((e > 1e-6).sum())}
+
+
+def rel_cols(a, b):
+    """Per-column |a - b| / |b| of SoA arrays [k][n]."""
+    a, b = np.atleast_2d(a), np.atleast_2d(b)
+    return np.linalg.norm(a - b, axis=0) / np.linalg.norm(b, axis=0)
+
+
+def same_rows(a, b):
+    """Every row's count and, for non-overflowing rows, its neighbours agree."""
+    if not np.array_equal(a[:, 0], b[:, 0]):
+        return False
+    keep = np.arange(a.shape[1])[None, :] <= np.maximum(a[:, :1], 0)
+    return bool(np.array_equal(np.where(keep, a, 0), np.where(keep, b, 0)))
+
+
+def reference_vs_fp64(res, ref, rl):
+    """The reference CPU back end's output on its sample against the FP64 kernel."""
+    racc, rjrk, rpot, rlst = res
+    k = len(rpot)
+    return {"n_i": int(k), "acc": err_stats(rel_cols(racc.T, ref[0:3, :k])),
+            "jerk": err_stats(rel_cols(rjrk.T, ref[3:6, :k])),
+            "pot": err_stats(rel_cols(rpot, ref[6, :k])),
+            "lists_identical": same_rows(rlst, rl[:k])}
+
+
 def all_i_vs_fp64_kernel(L, ni, di, do, dl, out, lst):
     """Every i of the step against the FP64 kernel (nb6_set_regf_precision(2), which
     matches the CPU oracle to 1e-14): per-particle vector relative errors of acc, jerk
@@ -730,22 +760,12 @@ def all_i_vs_fp64_kernel(L, ni, di, do, dl, out, lst):
         L.nb6_set_regf_precision(0)
     ref = do.to_host(np.float64, (7, ni))
     rl = dl.to_host(np.int32, (ni, LMAX))
-
-    def rel(a, b):
-        return np.linalg.norm(np.atleast_2d(a - b), axis=0) / np.linalg.norm(np.atleast_2d(b), axis=0)
-    ea, ej, ep = rel(out[0:3], ref[0:3]), rel(out[3:6], ref[3:6]), rel(out[6], ref[6])
-    same_counts = bool(np.array_equal(lst[:, 0], rl[:, 0]))
-    keep = np.arange(LMAX)[None, :] <= np.maximum(lst[:, :1], 0)
-    same_rows = same_counts and bool(np.array_equal(np.where(keep, lst, 0), np.where(keep, rl, 0)))
+    ea = rel_cols(out[0:3], ref[0:3])
     ovf = lst[:, 0] < 0
-
-    def stats(e):
-        return {"max": float(e.max()), "p99_9": float(np.percentile(e, 99.9)),
-                "p99": float(np.percentile(e, 99)), "median": float(np.median(e)),
-                "n_above_1e-6": int((e > 1e-6).sum())}
-    return {"n_i": int(ni), "acc": stats(ea), "jerk": stats(ej), "pot": stats(ep),
-            "lists_identical": same_rows, "overflow_rows": int(ovf.sum()),
-            "acc_max_overflow_rows": float(ea[ovf].max()) if ovf.any() else None}
+    return {"n_i": int(ni), "acc": err_stats(ea), "jerk": err_stats(rel_cols(out[3:6], ref[3:6])),
+            "pot": err_stats(rel_cols(out[6], ref[6])), "lists_identical": same_rows(lst, rl),
+            "overflow_rows": int(ovf.sum()),
+            "acc_max_overflow_rows": float(ea[ovf].max()) if ovf.any() else None}, ref, rl
 
 
 def main():
@@ -834,7 +854,7 @@ def main():
 
     out = do.to_host(np.float64, (7, ni))
     lst = dl.to_host(np.int32, (ni, LMAX))
-    full = all_i_vs_fp64_kernel(L, ni, di, do, dl, out, lst)
+    full, ref64, rl64 = all_i_vs_fp64_kernel(L, ni, di, do, dl, out, lst)
     for d in (dj, di, do, dl):
         d.free()
     g.close()
@@ -852,7 +872,10 @@ def main():
     # as the checker of the GPU result on the first 256 i-particles
     cpu, acc = None, None
     if not args.no_cpu:
-        cpu = cpu_baseline(m, x, v, idx, h2, dtr)
+        base = cpu_baseline(m, x, v, idx, h2, dtr)
+        if base is not None:
+            cpu, res = base
+            full["reference_cpu_vs_fp64_kernel"] = reference_vs_fp64(res, ref64, rl64)
         acc = accuracy(m, x, v, idx, h2, dtr, out, lst)
     rec = {"metric": "regf pair-interactions/s at N=1M (%FP32 peak) at <=1e-6 rel err vs FP64",
            "value": value, "unit": "pair-interactions/s", "n_gpus": 1, "steps": args.steps,

@@ -116,27 +116,6 @@ __global__ __launch_bounds__(256) void ens_scatter_kernel(const double *__restri
     perm[ps] = j;
 }
 
-// FP32 tile-relative records and tile origins on the padded layout
-__global__ __launch_bounds__(256) void ens_build_kernel(const double *__restrict__ rec64,
-                                                        int npad, float4 *__restrict__ rec32,
-                                                        double *__restrict__ torg) {
-    const int s = blockIdx.x * 256 + threadIdx.x;
-    if (s >= npad) return;
-    const int t = s / JTILE;
-    const double *p = rec64 + (size_t)t * JTILE * 8;
-    const double *r = rec64 + (size_t)s * 8;
-    rec32[2 * (size_t)s] = make_float4((float)(r[0] - p[0]), (float)(r[1] - p[1]),
-                                       (float)(r[2] - p[2]), (float)r[3]);
-    rec32[2 * (size_t)s + 1] = make_float4((float)(r[4] - p[4]), (float)(r[5] - p[5]),
-                                           (float)(r[6] - p[6]), 0.f);
-    if (s == t * JTILE) {
-        double *q = torg + (size_t)t * 8;
-        q[0] = p[0]; q[1] = p[1]; q[2] = p[2];
-        q[3] = p[4]; q[4] = p[5]; q[5] = p[6];
-        q[6] = 0.0; q[7] = 0.0;
-    }
-}
-
 __global__ __launch_bounds__(256) void ens_soa_to_aos(const double *__restrict__ soa, int n,
                                                       int stride, double *__restrict__ aos) {
     const int j = blockIdx.x * 256 + threadIdx.x;
@@ -324,10 +303,10 @@ int ens_send_device(int nc, const int *offsets, const double *d_soa, int stride,
     hipLaunchKernelGGL(ens_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, s, in64, n,
                        e.d_svals.as<int>(), e.d_cid.as<int>(), e.d_off.as<int>(),
                        e.d_padoff.as<int>(), e.js.rec64.as<double>(), e.js.perm.as<int>());
-    hipLaunchKernelGGL(ens_build_kernel, dim3((np + 255) / 256), dim3(256), 0, s,
-                       e.js.rec64.as<double>(), np, e.js.rec32.as<float4>(), e.js.torg.as<double>());
     NB6_HIP(hipGetLastError());
-    jset_tile_boxes(e.js, np, s);
+    // tile origins and FP32 records on the padded layout: P from a tile's first record
+    // (padding sits at the end of a cluster's last tile)
+    jset_tiles(e.js, np, 0, s);
     e.js.n = np;
     e.js.ntile = np / JTILE;
     // maximum mass, for the m_flag error band

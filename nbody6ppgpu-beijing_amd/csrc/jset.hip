@@ -3,8 +3,11 @@
 // The reference packs j into FP32 absolute coordinates (Jparticle,
 // gpunb.velocity.cu:58-93).  Here the j-set is kept in FP64 and re-ordered
 // along a Morton (Z-order) curve on the device at every send, so that each tile
-// of 256 consecutive j occupies a small region of space.  For every tile t the
-// first particle is the tile origin (P_t, V_t); the regular-force kernel streams
+// of 256 consecutive j occupies a small region of space.  Every tile t has an
+// origin (P_t, V_t): P_t the position of its middle record, V_t the mean velocity
+// of its massive records (the middle record's own velocity would put the orbital
+// velocity of a tight binary member into every relative velocity of the tile);
+// the regular-force kernel streams
 //   rec32[s] = { x - Px, y - Py, z - Pz, m,  vx - Vx, vy - Vy, vz - Vz, 0 }  (FP32)
 // and forms the i side relative to the same origin in FP64, so the FP32
 // separation carries the rounding error of the tile's extent, not of |x|.
@@ -57,31 +60,63 @@ __global__ __launch_bounds__(256) void morton_kernel(const double *__restrict__ 
     val[j] = j;
 }
 
-// sorted position s <- original j = perm[s]; tile origin = the tile's middle record
+// sorted position s <- original j = perm[s]
 __global__ __launch_bounds__(256) void build_kernel(const double *__restrict__ in, int n,
                                                     const int *__restrict__ perm,
-                                                    double *__restrict__ rec64,
-                                                    float4 *__restrict__ rec32,
-                                                    double *__restrict__ torg) {
+                                                    double *__restrict__ rec64) {
     const int s = blockIdx.x * 256 + threadIdx.x;
     if (s >= n) return;
-    const int t = s / JTILE;
-    const int c = min(JTILE, n - t * JTILE);
-    const double *p = in + (size_t)perm[t * JTILE + c / 2] * 8;
     const double *r = in + (size_t)perm[s] * 8;
     double *o = rec64 + (size_t)s * 8;
 #pragma unroll
     for (int f = 0; f < 8; f++) o[f] = r[f];
+}
+
+// One work-group per tile: P = position of record `pos` of the tile (its middle one,
+// or its first), V = mean velocity of the tile's records with m > 0 (that record's
+// velocity if there are none).
+__global__ __launch_bounds__(256) void origin_kernel(const double *__restrict__ rec64, int n,
+                                                     int middle, double *__restrict__ torg) {
+    __shared__ double sh[4][4];
+    const int t = blockIdx.x;
+    const int c = min(JTILE, n - t * JTILE);
+    const int k = threadIdx.x;
+    const double *r = rec64 + ((size_t)t * JTILE + k) * 8;
+    const bool use = k < c && r[3] > 0.0;
+    double q[4] = {use ? r[4] : 0.0, use ? r[5] : 0.0, use ? r[6] : 0.0, use ? 1.0 : 0.0};
+#pragma unroll
+    for (int f = 0; f < 4; f++)
+        for (int o = 32; o > 0; o >>= 1) q[f] += __shfl_xor(q[f], o, 64);
+    if ((k & 63) == 0)
+        for (int f = 0; f < 4; f++) sh[k >> 6][f] = q[f];
+    __syncthreads();
+    if (k == 0) {
+        double v[4];
+        for (int f = 0; f < 4; f++) v[f] = (sh[0][f] + sh[1][f]) + (sh[2][f] + sh[3][f]);
+        const double *p = rec64 + ((size_t)t * JTILE + (middle ? c / 2 : 0)) * 8;
+        double *o = torg + (size_t)t * 8;
+        o[0] = p[0]; o[1] = p[1]; o[2] = p[2];
+        if (v[3] > 0.0) {
+            o[3] = v[0] / v[3]; o[4] = v[1] / v[3]; o[5] = v[2] / v[3];
+        } else {
+            o[3] = p[4]; o[4] = p[5]; o[5] = p[6];
+        }
+        o[6] = 0.0; o[7] = 0.0;
+    }
+}
+
+// FP32 records relative to their tile's origin
+__global__ __launch_bounds__(256) void rec32_kernel(const double *__restrict__ rec64, int n,
+                                                    const double *__restrict__ torg,
+                                                    float4 *__restrict__ rec32) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= n) return;
+    const double *p = torg + (size_t)(s / JTILE) * 8;
+    const double *r = rec64 + (size_t)s * 8;
     rec32[2 * (size_t)s] = make_float4((float)(r[0] - p[0]), (float)(r[1] - p[1]),
                                        (float)(r[2] - p[2]), (float)r[3]);
-    rec32[2 * (size_t)s + 1] = make_float4((float)(r[4] - p[4]), (float)(r[5] - p[5]),
-                                           (float)(r[6] - p[6]), 0.f);
-    if (s == t * JTILE) {
-        double *q = torg + (size_t)t * 8;
-        q[0] = p[0]; q[1] = p[1]; q[2] = p[2];
-        q[3] = p[4]; q[4] = p[5]; q[5] = p[6];
-        q[6] = 0.0; q[7] = 0.0;
-    }
+    rec32[2 * (size_t)s + 1] = make_float4((float)(r[4] - p[3]), (float)(r[5] - p[4]),
+                                           (float)(r[6] - p[5]), 0.f);
 }
 
 // SoA [7][stride] (x y z vx vy vz m) -> AoS [n][8] (x y z m vx vy vz 0)
@@ -290,8 +325,17 @@ void jset_build(JSet &js, int n, const double *box, double mmax, hipStream_t s) 
     void *tmp = js.sort_tmp.get(tb);
     NB6_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, v1, n, 0, 63, s));
     hipLaunchKernelGGL(build_kernel, dim3((n + 255) / 256), dim3(256), 0, s, js.in64.as<double>(), n,
-                       js.perm.as<int>(), js.rec64.as<double>(), js.rec32.as<float4>(),
+                       js.perm.as<int>(), js.rec64.as<double>());
+    NB6_HIP(hipGetLastError());
+    jset_tiles(js, n, 1, s);
+}
+
+void jset_tiles(JSet &js, int n, int middle, hipStream_t s) {
+    const int nt = (n + JTILE - 1) / JTILE;
+    hipLaunchKernelGGL(origin_kernel, dim3(nt), dim3(256), 0, s, js.rec64.as<double>(), n, middle,
                        js.torg.as<double>());
+    hipLaunchKernelGGL(rec32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, js.rec64.as<double>(),
+                       n, js.torg.as<double>(), js.rec32.as<float4>());
     NB6_HIP(hipGetLastError());
     jset_tile_boxes(js, n, s);
 }

@@ -87,6 +87,9 @@ struct JSet {
 };
 // per-tile bounds of rec32 (after rec32 is built): one work-group per tile
 void jset_tile_boxes(JSet &js, int n, hipStream_t s);
+// tile origins (P from the middle record, or the first: middle = 0), FP32 records,
+// tile boxes, from the sorted rec64 of n records
+void jset_tiles(JSet &js, int n, int middle, hipStream_t s);
 void jset_reserve(JSet &js, int cap);
 void jset_release(JSet &js);
 // in64 already holds n records; box = {lo x,y,z, hi x,y,z} or nullptr (computed)
