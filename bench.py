@@ -180,7 +180,7 @@ def bench_ensemble(steps, warmup):
     out_b = do.to_host(np.float64, (7, ntot))
     lst_b = dl.to_host(np.int32, (ntot, lmax))
     L.nb6_ens_close()
-    L.nb6_set_regf_config(0, 1, 2048)      # the looped path runs the default configuration
+    L.nb6_set_regf_config(0, 1, 4096)      # the looped path runs the default configuration
 
     # looped: the plain path, one cluster after the other
     g = lib.Gpunb(int(sizes.max()))

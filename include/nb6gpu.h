@@ -132,7 +132,7 @@ int nb6_pot_device(int n, int n_stride, const double *d_x, int i0, int ni,
 /* Tile configuration of the regular-force kernel: work-group size = LDS j-tile
  * (128, 256 or 512; 0 = chosen per call: 512 for ni >= 65536, else 256),
  * i-particles per lane (1 or 2), and the minimum number of work-groups a call is
- * split into (j-chunks are added until it is reached).  Default: 0, 1, 2048. */
+ * split into (j-chunks are added until it is reached).  Default: 0, 1, 4096. */
 int nb6_set_regf_config(int threads, int i_per_lane, int min_groups);
 
 /* Ordering and pruning of the regular-force kernel: i-sets of at least isort_min

@@ -258,14 +258,6 @@ __device__ __forceinline__ double gsum(double v) {
     return v;
 }
 
-// work-group b of G -> position in the launch order such that XCD x (= b mod 8)
-// receives a contiguous range of work
-__device__ __forceinline__ int xcd_order(int b, int G) {
-    const int xcd = b & 7, k = b >> 3;
-    const int per = G >> 3, rem = G & 7;
-    return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
-}
-
 template <int LPI, bool PRE>
 __global__ __launch_bounds__(256) void irr_force_kernel(
     double ti, int ni, const int *__restrict__ addr /* in processing order */,

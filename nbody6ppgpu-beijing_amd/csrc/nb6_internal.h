@@ -86,6 +86,15 @@ struct JSet {
     DBuf in64, rec64, rec32, torg, perm, keys, vals, stat, sort_tmp, tbox;
 };
 // per-tile bounds of rec32 (after rec32 is built): one work-group per tile
+// Work-group b of G -> position in the work order such that XCD x (work-groups are
+// dealt to the 8 XCDs round robin, b mod 8) receives a contiguous range of the work,
+// and its L2 sees one region of it at a time.
+__device__ __forceinline__ int xcd_order(int b, int G) {
+    const int xcd = b & 7, k = b >> 3;
+    const int per = G >> 3, rem = G & 7;
+    return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
+}
+
 void jset_tile_boxes(JSet &js, int n, hipStream_t s);
 // tile origins (P from the middle record, or the first: middle = 0), FP32 records,
 // tile boxes, from the sorted rec64 of n records
@@ -99,7 +108,7 @@ void jset_from_soa(JSet &js, int n, int stride, const double *d_soa, hipStream_t
 struct RegfConfig {
     int threads = 0;      // work-group size = LDS j-tile (128, 256 or 512; 0: by ni)
     int ipl = 1;          // i-particles per lane (1 or 2)
-    int min_groups = 2048;
+    int min_groups = 4096;
     int isort_min = 2048; // i-sets this large are Morton-ordered (0: never)
     int far_test = 1;     // skip the pair test on tiles far from the whole wavefront
     int prec = 0;         // pair arithmetic: 0 FP32, 1 float-float, 2 FP64 (study variants)

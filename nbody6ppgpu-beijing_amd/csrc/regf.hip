@@ -73,6 +73,7 @@ struct TileParams {
     const double *__restrict__ rec64;   // sorted AoS [nj][8] (precision variants)
     int nj;
     int chunk_len;                   // multiple of the block size T and of JTILE
+    int nib;                         // i-blocks of the grid (regf_tile_kernel)
     double mmax;
     double *__restrict__ part;       // [nchunk][7][ni]
     int *__restrict__ cnt;           // [nchunk][2][ni]: hard, soft
@@ -419,10 +420,14 @@ template <int T, int IPL, bool MFLAG>
 __global__ __launch_bounds__(T, (regf_waves<T, IPL>())) void regf_tile_kernel(TileParams p) {
     __shared__ float4 sA[2][T];   // x', y', z', m
     __shared__ float4 sB[2][T];   // vx', vy', vz', 0
-    const int chunk = blockIdx.y;
+    // (chunk, i-block) items chunk-major, each XCD taking a contiguous run: the
+    // work-groups resident on one XCD stream the same j-chunk through its L2
+    const int w = xcd_order(blockIdx.x, gridDim.x);
+    const int chunk = w / p.nib;
+    const int ib = w - chunk * p.nib;
     const int j0 = chunk * p.chunk_len;
-    tile_work<T, IPL, MFLAG>(p, blockIdx.x * (T * IPL), p.ni, j0, min(j0 + p.chunk_len, p.nj),
-                             chunk, sA, sB);
+    tile_work<T, IPL, MFLAG>(p, ib * (T * IPL), p.ni, j0, min(j0 + p.chunk_len, p.nj), chunk, sA,
+                             sB);
 }
 
 // Persistent form for a batch of independent clusters: each work-group takes the
@@ -852,6 +857,7 @@ void launch_items_t(bool mflag, int grid, const TileParams &tp, const int4 *item
 // dispatch on the tile configuration (block size T in {128, 256, 512}, IPL in {1, 2})
 void launch_tile(const RegfConfig &cfg, bool mflag, dim3 grid, const TileParams &tp, hipStream_t s) {
     if (cfg.prec == 1 || cfg.prec == 2) {
+        grid = dim3(tp.nib, grid.x / tp.nib);   // the variant kernels take (i-block, chunk)
         if (cfg.prec == 1) {
             if (mflag) hipLaunchKernelGGL((regf_var_kernel<1, true>), grid, dim3(256), 0, s, tp);
             else hipLaunchKernelGGL((regf_var_kernel<1, false>), grid, dim3(256), 0, s, tp);
@@ -951,10 +957,11 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
     tp.far_test = cfg.far_test;
     tp.rec64 = js.rec64.as<double>();
     tp.chunk_len = clen;
+    tp.nib = nib;
     tp.mmax = js.mmax;
     tp.part = part; tp.cnt = cnt;
     tp.hslot = hslot; tp.caph = caph; tp.sslot = sslot; tp.caps = caps;
-    dim3 grid(nib, nchunk);
+    dim3 grid(nib * nchunk);
     KTimer &t1 = ktimer("regf_tile");
     t1.start(s);
     launch_tile(cfg, m_flag != 0, grid, tp, s);
@@ -1051,11 +1058,11 @@ int regf_autotune(const JSet &js, int ni, int ni_stride, const double *d_i, int 
     const RegfConfig keep = cfg;
     if (cfg.prec != 0) { set_error(E_STATE, "nb6_regf_autotune: set the FP32 kernel first"); return E_STATE; }
     float best_ms = 1e30f;
-    int bt = 256, bipl = 1, bmg = 2048;
+    int bt = 256, bipl = 1, bmg = 4096;
     const int threads[3] = {128, 256, 512};
     for (int ti = 0; ti < 3; ti++)
         for (int ipl = 1; ipl <= 2; ipl++)
-            for (int mg : {1024, 2048, 4096}) {
+            for (int mg : {2048, 4096, 8192}) {
                 cfg.threads = threads[ti];
                 cfg.ipl = ipl;
                 cfg.min_groups = mg;

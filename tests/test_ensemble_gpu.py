@@ -99,7 +99,7 @@ def test_ensemble_autotune_keeps_results():
         lst = dl.to_host(np.int32, (ntot, 150))
     finally:
         L.nb6_ens_close()
-        L.nb6_set_regf_config(0, 1, 2048)
+        L.nb6_set_regf_config(0, 1, 4096)
         for d in (dj, di, do, dl):
             d.free()
     print("autotuned", list(best))

@@ -64,7 +64,7 @@ def test_regf_plummer_1k_all_i(gpunb, threads, ipl):
     g.send(m, x, v)
     got = g.regf(h2, dtr, x, v, lmax, nnbmax)
     ref = O.regf(m, x, v, h2, dtr, x, v, lmax, nnbmax)
-    lib.load().nb6_set_regf_config(0, 1, 2048)
+    lib.load().nb6_set_regf_config(0, 1, 4096)
     _compare(got, ref, f"C1 threads={threads} ipl={ipl}")
     # the self pair is always a neighbour (r2 = 0 < h2)
     for i in range(n):
@@ -113,7 +113,7 @@ def test_regf_fast_movers_and_outsiders(gpunb, threads, ipl, m_flag):
         g.send(m, x, v)
         got = g.regf(h2, dtr, xi, vi, lmax, nnbmax, m_flag)
     finally:
-        lib.load().nb6_set_regf_config(0, 1, 2048)
+        lib.load().nb6_set_regf_config(0, 1, 4096)
     ref = O.regf(m, x, v, h2, dtr, xi, vi, lmax, nnbmax, m_flag)
     # the velocity criterion must actually matter in this case
     ref0 = O.regf(m, x, v, h2, np.zeros_like(dtr), xi, vi, lmax, nnbmax, m_flag)
@@ -168,11 +168,11 @@ def test_regf_autotune_keeps_results(gpunb):
                                           None) == 0
         got = g.regf(h2, dtr, x[idx], v[idx], lmax, nnbmax)
     finally:
-        L.nb6_set_regf_config(0, 1, 2048)
+        L.nb6_set_regf_config(0, 1, 4096)
         for d in (di, do, dl):
             d.free()
     print("autotuned", list(best))
-    assert best[0] in (128, 256, 512) and best[1] in (1, 2) and best[2] in (1024, 2048, 4096)
+    assert best[0] in (128, 256, 512) and best[1] in (1, 2) and best[2] in (2048, 4096, 8192)
     ref = O.regf(m, x, v, h2, dtr, x[idx], v[idx], lmax, nnbmax)
     _compare(got, ref, "autotuned")
 
@@ -239,7 +239,7 @@ def test_regf_overflow_band_pairs_past_the_slots(gpunb, mode):
         got = g.regf(h2, dtr, xi, vi, lmax, nnbmax)
     finally:
         L.nb6_set_regf_precision(0)
-        L.nb6_set_regf_config(0, 1, 2048)
+        L.nb6_set_regf_config(0, 1, 4096)
     assert lib.load().nb6_last_regf_chunks() == 1
     ref = O.regf(m, x, v, h2, dtr, xi, vi, lmax, nnbmax)
     assert ref[3][0, 0] < -200 and ref[3][1, 0] >= 0
@@ -264,7 +264,7 @@ def test_regf_overflow_band_pairs_without_slots_give_nan(gpunb):
     try:
         got = g.regf(h2, dtr, xi, vi, lmax, nnbmax)
     finally:
-        L.nb6_set_regf_config(0, 1, 2048)
+        L.nb6_set_regf_config(0, 1, 4096)
     ref = O.regf(m, x, v, h2, dtr, xi, vi, lmax, nnbmax)
     assert np.isnan(got[0][0]).all() and np.isnan(got[2][0])
     assert got[3][0, 0] <= ref[3][0, 0] < 0
