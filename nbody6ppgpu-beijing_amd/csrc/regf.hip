@@ -47,6 +47,7 @@ namespace nb6 {
 namespace {
 
 constexpr int RF_FLUSH = 64;          // j per FP32 partial sum
+constexpr int NACC = 7;               // FP32 partials: acc x y z, pot, jerk x y z
 constexpr int CAP_SOFT = 1024;        // soft (near-field) slots per (i, chunk)
 constexpr int BAND_SLOTS = 16;        // of which kept for band pairs of overflowing rows
 constexpr int NB_SORT_MAX = 2048;     // largest neighbour row the finalize sorts
@@ -272,20 +273,20 @@ __device__ __forceinline__ void pair_step(const TileParams &p, const IC &ck, con
                 }
             }
         }
+        // jerk term m/r^3 (dv - 3 (dx.dv / r^2) dx) summed whole: its two parts partly
+        // cancel within a pair, so the FP32 partial sums stay smaller than J1 and 3 J2
+        // would separately
         const float rinv2 = rinv * rinv;
         const float mr1 = a.w * rinv;
         const float mr3 = mr1 * rinv2;
-        const float be = (q.rv * rinv2) * mr3;
+        const float al = q.rv * (-3.0f * rinv2);
         acc[0] = fmaf(mr3, q.dx, acc[0]);
         acc[1] = fmaf(mr3, q.dy, acc[1]);
         acc[2] = fmaf(mr3, q.dz, acc[2]);
         acc[3] += mr1;
-        acc[4] = fmaf(mr3, q.du, acc[4]);
-        acc[5] = fmaf(mr3, q.dv, acc[5]);
-        acc[6] = fmaf(mr3, q.dw, acc[6]);
-        acc[7] = fmaf(be, q.dx, acc[7]);
-        acc[8] = fmaf(be, q.dy, acc[8]);
-        acc[9] = fmaf(be, q.dz, acc[9]);
+        acc[4] = fmaf(mr3, fmaf(al, q.dx, q.du), acc[4]);
+        acc[5] = fmaf(mr3, fmaf(al, q.dy, q.dv), acc[5]);
+        acc[6] = fmaf(mr3, fmaf(al, q.dz, q.dw), acc[6]);
     }
 }
 
@@ -299,7 +300,7 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
     ILane L[IPL];
     int nh[IPL], ns[IPL];
     int *hp[IPL], *sp[IPL];
-    float a32[IPL][10];
+    float a32[IPL][NACC];
     double a64[IPL][7];
 
 #pragma unroll
@@ -316,7 +317,7 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
         hp[k] = p.hslot + row * p.caph;
         sp[k] = p.sslot + row * p.caps;
 #pragma unroll
-        for (int f = 0; f < 10; f++) a32[k][f] = 0.f;
+        for (int f = 0; f < NACC; f++) a32[k][f] = 0.f;
 #pragma unroll
         for (int f = 0; f < 7; f++) a64[k][f] = 0.0;
     }
@@ -377,18 +378,18 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
                                                sp[k], a32[k]);
                 }
             }
-            // flush FP32 partials into FP64 (jerk = J1 - 3 J2)
+            // flush FP32 partials into FP64
 #pragma unroll
             for (int k = 0; k < IPL; k++) {
                 a64[k][0] += (double)a32[k][0];
                 a64[k][1] += (double)a32[k][1];
                 a64[k][2] += (double)a32[k][2];
-                a64[k][3] += (double)a32[k][4] - 3.0 * (double)a32[k][7];
-                a64[k][4] += (double)a32[k][5] - 3.0 * (double)a32[k][8];
-                a64[k][5] += (double)a32[k][6] - 3.0 * (double)a32[k][9];
+                a64[k][3] += (double)a32[k][4];
+                a64[k][4] += (double)a32[k][5];
+                a64[k][5] += (double)a32[k][6];
                 a64[k][6] += (double)a32[k][3];
 #pragma unroll
-                for (int f = 0; f < 10; f++) a32[k][f] = 0.f;
+                for (int f = 0; f < NACC; f++) a32[k][f] = 0.f;
             }
         }
 
