@@ -274,8 +274,8 @@ __device__ __forceinline__ void pair_step(const TileParams &p, const IC &ck, con
             }
         }
         // jerk term m/r^3 (dv - 3 (dx.dv / r^2) dx) summed whole: its two parts partly
-        // cancel within a pair, so the FP32 partial sums stay smaller than J1 and 3 J2
-        // would separately
+        // cancel within a pair, so the FP32 partial sum stays smaller than the sums of
+        // m dv / r^3 and of 3 m (dx.dv) dx / r^5 would be apart
         const float rinv2 = rinv * rinv;
         const float mr1 = a.w * rinv;
         const float mr3 = mr1 * rinv2;
