@@ -615,6 +615,100 @@ def bench_regcor(steps, warmup, no_cpu):
             "cpu_baseline": cpu, "check_vs_oracle": check}
 
 
+def bench_c2(steps, warmup, no_cpu):
+    """Single regular-force calls at N = 65,536 (Plummer, Kroupa masses, all N as j) for
+    i-sets of 1, 256, 4096 and 65,536 particles with log-normal neighbour counts
+    (median 80, cap 500), lmax 500 and nnbmax 450 (configure.ac / input.F for 64k).
+    Per size: the device-resident call (nb6_regf_device, inputs in HBM) and the host ABI
+    call gpunb_regf_ (host arrays in and out), each timed over `steps` calls after
+    `warmup`; the kernel times; every particle against the FP64 kernel; and the
+    reference's reg.avx.cpp on the host cores for the same call."""
+    n, lmax, nnbmax = 65536, 500, 450
+    seed = SEED + 200
+    mass = ics.kroupa_masses(n, seed)
+    m, x, v = ics.plummer(n, seed + 1, masses=mass)
+    rng = np.random.Generator(np.random.PCG64(seed + 2))
+    L = lib.load()
+    g = lib.Gpunb(n)
+    g.send(m, x, v)
+    ref, cores = None, None
+    if not no_cpu:
+        import oracle as O
+        if O.ref_available():
+            ref = O.RefReg(n)
+            ref.send(m, x, v)
+            cores = O.num_threads()
+    rows = []
+    for ni in (1, 256, 4096, 65536):
+        idx = np.sort(rng.choice(n, size=ni, replace=False))
+        counts = ics.lognormal_counts(ni, seed + 3 + ni, median=80.0)
+        rs = ics.rs_for_counts(x, idx, counts)
+        h2 = rs * rs
+        dtr = np.exp(rng.uniform(np.log(2.0 ** -10), np.log(2.0 ** -4), ni))
+        ib = np.ascontiguousarray(np.concatenate([x[idx].T, v[idx].T, h2[None], dtr[None]]))
+        di = lib.DeviceArray.from_host(ib)
+        do = lib.DeviceArray(7 * ni * 8)
+        dl = lib.DeviceArray(ni * lmax * 4)
+
+        def dev_call():
+            if L.nb6_regf_device(ni, ni, di.ptr, lmax, nnbmax, 0, do.ptr, dl.ptr, None) != 0:
+                lib.check_error()
+
+        for _ in range(warmup):
+            dev_call()
+        L.nb6_sync()
+        kt, kf = [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            dev_call()
+            kt.append(L.nb6_kernel_ms(b"regf_tile"))
+            kf.append(L.nb6_kernel_ms(b"regf_finalize"))
+        L.nb6_sync()
+        dev_ms = (time.perf_counter() - t0) * 1e3 / steps
+        out = do.to_host(np.float64, (7, ni))
+        lst = dl.to_host(np.int32, (ni, lmax))
+        full, ref64, _ = all_i_vs_fp64_kernel(L, ni, di, do, dl, out, lst, lmax, nnbmax)
+        xi, vi = x[idx], v[idx]
+        for _ in range(warmup):
+            g.regf(h2, dtr, xi, vi, lmax, nnbmax)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.regf(h2, dtr, xi, vi, lmax, nnbmax)
+        abi_ms = (time.perf_counter() - t0) * 1e3 / steps
+        row = {"ni": ni, "pairs": ni * n, "device_call_ms": dev_ms,
+               "tile_kernel_ms": float(np.median(kt)), "finalize_kernel_ms": float(np.median(kf)),
+               "j_chunks": L.nb6_last_regf_chunks(), "host_abi_call_ms": abi_ms,
+               "pairs_per_s_device": ni * n / (dev_ms * 1e-3),
+               "pairs_per_s_host_abi": ni * n / (abi_ms * 1e-3),
+               "acc_max_vs_fp64": full["acc"]["max"], "jerk_p99_vs_fp64": full["jerk"]["p99"],
+               "pot_max_vs_fp64": full["pot"]["max"], "lists_identical": full["lists_identical"]}
+        if ref is not None:
+            reps = max(1, min(steps, int(2e9 // (ni * n))))
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                res = ref.regf(h2, dtr, xi, vi, lmax, nnbmax)
+            row["reference_cpu_call_ms"] = (time.perf_counter() - t0) * 1e3 / reps
+            row["reference_cpu_acc_max_vs_fp64"] = float(rel_cols(res[0].T, ref64[0:3]).max())
+            row["reference_cpu_pot_max_vs_fp64"] = float(rel_cols(res[2], ref64[6]).max())
+        log(f"c2 ni={ni}: device {dev_ms:.3f} ms (tile {row['tile_kernel_ms']:.3f}, finalize "
+            f"{row['finalize_kernel_ms']:.3f}, {row['j_chunks']} chunks), host ABI "
+            f"{abi_ms:.3f} ms, acc {row['acc_max_vs_fp64']:.1e}")
+        rows.append(row)
+        for d in (di, do, dl):
+            d.free()
+    g.close()
+    if ref is not None:
+        ref.close()
+    return {"metric": "gpunb_regf call time and pair-interactions/s at N=65536 by i-set size",
+            "value": rows[-1]["pairs_per_s_device"], "unit": "pair-interactions/s", "n_gpus": 1,
+            "steps": steps, "warmup": warmup, "higher_is_better": True, "dtype": "fp32",
+            "data": "synthetic (seeded Plummer, Kroupa masses)",
+            "config": {"workload": "c2: one gpunb_regf call, N=65536, ni in 1/256/4096/65536",
+                       "n_j": n, "lmax": lmax, "nnbmax": nnbmax,
+                       "cpu_cores": cores},
+            "calls": rows}
+
+
 def bench_energy(steps, warmup, no_cpu):
     """Energy path of the GPU build (energy.F:28-62: GPUPOT over all bodies, PHICOR,
     POT/ZKIN sums) through nb6_energy_device on the N = 1M King W0 = 6 system with its
@@ -746,20 +840,20 @@ def reference_vs_fp64(res, ref, rl):
             "lists_identical": same_rows(rlst, rl[:k])}
 
 
-def all_i_vs_fp64_kernel(L, ni, di, do, dl, out, lst):
+def all_i_vs_fp64_kernel(L, ni, di, do, dl, out, lst, lmax=LMAX, nnbmax=NNBMAX):
     """Every i of the step against the FP64 kernel (nb6_set_regf_precision(2), which
     matches the CPU oracle to 1e-14): per-particle vector relative errors of acc, jerk
     and pot, and whether every row (count and neighbours) is identical.  Runs after
     the timed region."""
     L.nb6_set_regf_precision(2)
     try:
-        if L.nb6_regf_device(ni, ni, di.ptr, LMAX, NNBMAX, 0, do.ptr, dl.ptr, None) != 0:
+        if L.nb6_regf_device(ni, ni, di.ptr, lmax, nnbmax, 0, do.ptr, dl.ptr, None) != 0:
             lib.check_error()
         L.nb6_sync()
     finally:
         L.nb6_set_regf_precision(0)
     ref = do.to_host(np.float64, (7, ni))
-    rl = dl.to_host(np.int32, (ni, LMAX))
+    rl = dl.to_host(np.int32, (ni, lmax))
     ea = rel_cols(out[0:3], ref[0:3])
     ovf = lst[:, 0] < 0
     return {"n_i": int(ni), "acc": err_stats(ea), "jerk": err_stats(rel_cols(out[3:6], ref[3:6])),
@@ -776,7 +870,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--workload", default="regf",
                     choices=["regf", "ensemble", "learned-rs", "precision", "irr", "regcor",
-                             "energy"],
+                             "energy", "c2"],
                     help="regf: the headline N=1M step; ensemble: 256-cluster batch vs loop; "
                          "learned-rs: overflow retries of the RS network vs the reference rule")
     ap.add_argument("--rs-train-samples", type=int, default=10_000_000)
@@ -806,6 +900,9 @@ def main():
         return 0
     if args.workload == "regcor":
         print(json.dumps(bench_regcor(args.steps, args.warmup, args.no_cpu)), flush=True)
+        return 0
+    if args.workload == "c2":
+        print(json.dumps(bench_c2(args.steps, args.warmup, args.no_cpu)), flush=True)
         return 0
     if args.workload == "energy":
         print(json.dumps(bench_energy(args.steps, args.warmup, args.no_cpu)), flush=True)

@@ -928,15 +928,17 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
     const int nj = js.n;
     const int unit = std::max(cfg.threads, JTILE);       // chunk granularity (j)
     const int nunit = (nj + unit - 1) / unit;
-    int want = (cfg.min_groups + nib - 1) / nib;
-    int maxc = std::max(1, nunit * unit / (4 * JTILE));
-    int nchunk = std::max(1, std::min(want, maxc));
+    // small i-sets get chunks down to one unit each, so that even ni = 1 spreads over
+    // the whole GPU
+    const int want = (cfg.min_groups + nib - 1) / nib;
+    int nchunk = std::max(1, std::min(want, nunit));
     const int upc = (nunit + nchunk - 1) / nchunk;        // units per chunk
     const int clen = upc * unit;
     nchunk = (nunit + upc - 1) / upc;
     const int nnbeff = std::max(0, std::min(nnbmax, lmax - 1));
-    const int caph = nnbeff + 8;
-    const int caps = CAP_SOFT;
+    // a chunk cannot hold more candidates than records
+    const int caph = std::min(nnbeff + 8, clen);
+    const int caps = std::min(CAP_SOFT, clen);
 
     // Morton-ordered i: a wavefront's particles are neighbours in space, so their
     // candidates fall on the same j and most tiles are far from all of them
