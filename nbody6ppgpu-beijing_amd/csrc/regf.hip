@@ -32,7 +32,8 @@
 //    the reference computes them; if even the soft slots run out the row's
 //    forces are NaN.
 //
-//  * regf_finalize_kernel: one wavefront per i.  Sums the FP64 chunk partials,
+//  * regf_finalize_kernel: one wavefront per i (four, regf_finalize_wide_kernel, when
+//    there are 32 chunks or more, or 16 and at most 16384 i).  Sums the FP64 chunk partials,
 //    re-tests every hard candidate in FP64 with the reference formula (no FMA
 //    contraction), adds the FP64 force of every non-neighbour candidate, maps
 //    neighbours back to the caller's indices and writes each row in ascending
@@ -51,6 +52,7 @@ constexpr int NACC = 7;               // FP32 partials: acc x y z, pot, jerk x y
 constexpr int CAP_SOFT = 1024;        // soft (near-field) slots per (i, chunk)
 constexpr int BAND_SLOTS = 16;        // of which kept for band pairs of overflowing rows
 constexpr int NB_SORT_MAX = 2048;     // largest neighbour row the finalize sorts
+constexpr int FIN_WIDE_NI = 16384;    // i-set size below which 16 chunks take the wide finalize
 
 // rounding-error margins (u = 2^-24)
 constexpr double U24 = 5.9604644775390625e-08;
@@ -713,6 +715,75 @@ __device__ __forceinline__ bool classify(const double *__restrict__ rec, const I
     return isnb;
 }
 
+// The FP64 pass over one i's candidates, in the chunks c = c0 + cstep (64 p + lane),
+// p = 0, 1, ...: every hard candidate is re-tested (neighbours into buf), every
+// non-neighbour candidate's force is added to s.  SHARED: several wavefronts fill one
+// row, taking buffer positions from *nshared; otherwise this wavefront's nnb counts them.
+template <bool MFLAG, bool SHARED>
+__device__ __forceinline__ void fin_candidates(const FinParams &f, const IF64 &I, int i, int jb,
+                                               int c0, int cstep, int lane, double *s, int *buf,
+                                               int *nshared, int &nnb, int &dropped, int &unres,
+                                               bool &dropped_any) {
+    for (int cg = 0; c0 + cstep * cg < f.nchunk; cg += 64) {
+        const int c = c0 + cstep * (cg + lane);
+        const int nh = c < f.nchunk ? f.cnt[((size_t)c * 2 + 0) * f.ni + i] : 0;
+        const int ns = c < f.nchunk ? f.cnt[((size_t)c * 2 + 1) * f.ni + i] : 0;
+        const int over = wave_sum_i(nh > f.caph ? nh - f.caph : 0);
+        dropped += over;
+        dropped_any = dropped_any || over > 0;
+        unsigned long long mask = __ballot(nh > 0 || ns > 0);
+        while (mask) {
+            const int cl = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            const size_t rowc = (size_t)(c0 + cstep * (cg + cl)) * f.ni + i;
+            const int nhc = min(__shfl(nh, cl, 64), f.caph);
+            const int nsc = __shfl(ns, cl, 64);
+            const int *hs = f.hslot + rowc * f.caph;
+            for (int kb = 0; kb < nhc; kb += 64) {
+                const int k = kb + lane;
+                bool isnb = false;
+                int jo = 0;
+                if (k < nhc) {
+                    const int sj = hs[k];
+                    if ((unsigned)sj < (unsigned)f.nj) {
+                        isnb = classify<MFLAG>(f.rec64 + (size_t)sj * 8, I, true, s);
+                        jo = f.perm[sj] - jb;
+                    }
+                }
+                const unsigned long long b = __ballot(isnb);
+                int base = nnb;
+                if (SHARED) {
+                    int t = 0;
+                    if (lane == 0 && b) t = atomicAdd(nshared, __popcll(b));
+                    base = __shfl(t, 0, 64);
+                }
+                const int pos = base + lane_prefix(b);
+                if (isnb && pos < NB_SORT_MAX) buf[pos] = jo;
+                nnb += __popcll(b);
+            }
+            // soft slots: near-field pairs (force only) and, marked ~j, band pairs
+            // beyond the hard slots of an overflowing row; those were counted in
+            // dropped and are re-counted by the FP64 test here
+            const int *ss = f.sslot + rowc * f.caps;
+            const int nsr = min(nsc, f.caps);
+            unres += nsc - nsr;
+            for (int kb = 0; kb < nsr; kb += 64) {
+                const int k = kb + lane;
+                bool band = false, isnb = false;
+                if (k < nsr) {
+                    const int e = ss[k];
+                    band = e < 0;
+                    const int sj = band ? ~e : e;
+                    if ((unsigned)sj < (unsigned)f.nj)
+                        isnb = classify<MFLAG>(f.rec64 + (size_t)sj * 8, I, band, s);
+                }
+                dropped += __popcll(__ballot(isnb)) - __popcll(__ballot(band));
+            }
+        }
+    }
+
+}
+
 template <bool MFLAG>
 __global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
     __shared__ int nbuf[4][NB_SORT_MAX];
@@ -746,57 +817,8 @@ __global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
     int dropped = 0;      // hard candidates beyond the slot capacity
     int unres = 0;        // band pairs of an overflowing row left without a slot
     bool dropped_any = false;
-    for (int cb = 0; cb < f.nchunk; cb += 64) {
-        const int c = cb + lane;
-        const int nh = c < f.nchunk ? f.cnt[((size_t)c * 2 + 0) * f.ni + i] : 0;
-        const int ns = c < f.nchunk ? f.cnt[((size_t)c * 2 + 1) * f.ni + i] : 0;
-        const int over = wave_sum_i(nh > f.caph ? nh - f.caph : 0);
-        dropped += over;
-        dropped_any = dropped_any || over > 0;
-        unsigned long long mask = __ballot(nh > 0 || ns > 0);
-        while (mask) {
-            const int cl = __builtin_ctzll(mask);
-            mask &= mask - 1;
-            const size_t rowc = (size_t)(cb + cl) * f.ni + i;
-            const int nhc = min(__shfl(nh, cl, 64), f.caph);
-            const int nsc = __shfl(ns, cl, 64);
-            const int *hs = f.hslot + rowc * f.caph;
-            for (int kb = 0; kb < nhc; kb += 64) {
-                const int k = kb + lane;
-                bool isnb = false;
-                int jo = 0;
-                if (k < nhc) {
-                    const int sj = hs[k];
-                    if ((unsigned)sj < (unsigned)f.nj) {
-                        isnb = classify<MFLAG>(f.rec64 + (size_t)sj * 8, I, true, s);
-                        jo = f.perm[sj] - jb;
-                    }
-                }
-                const unsigned long long b = __ballot(isnb);
-                const int pos = nnb + lane_prefix(b);
-                if (isnb && pos < NB_SORT_MAX) buf[pos] = jo;
-                nnb += __popcll(b);
-            }
-            // soft slots: near-field pairs (force only) and, marked ~j, band pairs
-            // beyond the hard slots of an overflowing row; those were counted in
-            // dropped and are re-counted by the FP64 test here
-            const int *ss = f.sslot + rowc * f.caps;
-            const int nsr = min(nsc, f.caps);
-            unres += nsc - nsr;
-            for (int kb = 0; kb < nsr; kb += 64) {
-                const int k = kb + lane;
-                bool band = false, isnb = false;
-                if (k < nsr) {
-                    const int e = ss[k];
-                    band = e < 0;
-                    const int sj = band ? ~e : e;
-                    if ((unsigned)sj < (unsigned)f.nj)
-                        isnb = classify<MFLAG>(f.rec64 + (size_t)sj * 8, I, band, s);
-                }
-                dropped += __popcll(__ballot(isnb)) - __popcll(__ballot(band));
-            }
-        }
-    }
+    fin_candidates<MFLAG, false>(f, I, i, jb, 0, 1, lane, s, buf, nullptr, nnb, dropped, unres,
+                                 dropped_any);
 
 #pragma unroll
     for (int q = 0; q < 7; q++) s[q] = wave_sum(s[q]);
@@ -828,6 +850,81 @@ __global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
             const int e = buf[k];
             int rank = 0;
             for (int m = 0; m < nnb; m++) rank += buf[m] < e;
+            row[1 + rank] = e;
+        }
+    }
+}
+
+// One work-group of four wavefronts per i, for small i-sets, where one wavefront per
+// i leaves most of the GPU idle while it waits on its gathers: wavefront w takes the
+// chunks c = w (mod 4), so the few chunks that hold an i's candidates (they lie close
+// together in the Morton order) are spread over the four.  Forces are combined in a
+// fixed order; the row is rank-sorted as in regf_finalize_kernel.
+template <bool MFLAG>
+__global__ __launch_bounds__(256) void regf_finalize_wide_kernel(FinParams f) {
+    __shared__ int buf[NB_SORT_MAX];
+    __shared__ double red[4][7];
+    __shared__ int cnts[4][3];
+    __shared__ int nshared;
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int i = blockIdx.x;   // slot
+    const int r = f.imap ? f.imap[i] : i;
+    if (r < 0) return;   // the whole work-group leaves together
+    if (threadIdx.x == 0) nshared = 0;
+    __syncthreads();
+
+    double s[7];
+#pragma unroll
+    for (int q = 0; q < 7; q++) s[q] = 0.0;
+    for (int c = w + 4 * lane; c < f.nchunk; c += 256) {
+        const double *pp = f.part + (size_t)c * 7 * f.ni + i;
+#pragma unroll
+        for (int q = 0; q < 7; q++) s[q] += pp[(size_t)q * f.ni];
+    }
+    const int is = f.ib_stride;
+    IF64 I;
+    I.x = f.ib[0 * is + r]; I.y = f.ib[1 * is + r]; I.z = f.ib[2 * is + r];
+    I.u = f.ib[3 * is + r]; I.v = f.ib[4 * is + r]; I.w = f.ib[5 * is + r];
+    I.h2 = f.ib[6 * is + r]; I.dtr = f.ib[7 * is + r];
+    const int jb = f.jbase ? f.jbase[r] : 0;
+    int nnb = 0, dropped = 0, unres = 0;
+    bool dropped_any = false;
+    fin_candidates<MFLAG, true>(f, I, i, jb, w, 4, lane, s, buf, &nshared, nnb, dropped, unres,
+                                dropped_any);
+#pragma unroll
+    for (int q = 0; q < 7; q++) s[q] = wave_sum(s[q]);
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < 7; q++) red[w][q] = s[q];
+        cnts[w][0] = dropped;
+        cnts[w][1] = unres;
+        cnts[w][2] = dropped_any ? 1 : 0;
+    }
+    __syncthreads();
+    const int nall = nshared;
+    const int tot = nall + cnts[0][0] + cnts[1][0] + cnts[2][0] + cnts[3][0];
+    const int unr = cnts[0][1] + cnts[1][1] + cnts[2][1] + cnts[3][1];
+    const bool dany = (cnts[0][2] | cnts[1][2] | cnts[2][2] | cnts[3][2]) != 0;
+    // as in regf_finalize_kernel: a row whose hard slots overflowed reports -count;
+    // band pairs left without a slot make the row's forces NaN
+    const bool ovf = tot > f.nnbmax || dany;
+    int *row = f.list + (size_t)r * f.lmax;
+    if (threadIdx.x == 0) {
+        double *o = f.out + r;
+        const int os = f.out_stride;
+#pragma unroll
+        for (int q = 0; q < 7; q++) {
+            const double v = (red[0][q] + red[1][q]) + (red[2][q] + red[3][q]);
+            o[q * os] = unr > 0 ? __builtin_nan("") : v;
+        }
+        row[0] = ovf ? -tot : tot;
+    }
+    if (!ovf && nall > 0) {
+        for (int k = threadIdx.x; k < nall; k += 256) {
+            const int e = buf[k];
+            int rank = 0;
+            for (int m = 0; m < nall; m++) rank += buf[m] < e;
             row[1 + rank] = e;
         }
     }
@@ -981,11 +1078,19 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
     fp.out = d_out; fp.out_stride = ni_stride; fp.list = d_list;
     KTimer &t2 = ktimer("regf_finalize");
     t2.start(s);
-    const dim3 fgrid((ni + 3) / 4);
-    if (m_flag)
-        hipLaunchKernelGGL(regf_finalize_kernel<true>, fgrid, dim3(256), 0, s, fp);
-    else
-        hipLaunchKernelGGL(regf_finalize_kernel<false>, fgrid, dim3(256), 0, s, fp);
+    if (nchunk >= 32 || (ni <= FIN_WIDE_NI && nchunk >= 16)) {
+        // few i, or many chunks per i: four wavefronts per i
+        if (m_flag)
+            hipLaunchKernelGGL(regf_finalize_wide_kernel<true>, dim3(ni), dim3(256), 0, s, fp);
+        else
+            hipLaunchKernelGGL(regf_finalize_wide_kernel<false>, dim3(ni), dim3(256), 0, s, fp);
+    } else {
+        const dim3 fgrid((ni + 3) / 4);
+        if (m_flag)
+            hipLaunchKernelGGL(regf_finalize_kernel<true>, fgrid, dim3(256), 0, s, fp);
+        else
+            hipLaunchKernelGGL(regf_finalize_kernel<false>, fgrid, dim3(256), 0, s, fp);
+    }
     NB6_HIP(hipGetLastError());
     t2.stop(s);
     return OK;
