@@ -170,8 +170,8 @@ struct Gpunb {
     bool open = false;
     int nbmax = 0;
     JSet js;
-    HBuf hj, hi, hout, hlist;
-    DBuf di, dout, dlist;
+    HBuf hj, hi, hout, hlist, hoff;
+    DBuf di, dout, dlist, dlen, doff, dpack, dtmp;
     double time_send = 0, time_grav = 0, time_reduce = 0, time_out = 0;
     long long num_inter = 0;
     int icall = 0, ini = 0, isend = 0;
@@ -248,8 +248,9 @@ void gpunb_close_(void) {
     }
     g.open = false;
     jset_release(g.js);
-    g.hj.release(); g.hi.release(); g.hout.release(); g.hlist.release();
+    g.hj.release(); g.hi.release(); g.hout.release(); g.hlist.release(); g.hoff.release();
     g.di.release(); g.dout.release(); g.dlist.release();
+    g.dlen.release(); g.doff.release(); g.dpack.release(); g.dtmp.release();
     regf_release();
     g.nbmax = 0;
 }
@@ -339,12 +340,46 @@ static void regf_finish(Gpunb &g, int ni, const double *di, int lmax, int nnbmax
     int *dlist = static_cast<int *>(g.dlist.get((size_t)ni * lmax * sizeof(int)));
     if (regf_launch(g.js, ni, ni, di, lmax, nnbmax, m_flag, dout, dlist, s) != OK) return;
     double *ho = static_cast<double *>(g.hout.get((size_t)7 * ni * sizeof(double)));
-    int *hl = static_cast<int *>(g.hlist.get((size_t)ni * lmax * sizeof(int)));
     NB6_HIP(hipMemcpyAsync(ho, dout, (size_t)7 * ni * sizeof(double), hipMemcpyDeviceToHost, s));
-    NB6_HIP(hipMemcpyAsync(hl, dlist, (size_t)ni * lmax * sizeof(int), hipMemcpyDeviceToHost, s));
+    const int nnbeff = std::max(0, std::min(nnbmax, lmax - 1));
+    const size_t full = (size_t)ni * lmax * sizeof(int);
+    if (full > ((size_t)2 << 20)) {
+        // large blocks: only the rows' used part crosses PCIe (about nnb + 1 of lmax ints
+        // per row), packed on the device and spread into the caller's rows here
+        int *len = static_cast<int *>(g.dlen.get((size_t)(ni + 1) * sizeof(int)));
+        int *off = static_cast<int *>(g.doff.get((size_t)(ni + 1) * sizeof(int)));
+        int *pack = static_cast<int *>(g.dpack.get(full));
+        pack_rows(dlist, ni, lmax, nnbeff, len, off, pack, g.dtmp, s);
+        int *hoff = static_cast<int *>(g.hoff.get((size_t)(ni + 1) * sizeof(int)));
+        NB6_HIP(hipMemcpyAsync(hoff, off, (size_t)(ni + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+        NB6_HIP(hipStreamSynchronize(s));
+        const size_t total = (size_t)hoff[ni];
+        int *hp = static_cast<int *>(g.hlist.get(total * sizeof(int)));
+        NB6_HIP(hipMemcpyAsync(hp, pack, total * sizeof(int), hipMemcpyDeviceToHost, s));
+        NB6_HIP(hipStreamSynchronize(s));
+        const double t1 = wtime();
+        par_for(ni, [&](int a, int b) {
+            for (int i = a; i < b; i++) {
+                acc[i][0] = ho[0 * (size_t)ni + i];
+                acc[i][1] = ho[1 * (size_t)ni + i];
+                acc[i][2] = ho[2 * (size_t)ni + i];
+                jrk[i][0] = ho[3 * (size_t)ni + i];
+                jrk[i][1] = ho[4 * (size_t)ni + i];
+                jrk[i][2] = ho[5 * (size_t)ni + i];
+                pot[i] = ho[6 * (size_t)ni + i];
+                std::memcpy(list + (size_t)i * lmax, hp + hoff[i],
+                            (size_t)(hoff[i + 1] - hoff[i]) * sizeof(int));
+            }
+        });
+        const double t2 = wtime();
+        g.time_grav += t1 - t0;
+        g.time_reduce += t2 - t1;
+        return;
+    }
+    int *hl = static_cast<int *>(g.hlist.get(full));
+    NB6_HIP(hipMemcpyAsync(hl, dlist, full, hipMemcpyDeviceToHost, s));
     NB6_HIP(hipStreamSynchronize(s));
     const double t1 = wtime();
-    const int nnbeff = std::max(0, std::min(nnbmax, lmax - 1));
     par_for(ni, [&](int a, int b) {
         for (int i = a; i < b; i++) {
             acc[i][0] = ho[0 * (size_t)ni + i];

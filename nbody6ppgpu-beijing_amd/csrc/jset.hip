@@ -362,6 +362,47 @@ const int *jset_sort_i(const JSet &js, int ni, int stride, const double *d_i, Re
     return v1;
 }
 
+namespace {
+// row i of a [n][lmax] list array holds 1 + count ints (1 if the count is negative or
+// above maxc); len[n] = 0 so that the exclusive scan's last entry is the total
+__global__ __launch_bounds__(256) void row_len_kernel(const int *__restrict__ list, int n, int lmax,
+                                                      int maxc, int *__restrict__ len) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i > n) return;
+    int l = 0;
+    if (i < n) {
+        const int c = list[(size_t)i * lmax];
+        l = 1 + (c > 0 && c <= maxc ? c : 0);
+    }
+    len[i] = l;
+}
+
+// one wavefront per row: copy its 1 + count ints to pack + off[i]
+__global__ __launch_bounds__(256) void row_pack_kernel(const int *__restrict__ list, int n, int lmax,
+                                                       const int *__restrict__ off,
+                                                       int *__restrict__ pack) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const int lane = threadIdx.x & 63;
+    const int o = off[i], l = off[i + 1] - o;
+    const int *src = list + (size_t)i * lmax;
+    for (int k = lane; k < l; k += 64) pack[o + k] = src[k];
+}
+}  // namespace
+
+void pack_rows(const int *list, int n, int lmax, int maxc, int *len, int *off, int *pack,
+               DBuf &tmp, hipStream_t s) {
+    hipLaunchKernelGGL(row_len_kernel, dim3((n + 1 + 255) / 256), dim3(256), 0, s, list, n, lmax,
+                       maxc, len);
+    NB6_HIP(hipGetLastError());
+    size_t tb = 0;
+    NB6_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, len, off, n + 1, s));
+    void *t = tmp.get(tb);
+    NB6_HIP(hipcub::DeviceScan::ExclusiveSum(t, tb, len, off, n + 1, s));
+    hipLaunchKernelGGL(row_pack_kernel, dim3((n + 3) / 4), dim3(256), 0, s, list, n, lmax, off, pack);
+    NB6_HIP(hipGetLastError());
+}
+
 void jset_from_soa(JSet &js, int n, int stride, const double *d_soa, hipStream_t s) {
     if (n > 0) {
         hipLaunchKernelGGL(soa_to_aos_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_soa, n,

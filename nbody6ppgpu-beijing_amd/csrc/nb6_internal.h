@@ -104,6 +104,12 @@ void jset_release(JSet &js);
 // in64 already holds n records; box = {lo x,y,z, hi x,y,z} or nullptr (computed)
 void jset_build(JSet &js, int n, const double *box, double mmax, hipStream_t s);
 void jset_from_soa(JSet &js, int n, int stride, const double *d_soa, hipStream_t s);
+// Rows [count, entries...] of a [n][lmax] list array packed end to end on the device:
+// off[i] (n + 1 entries, off[n] = total) and pack[off[i] .. off[i + 1]) = the row's
+// 1 + count leading ints (only the count if it is negative or above maxc); len is
+// n + 1 ints of scratch (jset.hip).
+void pack_rows(const int *list, int n, int lmax, int maxc, int *len, int *off, int *pack,
+               DBuf &tmp, hipStream_t s);
 
 struct RegfConfig {
     int threads = 0;      // work-group size = LDS j-tile (128, 256 or 512; 0: by ni)
