@@ -196,6 +196,28 @@ def test_regf_overflow_exact_negative_counts(gpunb):
     assert vec_rel_err(got[2][:, None], ref[2][:, None]).max() <= REL_TOL
 
 
+def test_regf_overflow_rows_through_the_packed_copy(gpunb):
+    """A host-ABI call whose [ni][lmax] list array exceeds 2 MB comes back through the
+    packed copy (only each row's count and neighbours cross to the host): the same
+    counts, overflow signs and rows as the oracle, with rows past the count untouched."""
+    m, x, v, idx, h2, dtr = c2_inputs(n=16384, ni=4096, counts_median=60.0)
+    n = m.size
+    lmax, nnbmax = 200, 64
+    assert 4096 * lmax * 4 > 2 << 20
+    g = gpunb(n)
+    g.send(m, x, v)
+    got = g.regf(h2, dtr, x[idx], v[idx], lmax, nnbmax)
+    ref = O.regf(m, x, v, h2, dtr, x[idx], v[idx], lmax, nnbmax)
+    cnt = got[3][:, 0]
+    assert (cnt < 0).any() and (cnt > 0).any()
+    assert (cnt == ref[3][:, 0]).all()
+    ok = cnt >= 0
+    assert rows_as_sets(got[3][ok]) == rows_as_sets(ref[3][ok])
+    past = np.arange(lmax)[None, :] > np.maximum(cnt, 0)[:, None]
+    assert (got[3][past] == 0).all(), "entries past a row's count were written"
+    assert vec_rel_err(got[0], ref[0]).max() <= REL_TOL
+
+
 def _crowded_row(seed, n_bg, n_in, n_band, n_shell, rs=0.05, eps=2.0e-7):
     """Background particles, then around the point c: n_in inside 0.9 RS, n_band at
     RS (1 +- eps) (alternately inside and outside, all within the FP32 kernel's band
