@@ -665,6 +665,7 @@ def bench_c2(steps, warmup, no_cpu):
             kf.append(L.nb6_kernel_ms(b"regf_finalize"))
         L.nb6_sync()
         dev_ms = (time.perf_counter() - t0) * 1e3 / steps
+        chunks = L.nb6_last_regf_chunks()
         out = do.to_host(np.float64, (7, ni))
         lst = dl.to_host(np.int32, (ni, lmax))
         full, ref64, _ = all_i_vs_fp64_kernel(L, ni, di, do, dl, out, lst, lmax, nnbmax)
@@ -677,7 +678,7 @@ def bench_c2(steps, warmup, no_cpu):
         abi_ms = (time.perf_counter() - t0) * 1e3 / steps
         row = {"ni": ni, "pairs": ni * n, "device_call_ms": dev_ms,
                "tile_kernel_ms": float(np.median(kt)), "finalize_kernel_ms": float(np.median(kf)),
-               "j_chunks": L.nb6_last_regf_chunks(), "host_abi_call_ms": abi_ms,
+               "j_chunks": chunks, "host_abi_call_ms": abi_ms,
                "pairs_per_s_device": ni * n / (dev_ms * 1e-3),
                "pairs_per_s_host_abi": ni * n / (abi_ms * 1e-3),
                "acc_max_vs_fp64": full["acc"]["max"], "jerk_p99_vs_fp64": full["jerk"]["p99"],
