@@ -698,8 +698,13 @@ __device__ __forceinline__ bool classify(const double *__restrict__ rec, const I
     const double thr = MFLAG ? r0.w * I.h2 : I.h2;
     const bool isnb = test && rmin < thr;
     if (!isnb && r2 > 0.0) {
-        const double rinv2 = 1.0 / r2;
-        const double rinv = sqrt(rinv2);
+        // 1/sqrt(r2): v_rsq_f64 (~23 bits) and two Newton steps, y (3/2 - r2 y^2 / 2),
+        // which leave a few ulp, in place of a correctly rounded divide and square root
+        const double hr = 0.5 * r2;
+        double rinv = __builtin_amdgcn_rsq(r2);
+        rinv = rinv * fma(-hr * rinv, rinv, 1.5);
+        rinv = rinv * fma(-hr * rinv, rinv, 1.5);
+        const double rinv2 = rinv * rinv;
         const double mr1 = r0.w * rinv;
         const double mr3 = mr1 * rinv2;
         const double rv = dx * du + dy * dv + dz * dw;
