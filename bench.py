@@ -436,6 +436,26 @@ def bench_irr(steps, warmup, no_cpu):
     out = dout.to_host(np.float64, (ni, 8))
     kt = float(np.mean(kms)) * 1e-3
     alg = nnb_tot * (4 + 72) + ni * (72 + 52)
+    # call time by block size: the first nb of the same active particles, on the device
+    # (addresses and results in HBM) and through irr_simd_firr_vec_ (host arrays)
+    sweep = []
+    for nb in (1, 64, 1024, 16384):
+        for _ in range(warmup):
+            L.nb6_irr_firr_device(ti, nb, da.ptr, dout.ptr, None)
+            irr.firr_vec(ti, idx[:nb] + 1)
+        L.nb6_sync()
+        t_0 = time.perf_counter()
+        for _ in range(steps):
+            if L.nb6_irr_firr_device(ti, nb, da.ptr, dout.ptr, None) != 0:
+                lib.check_error()
+        L.nb6_sync()
+        dev_ms = (time.perf_counter() - t_0) * 1e3 / steps
+        t_0 = time.perf_counter()
+        for _ in range(steps):
+            irr.firr_vec(ti, idx[:nb] + 1)
+        abi_ms = (time.perf_counter() - t_0) * 1e3 / steps
+        sweep.append({"ni": nb, "device_call_ms": dev_ms, "host_abi_call_ms": abi_ms})
+        log(f"irr ni={nb}: device {dev_ms:.3f} ms, host ABI {abi_ms:.3f} ms")
     check = None
     if not no_cpu:
         import oracle as O
@@ -468,7 +488,7 @@ def bench_irr(steps, warmup, no_cpu):
                          "unit": "GB/s", "frac": round(alg / kt / 8e12, 4),
                          "algorithmic_bytes": alg, "kernel": "irr_predict_all + irr_force_kernel",
                          "avg_launch_ms": kt * 1e3},
-            "check_vs_fp64": check}
+            "check_vs_fp64": check, "call_time_by_block_size": sweep}
 
 
 def bench_regcor(steps, warmup, no_cpu):
