@@ -260,28 +260,56 @@ void gpunb_send_(int *nj, double mj[], double xj[][3], double vj[][3]) {
     if (!g.open) { set_error(E_STATE, "gpunb_send: library not open"); return; }
     const int n = *nj;
     if (n < 0 || n > g.nbmax) { set_error(E_ARG, "gpunb_send: nj outside [0, nbmax]"); return; }
-    if (!check_finite(mj, n) || !check_finite(&xj[0][0], 3 * (size_t)n) ||
-        !check_finite(&vj[0][0], 3 * (size_t)n)) {
-        set_error(E_ARG, "gpunb_send: NaN in j-particle data");
-        return;
-    }
-    g.time_send -= wtime();
-    g.isend++;
+    const double tstart = wtime();
     Device &d = device();
-    // AoS records (x y z m vx vy vz 0) in the caller's order, plus box and max mass
+    // AoS records (x y z m vx vy vz 0) in the caller's order, plus box, max mass and a
+    // NaN check, in one pass spread over a few threads (one range each)
     double *h = static_cast<double *>(g.hj.get((size_t)8 * std::max(n, 1) * sizeof(double)));
+    const int nt = n > 65536 ? std::max(1, std::min(8, (int)std::thread::hardware_concurrency())) : 1;
+    std::vector<double> part((size_t)nt * 7);
+    std::vector<char> nan(nt, 0);
+    auto pack = [&](int t) {
+        const int a = (int)((long long)n * t / nt), b = (int)((long long)n * (t + 1) / nt);
+        double bx[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+        double mm = 0.0;
+        bool bad = false;
+        for (int j = a; j < b; j++) {
+            double *r = h + (size_t)8 * j;
+            r[0] = xj[j][0]; r[1] = xj[j][1]; r[2] = xj[j][2]; r[3] = mj[j];
+            r[4] = vj[j][0]; r[5] = vj[j][1]; r[6] = vj[j][2]; r[7] = 0.0;
+            for (int k = 0; k < 7; k++) bad = bad || !(r[k] == r[k]);
+            for (int k = 0; k < 3; k++) {
+                bx[k] = r[k] < bx[k] ? r[k] : bx[k];
+                bx[3 + k] = r[k] > bx[3 + k] ? r[k] : bx[3 + k];
+            }
+            mm = mj[j] > mm ? mj[j] : mm;
+        }
+        for (int k = 0; k < 6; k++) part[(size_t)t * 7 + k] = bx[k];
+        part[(size_t)t * 7 + 6] = mm;
+        nan[t] = bad;
+    };
+    if (nt == 1) {
+        pack(0);
+    } else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; t++) th.emplace_back(pack, t);
+        for (auto &x : th) x.join();
+    }
     double box[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
     double mmax = 0.0;
-    for (int j = 0; j < n; j++) {
-        double *r = h + (size_t)8 * j;
-        r[0] = xj[j][0]; r[1] = xj[j][1]; r[2] = xj[j][2]; r[3] = mj[j];
-        r[4] = vj[j][0]; r[5] = vj[j][1]; r[6] = vj[j][2]; r[7] = 0.0;
-        for (int k = 0; k < 3; k++) {
-            box[k] = r[k] < box[k] ? r[k] : box[k];
-            box[3 + k] = r[k] > box[3 + k] ? r[k] : box[3 + k];
+    for (int t = 0; t < nt; t++) {
+        if (nan[t]) {
+            set_error(E_ARG, "gpunb_send: NaN in j-particle data");
+            return;
         }
-        mmax = mj[j] > mmax ? mj[j] : mmax;
+        for (int k = 0; k < 3; k++) {
+            box[k] = std::min(box[k], part[(size_t)t * 7 + k]);
+            box[3 + k] = std::max(box[3 + k], part[(size_t)t * 7 + 3 + k]);
+        }
+        mmax = std::max(mmax, part[(size_t)t * 7 + 6]);
     }
+    g.time_send -= tstart;   // the pack counts as send time
+    g.isend++;
     if (n > 0)
         NB6_HIP(hipMemcpyAsync(g.js.in64.p, h, (size_t)8 * n * sizeof(double),
                                hipMemcpyHostToDevice, d.stream));

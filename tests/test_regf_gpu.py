@@ -218,6 +218,35 @@ def test_regf_overflow_rows_through_the_packed_copy(gpunb):
     assert vec_rel_err(got[0], ref[0]).max() <= REL_TOL
 
 
+@pytest.mark.parametrize("n", [1000, 200000])
+def test_send_rejects_nan_then_recovers(gpunb, n):
+    """gpunb_send_ packs the j-set on several host threads above 65,536 particles: a NaN
+    in any field of any range is reported (nothing is sent), and the next clean send
+    gives the oracle's lists and forces."""
+    m, x, v, idx, h2, dtr = c2_inputs(n=n, ni=64, counts_median=40.0)
+    g = gpunb(n)
+    L = lib.load()
+    L.nb6_set_abort_on_error(0)   # the library aborts on errors unless told otherwise
+    try:
+        for field, (row, col) in (("x", (n - 1, 2)), ("v", (n // 2, 0)), ("m", (7, None))):
+            xb, vb, mb = x.copy(), v.copy(), m.copy()
+            if field == "x":
+                xb[row, col] = np.nan
+            elif field == "v":
+                vb[row, col] = np.nan
+            else:
+                mb[row] = np.nan
+            with pytest.raises(RuntimeError, match="NaN in j-particle data"):
+                g.send(mb, xb, vb)
+    finally:
+        L.nb6_set_abort_on_error(1)
+    g.send(m, x, v)
+    lmax, nnbmax = 200, 150
+    got = g.regf(h2, dtr, x[idx], v[idx], lmax, nnbmax)
+    ref = O.regf(m, x, v, h2, dtr, x[idx], v[idx], lmax, nnbmax)
+    _compare(got, ref, f"after NaN sends n={n}")
+
+
 def _crowded_row(seed, n_bg, n_in, n_band, n_shell, rs=0.05, eps=2.0e-7):
     """Background particles, then around the point c: n_in inside 0.9 RS, n_band at
     RS (1 +- eps) (alternately inside and outside, all within the FP32 kernel's band
