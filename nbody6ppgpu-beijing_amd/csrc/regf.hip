@@ -10,8 +10,10 @@
 //
 //  * The j-set is Morton-ordered at send time (jset.hip) and streamed as FP32
 //    records relative to the origin of their 256-particle tile.  regf_tile_kernel
-//    (grid: i-blocks x j-chunks, T = 128, 256 or 512 threads, IPL i-particles per
-//    lane) stages T records per step in LDS by direct global-to-LDS loads
+//    (one-dimensional grid of (j-chunk, i-block) items, chunk-major, each XCD taking a
+//    contiguous run; T = 128, 256 or 512 threads, IPL i-particles per lane; chunks are
+//    added until a call has min_groups work-groups, down to one 256- or 512-record
+//    unit per chunk) stages T records per step in LDS by direct global-to-LDS loads
 //    (double-buffered), forms each lane's i-particle relative to the same tile origin
 //    in FP64, and evaluates every pair in FP32: v_rsq_f32 refined by one
 //    Newton-Raphson step, FP32 partial sums flushed into FP64 every 64 j.
