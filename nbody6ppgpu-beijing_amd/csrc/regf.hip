@@ -34,8 +34,8 @@
 //    the reference computes them; if even the soft slots run out the row's
 //    forces are NaN.
 //
-//  * regf_finalize_kernel: one wavefront per i (four, regf_finalize_wide_kernel, when
-//    there are 32 chunks or more, or 16 and at most 16384 i).  Sums the FP64 chunk partials,
+//  * regf_finalize_kernel: one wavefront per i (regf_finalize_wide_kernel: eight for at
+//    most 16384 i in 16 chunks or more, four for 32 chunks or more).  Sums the FP64 chunk partials,
 //    re-tests every hard candidate in FP64 with the reference formula (no FMA
 //    contraction), adds the FP64 force of every non-neighbour candidate, maps
 //    neighbours back to the caller's indices and writes each row in ascending
@@ -862,16 +862,16 @@ __global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
     }
 }
 
-// One work-group of four wavefronts per i, for small i-sets, where one wavefront per
-// i leaves most of the GPU idle while it waits on its gathers: wavefront w takes the
-// chunks c = w (mod 4), so the few chunks that hold an i's candidates (they lie close
-// together in the Morton order) are spread over the four.  Forces are combined in a
-// fixed order; the row is rank-sorted as in regf_finalize_kernel.
-template <bool MFLAG>
-__global__ __launch_bounds__(256) void regf_finalize_wide_kernel(FinParams f) {
+// One work-group of W wavefronts per i, for small i-sets, where one wavefront per i
+// leaves most of the GPU idle while it waits on its gathers: wavefront w takes the
+// chunks c = w (mod W), so the few chunks that hold an i's candidates (they lie close
+// together in the Morton order) are spread over the W.  Forces are combined in a fixed
+// order; the row is rank-sorted as in regf_finalize_kernel.
+template <bool MFLAG, int W>
+__global__ __launch_bounds__(64 * W) void regf_finalize_wide_kernel(FinParams f) {
     __shared__ int buf[NB_SORT_MAX];
-    __shared__ double red[4][7];
-    __shared__ int cnts[4][3];
+    __shared__ double red[W][7];
+    __shared__ int cnts[W][3];
     __shared__ int nshared;
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
@@ -884,7 +884,7 @@ __global__ __launch_bounds__(256) void regf_finalize_wide_kernel(FinParams f) {
     double s[7];
 #pragma unroll
     for (int q = 0; q < 7; q++) s[q] = 0.0;
-    for (int c = w + 4 * lane; c < f.nchunk; c += 256) {
+    for (int c = w + W * lane; c < f.nchunk; c += 64 * W) {
         const double *pp = f.part + (size_t)c * 7 * f.ni + i;
 #pragma unroll
         for (int q = 0; q < 7; q++) s[q] += pp[(size_t)q * f.ni];
@@ -897,7 +897,7 @@ __global__ __launch_bounds__(256) void regf_finalize_wide_kernel(FinParams f) {
     const int jb = f.jbase ? f.jbase[r] : 0;
     int nnb = 0, dropped = 0, unres = 0;
     bool dropped_any = false;
-    fin_candidates<MFLAG, true>(f, I, i, jb, w, 4, lane, s, buf, &nshared, nnb, dropped, unres,
+    fin_candidates<MFLAG, true>(f, I, i, jb, w, W, lane, s, buf, &nshared, nnb, dropped, unres,
                                 dropped_any);
 #pragma unroll
     for (int q = 0; q < 7; q++) s[q] = wave_sum(s[q]);
@@ -910,9 +910,14 @@ __global__ __launch_bounds__(256) void regf_finalize_wide_kernel(FinParams f) {
     }
     __syncthreads();
     const int nall = nshared;
-    const int tot = nall + cnts[0][0] + cnts[1][0] + cnts[2][0] + cnts[3][0];
-    const int unr = cnts[0][1] + cnts[1][1] + cnts[2][1] + cnts[3][1];
-    const bool dany = (cnts[0][2] | cnts[1][2] | cnts[2][2] | cnts[3][2]) != 0;
+    int tot = nall, unr = 0, dq = 0;
+#pragma unroll
+    for (int q = 0; q < W; q++) {
+        tot += cnts[q][0];
+        unr += cnts[q][1];
+        dq |= cnts[q][2];
+    }
+    const bool dany = dq != 0;
     // as in regf_finalize_kernel: a row whose hard slots overflowed reports -count;
     // band pairs left without a slot make the row's forces NaN
     const bool ovf = tot > f.nnbmax || dany;
@@ -922,13 +927,15 @@ __global__ __launch_bounds__(256) void regf_finalize_wide_kernel(FinParams f) {
         const int os = f.out_stride;
 #pragma unroll
         for (int q = 0; q < 7; q++) {
-            const double v = (red[0][q] + red[1][q]) + (red[2][q] + red[3][q]);
+            double v = 0.0;   // a fixed order
+#pragma unroll
+            for (int u = 0; u < W; u++) v += red[u][q];
             o[q * os] = unr > 0 ? __builtin_nan("") : v;
         }
         row[0] = ovf ? -tot : tot;
     }
     if (!ovf && nall > 0) {
-        for (int k = threadIdx.x; k < nall; k += 256) {
+        for (int k = threadIdx.x; k < nall; k += 64 * W) {
             const int e = buf[k];
             int rank = 0;
             for (int m = 0; m < nall; m++) rank += buf[m] < e;
@@ -1085,12 +1092,18 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
     fp.out = d_out; fp.out_stride = ni_stride; fp.list = d_list;
     KTimer &t2 = ktimer("regf_finalize");
     t2.start(s);
-    if (nchunk >= 32 || (ni <= FIN_WIDE_NI && nchunk >= 16)) {
-        // few i, or many chunks per i: four wavefronts per i
+    if (ni <= FIN_WIDE_NI && nchunk >= 16) {
+        // few i: eight wavefronts per i
         if (m_flag)
-            hipLaunchKernelGGL(regf_finalize_wide_kernel<true>, dim3(ni), dim3(256), 0, s, fp);
+            hipLaunchKernelGGL((regf_finalize_wide_kernel<true, 8>), dim3(ni), dim3(512), 0, s, fp);
         else
-            hipLaunchKernelGGL(regf_finalize_wide_kernel<false>, dim3(ni), dim3(256), 0, s, fp);
+            hipLaunchKernelGGL((regf_finalize_wide_kernel<false, 8>), dim3(ni), dim3(512), 0, s, fp);
+    } else if (nchunk >= 32) {
+        // many chunks per i: four wavefronts per i
+        if (m_flag)
+            hipLaunchKernelGGL((regf_finalize_wide_kernel<true, 4>), dim3(ni), dim3(256), 0, s, fp);
+        else
+            hipLaunchKernelGGL((regf_finalize_wide_kernel<false, 4>), dim3(ni), dim3(256), 0, s, fp);
     } else {
         const dim3 fgrid((ni + 3) / 4);
         if (m_flag)
