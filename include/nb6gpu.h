@@ -154,6 +154,17 @@ int nb6_set_regf_precision(int mode);
 double nb6_kernel_ms(const char *name);
 int    nb6_last_regf_chunks(void);
 
+/* Work-array budget of one regular-force call, in MiB (default 32768).  The work
+ * arrays hold one row per (i-particle, j-chunk): FP64 partials, candidate counts and
+ * candidate slots, (nnbmax + 8 + 1024) * 4 + 64 bytes each (about 6.3 KB at
+ * nnbmax 550).  A call whose rows would exceed the budget runs in batches of whole
+ * i-blocks; chunks, finalize form and results are those of the unbatched call.
+ * nb6_last_regf_batches() returns the number of batches of the last call.  (No
+ * counterpart in the reference, whose i-sets are capped at NIMAX = 2048,
+ * gpunb.velocity.cu:24,751.) */
+int nb6_set_regf_slot_budget(int mib);
+int nb6_last_regf_batches(void);
+
 /* Ensemble of independent clusters in one device j-set (no counterpart in the
  * reference, which holds one j-set per library instance).
  *   nb6_ens_send_device: offsets[nclusters+1] (host) index the concatenated SoA

@@ -291,8 +291,16 @@ void gpunb_send_(int *nj, double mj[], double xj[][3], double vj[][3]) {
     if (nt == 1) {
         pack(0);
     } else {
+        // a thread that cannot be created (std::system_error, e.g. a process limit)
+        // leaves its range, and every later one, to the calling thread; every thread
+        // that was created is joined, so no joinable std::thread is ever destroyed
         std::vector<std::thread> th;
-        for (int t = 0; t < nt; t++) th.emplace_back(pack, t);
+        int t = 0;
+        try {
+            for (; t < nt; t++) th.emplace_back(pack, t);
+        } catch (...) {
+        }
+        for (int r = t; r < nt; r++) pack(r);
         for (auto &x : th) x.join();
     }
     double box[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
@@ -629,6 +637,12 @@ int nb6_device_copy(void *dst, const void *src, unsigned long long bytes, void *
     return OK;
 }
 int nb6_last_regf_chunks(void) { return regf_last_chunks(); }
+int nb6_last_regf_batches(void) { return regf_last_batches(); }
+int nb6_set_regf_slot_budget(int mib) {
+    if (mib < 1) { set_error(E_ARG, "nb6_set_regf_slot_budget: budget must be at least 1 MiB"); return E_ARG; }
+    regf_config().slot_budget_mb = mib;
+    return OK;
+}
 
 static hipStream_t pick(void *stream) {
     return stream ? static_cast<hipStream_t>(stream) : device().stream;
@@ -638,12 +652,17 @@ void nb6_jp_open_(int *nmax) { (void)jp_open(*nmax); }
 void nb6_jp_close_(void) { (void)jp_close(); }
 void nb6_jp_set_(int *n, int addr[], double x0[][3], double x0dot[][3], double f[][3],
                  double fdot[][3], double m[], double t0[]) {
-    if (!check_finite(&x0[0][0], 3 * (size_t)*n) || !check_finite(&x0dot[0][0], 3 * (size_t)*n) ||
-        !check_finite(&f[0][0], 3 * (size_t)*n) || !check_finite(&fdot[0][0], 3 * (size_t)*n)) {
+    const int k = *n;
+    if (k < 0) { set_error(E_ARG, "nb6_jp_set: n must not be negative"); return; }
+    if (k == 0) return;
+    const size_t k3 = 3 * (size_t)k;
+    if (!check_finite(&x0[0][0], k3) || !check_finite(&x0dot[0][0], k3) ||
+        !check_finite(&f[0][0], k3) || !check_finite(&fdot[0][0], k3) ||
+        !check_finite(m, (size_t)k) || !check_finite(t0, (size_t)k)) {
         set_error(E_ARG, "nb6_jp_set: NaN in particle data");
         return;
     }
-    (void)jp_set(*n, addr, x0, x0dot, f, fdot, m, t0);
+    (void)jp_set(k, addr, x0, x0dot, f, fdot, m, t0);
 }
 void nb6_jp_predict_send_(double *time, int *nj) {
     Gpunb &g = g_nb;

@@ -703,7 +703,9 @@ int irr_profile(int rank) {
         const double gf = 60.0 * double(g.num_inter) * 1e-9 / (g.time_grav > 0 ? g.time_grav : 1e-30);
         const double us = 1e6 * g.time_grav / double(g.num_fcall);
         const double nb = g.num_steps ? double(g.num_inter) / double(g.num_steps) : 0.0;
-        const int niavr = (int)(g.num_steps / g.num_fcall);
+        // the reference's <NI> field is neighbour pairs per call, num_inter / num_fcall
+        // (irr.avx.cpp:414); kept as it is, since the line is read by people and scripts
+        const int niavr = (int)(g.num_inter / g.num_fcall);
         fprintf(stderr,
                 "[R.%d GPU Irr.F ] Ncall: %llu <NI>: %d <NB>: %f grav: %f s, %f Gflops, %f usec\n",
                 rank, g.num_fcall, niavr, nb, g.time_grav, gf, us);

@@ -118,6 +118,8 @@ struct RegfConfig {
     int isort_min = 2048; // i-sets this large are Morton-ordered (0: never)
     int far_test = 1;     // skip the pair test on tiles far from the whole wavefront
     int prec = 0;         // pair arithmetic: 0 FP32, 1 float-float, 2 FP64 (study variants)
+    int slot_budget_mb = 32768;   // largest work-array footprint of one call; larger
+                                  // calls run in batches of i (same results)
 };
 RegfConfig &regf_config();
 // the configuration a call with ni i-particles runs with (threads resolved)
@@ -128,6 +130,7 @@ RegfConfig regf_effective(int ni);
 int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lmax,
                 int nnbmax, int m_flag, double *d_out, int *d_list, hipStream_t s);
 int regf_last_chunks();
+int regf_last_batches();
 void regf_release();
 
 // Scratch of one regular-force pipeline: FP32 partials, candidate counts and slots.

@@ -946,6 +946,7 @@ __global__ __launch_bounds__(64 * W) void regf_finalize_wide_kernel(FinParams f)
 
 RegfWork g_work;
 int g_last_chunks = 0;
+int g_last_batches = 0;
 
 template <int T, int IPL>
 void launch_tile_t(bool mflag, dim3 grid, const TileParams &tp, hipStream_t s) {
@@ -1050,6 +1051,29 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
     // a chunk cannot hold more candidates than records
     const int caph = std::min(nnbeff + 8, clen);
     const int caps = std::min(CAP_SOFT, clen);
+    // the finalize form depends on the whole call, never on the batching below, so a
+    // batched call gives the same bits as an unbatched one
+    const int fin_form = (ni <= FIN_WIDE_NI && nchunk >= 16) ? 8 : (nchunk >= 32 ? 4 : 1);
+
+    // Work arrays hold one row per (i, chunk): FP64 partials, counts and the hard and
+    // soft candidate slots.  Calls whose rows would exceed the slot budget run in
+    // batches of i (whole i-blocks); chunks and results do not change.
+    const size_t row_bytes = 7 * sizeof(double) + 2 * sizeof(int) +
+                             ((size_t)caph + (size_t)caps) * sizeof(int);
+    const size_t budget = (size_t)std::max(cfg.slot_budget_mb, 1) << 20;
+    int nbatch_i = ni;
+    if ((size_t)nchunk * (size_t)ni * row_bytes > budget) {
+        const size_t rows_max = budget / row_bytes;
+        nbatch_i = (int)std::max<size_t>((size_t)nbi, rows_max / (size_t)nchunk / nbi * nbi);
+        nbatch_i = std::min(nbatch_i, ni);
+    }
+    const size_t rows = (size_t)nchunk * nbatch_i;
+    double *part = static_cast<double *>(g_work.part.get(rows * 7 * sizeof(double)));
+    int *cnt = static_cast<int *>(g_work.cnt.get(rows * 2 * sizeof(int)));
+    int *hslot = static_cast<int *>(g_work.hslot.get(rows * caph * sizeof(int)));
+    int *sslot = static_cast<int *>(g_work.sslot.get(rows * caps * sizeof(int)));
+    g_last_chunks = nchunk;
+    g_last_batches = (ni + nbatch_i - 1) / nbatch_i;
 
     // Morton-ordered i: a wavefront's particles are neighbours in space, so their
     // candidates fall on the same j and most tiles are far from all of them
@@ -1057,62 +1081,67 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
                           ? jset_sort_i(js, ni, ni_stride, d_i, g_work, s)
                           : nullptr;
 
-    const size_t rows = (size_t)nchunk * ni;
-    double *part = static_cast<double *>(g_work.part.get(rows * 7 * sizeof(double)));
-    int *cnt = static_cast<int *>(g_work.cnt.get(rows * 2 * sizeof(int)));
-    int *hslot = static_cast<int *>(g_work.hslot.get(rows * caph * sizeof(int)));
-    int *sslot = static_cast<int *>(g_work.sslot.get(rows * caps * sizeof(int)));
-    g_last_chunks = nchunk;
-
-    TileParams tp;
-    tp.ib = d_i; tp.ni = ni; tp.ib_stride = ni_stride; tp.imap = imap;
-    tp.rec32 = js.rec32.as<float4>(); tp.torg = js.torg.as<double>(); tp.nj = nj;
-    tp.tbox = js.tbox.as<float>();
-    tp.far_test = cfg.far_test;
-    tp.rec64 = js.rec64.as<double>();
-    tp.chunk_len = clen;
-    tp.nib = nib;
-    tp.mmax = js.mmax;
-    tp.part = part; tp.cnt = cnt;
-    tp.hslot = hslot; tp.caph = caph; tp.sslot = sslot; tp.caps = caps;
-    dim3 grid(nib * nchunk);
     KTimer &t1 = ktimer("regf_tile");
-    t1.start(s);
-    launch_tile(cfg, m_flag != 0, grid, tp, s);
-    NB6_HIP(hipGetLastError());
-    t1.stop(s);
-
-    FinParams fp;
-    fp.ib = d_i; fp.ni = ni; fp.ib_stride = ni_stride; fp.imap = imap;
-    fp.rec64 = js.rec64.as<double>(); fp.perm = js.perm.as<int>(); fp.jbase = nullptr; fp.nj = nj;
-    fp.nchunk = nchunk;
-    fp.part = part; fp.cnt = cnt;
-    fp.hslot = hslot; fp.caph = caph; fp.sslot = sslot; fp.caps = caps;
-    fp.lmax = lmax; fp.nnbmax = nnbeff;
-    fp.out = d_out; fp.out_stride = ni_stride; fp.list = d_list;
     KTimer &t2 = ktimer("regf_finalize");
-    t2.start(s);
-    if (ni <= FIN_WIDE_NI && nchunk >= 16) {
-        // few i: eight wavefronts per i
-        if (m_flag)
-            hipLaunchKernelGGL((regf_finalize_wide_kernel<true, 8>), dim3(ni), dim3(512), 0, s, fp);
-        else
-            hipLaunchKernelGGL((regf_finalize_wide_kernel<false, 8>), dim3(ni), dim3(512), 0, s, fp);
-    } else if (nchunk >= 32) {
-        // many chunks per i: four wavefronts per i
-        if (m_flag)
-            hipLaunchKernelGGL((regf_finalize_wide_kernel<true, 4>), dim3(ni), dim3(256), 0, s, fp);
-        else
-            hipLaunchKernelGGL((regf_finalize_wide_kernel<false, 4>), dim3(ni), dim3(256), 0, s, fp);
-    } else {
-        const dim3 fgrid((ni + 3) / 4);
-        if (m_flag)
-            hipLaunchKernelGGL(regf_finalize_kernel<true>, fgrid, dim3(256), 0, s, fp);
-        else
-            hipLaunchKernelGGL(regf_finalize_kernel<false>, fgrid, dim3(256), 0, s, fp);
+    for (int b0 = 0; b0 < ni; b0 += nbatch_i) {
+        const int nb = std::min(nbatch_i, ni - b0);
+        // slots [b0, b0 + nb): through imap the rows are absolute; without it the
+        // SoA inputs and outputs are offset by b0
+        const int *bmap = imap ? imap + b0 : nullptr;
+        const double *bi = imap ? d_i : d_i + b0;
+        double *bout = imap ? d_out : d_out + b0;
+        int *blist = imap ? d_list : d_list + (size_t)b0 * lmax;
+        const int bnib = (nb + nbi - 1) / nbi;
+
+        TileParams tp;
+        tp.ib = bi; tp.ni = nb; tp.ib_stride = ni_stride; tp.imap = bmap;
+        tp.rec32 = js.rec32.as<float4>(); tp.torg = js.torg.as<double>(); tp.nj = nj;
+        tp.tbox = js.tbox.as<float>();
+        tp.far_test = cfg.far_test;
+        tp.rec64 = js.rec64.as<double>();
+        tp.chunk_len = clen;
+        tp.nib = bnib;
+        tp.mmax = js.mmax;
+        tp.part = part; tp.cnt = cnt;
+        tp.hslot = hslot; tp.caph = caph; tp.sslot = sslot; tp.caps = caps;
+        dim3 grid(bnib * nchunk);
+        if (b0 == 0) t1.start(s);
+        launch_tile(cfg, m_flag != 0, grid, tp, s);
+        NB6_HIP(hipGetLastError());
+        if (b0 + nb >= ni) t1.stop(s);
+
+        FinParams fp;
+        fp.ib = bi; fp.ni = nb; fp.ib_stride = ni_stride; fp.imap = bmap;
+        fp.rec64 = js.rec64.as<double>(); fp.perm = js.perm.as<int>(); fp.jbase = nullptr;
+        fp.nj = nj;
+        fp.nchunk = nchunk;
+        fp.part = part; fp.cnt = cnt;
+        fp.hslot = hslot; fp.caph = caph; fp.sslot = sslot; fp.caps = caps;
+        fp.lmax = lmax; fp.nnbmax = nnbeff;
+        fp.out = bout; fp.out_stride = ni_stride; fp.list = blist;
+        if (b0 == 0) t2.start(s);
+        if (fin_form == 8) {
+            // few i: eight wavefronts per i
+            if (m_flag)
+                hipLaunchKernelGGL((regf_finalize_wide_kernel<true, 8>), dim3(nb), dim3(512), 0, s, fp);
+            else
+                hipLaunchKernelGGL((regf_finalize_wide_kernel<false, 8>), dim3(nb), dim3(512), 0, s, fp);
+        } else if (fin_form == 4) {
+            // many chunks per i: four wavefronts per i
+            if (m_flag)
+                hipLaunchKernelGGL((regf_finalize_wide_kernel<true, 4>), dim3(nb), dim3(256), 0, s, fp);
+            else
+                hipLaunchKernelGGL((regf_finalize_wide_kernel<false, 4>), dim3(nb), dim3(256), 0, s, fp);
+        } else {
+            const dim3 fgrid((nb + 3) / 4);
+            if (m_flag)
+                hipLaunchKernelGGL(regf_finalize_kernel<true>, fgrid, dim3(256), 0, s, fp);
+            else
+                hipLaunchKernelGGL(regf_finalize_kernel<false>, fgrid, dim3(256), 0, s, fp);
+        }
+        NB6_HIP(hipGetLastError());
+        if (b0 + nb >= ni) t2.stop(s);
     }
-    NB6_HIP(hipGetLastError());
-    t2.stop(s);
     return OK;
 }
 
@@ -1176,6 +1205,7 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
 }
 
 int regf_last_chunks() { return g_last_chunks; }
+int regf_last_batches() { return g_last_batches; }
 
 // Sweep work-group size (= LDS j-tile) 128/256/512, i-particles per lane 1/2 and the
 // minimum number of work-groups (occupancy through j-chunking) on the caller's own

@@ -55,6 +55,20 @@ def main():
     # overflow: nnbmax below the counts of part of the rows
     m, x, v, idx, h2, dtr = c2_inputs(n=4096, ni=128, counts_median=60.0, seed=20261212)
     reg_case("reg_overflow4k", m, x, v, h2, dtr, x[idx], v[idx], 120, 64, 0)
+    # off-centre clusters: the reference's FP32 absolute coordinates then flip neighbour
+    # decisions near the threshold (pairs inside its rounding band)
+    m, x, v, idx, h2, dtr = c2_inputs(n=16384, ni=512, seed=20261214)
+    x = x + np.array([30.0, -20.0, 10.0])
+    reg_case("reg_shifted16k", m, x, v, h2, dtr, x[idx], v[idx], 500, 450, 0)
+    mass = ics.kroupa_masses(16384, 20261215)
+    m, x, v = ics.king(16384, 20261216, w0=6.0, masses=mass)
+    m, x, v = ics.add_binaries(m, x, v, 0.5, 20261217)
+    x = x + np.array([3.0, 2.0, -4.0])
+    rng = np.random.Generator(np.random.PCG64(20261218))
+    idx = np.sort(rng.choice(m.size, size=512, replace=False))
+    rs = ics.rs_for_counts(x, idx, ics.lognormal_counts(512, 20261219, median=80.0))
+    dtr = np.exp(rng.uniform(np.log(2.0 ** -10), np.log(2.0 ** -4), 512))
+    reg_case("reg_kingbin16k", m, x, v, rs * rs, dtr, x[idx], v[idx], 600, 550, 0)
 
     # potential: Kroupa Plummer with 50% binaries, N=4096 (all i)
     mm = ics.kroupa_masses(4096, 31)

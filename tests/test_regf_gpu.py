@@ -177,6 +177,31 @@ def test_regf_autotune_keeps_results(gpunb):
     _compare(got, ref, "autotuned")
 
 
+@pytest.mark.parametrize("ni", [1500, 4096])
+def test_regf_slot_budget_batches_give_the_same_bits(gpunb, ni):
+    """A call whose work arrays exceed the slot budget runs in batches of i-blocks
+    (unsorted i-set at ni = 1500, Morton-sorted at 4096); counts, lists and forces
+    are bit-identical to the unbatched call."""
+    m, x, v, idx, h2, dtr = c2_inputs(n=65536, ni=ni)
+    n = m.size
+    lmax, nnbmax = 500, 450
+    L = lib.load()
+    g = gpunb(n)
+    g.send(m, x, v)
+    a = g.regf(h2, dtr, x[idx], v[idx], lmax, nnbmax)
+    assert L.nb6_last_regf_batches() == 1
+    L.nb6_set_regf_slot_budget(48)
+    try:
+        b = g.regf(h2, dtr, x[idx], v[idx], lmax, nnbmax)
+        nbatch = L.nb6_last_regf_batches()
+    finally:
+        L.nb6_set_regf_slot_budget(32768)
+    print(f"ni={ni}: {nbatch} batches")
+    assert nbatch > 1
+    for u, w in zip(a, b):
+        assert np.array_equal(u, w)
+
+
 def test_regf_overflow_exact_negative_counts(gpunb):
     """Counts above nnbmax come back as -count with the exact count
     (reg.avx.cpp:320-322); the caller's RS rescale (util_gpu.F:82-86) needs it."""
