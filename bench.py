@@ -562,13 +562,13 @@ def bench_regcor(steps, warmup, no_cpu):
         L.nb6_device_copy(saved[k].ptr, d.ptr, d.nbytes, None)
     d_rows = lib.DeviceArray(d_rows0.nbytes)
     d_addr = lib.DeviceArray.from_host((idx + 1).astype(np.int32))
-    d_cnt = lib.DeviceArray(5 * 4)
+    d_cnt = lib.DeviceArray(lib.NCOUNT * 4)
 
     def restore():
         for k in keep:
             L.nb6_device_copy(rc.dev[k].ptr, saved[k].ptr, saved[k].nbytes, None)
         L.nb6_device_copy(d_rows.ptr, d_rows0.ptr, d_rows0.nbytes, None)
-        L.nb6_memcpy_htod(d_cnt.ptr, np.zeros(5, np.int32).ctypes.data, 20)
+        L.nb6_memcpy_htod(d_cnt.ptr, np.zeros(lib.NCOUNT, np.int32).ctypes.data, 4 * lib.NCOUNT)
 
     for _ in range(warmup):
         restore()
@@ -580,7 +580,7 @@ def bench_regcor(steps, warmup, no_cpu):
         rc.correct(P, ni, d_addr, d_force, ni, d_rows, d_cnt)
         kms.append(L.nb6_kernel_ms(b"regcor"))
     L.nb6_sync()
-    cnt = d_cnt.to_host(np.int32, (5,))
+    cnt = d_cnt.to_host(np.int32, (lib.NCOUNT,))
     kt = float(np.mean(kms)) * 1e-3
     out = rc.download()
     rows_g = d_rows.to_host(np.int32, (ni, LMAX))

@@ -165,6 +165,15 @@ int    nb6_last_regf_chunks(void);
 int nb6_set_regf_slot_budget(int mib);
 int nb6_last_regf_batches(void);
 
+/* FP64 pass of the regular force (default on; no counterpart in the reference).
+ * The FP32 kernel also sums error scales of its FP32 partial sums; the finalize
+ * flags every row whose scale could put its acc or jerk more than 1e-6 (relative)
+ * from FP64, and those rows are recomputed over all j in FP64 (same neighbour
+ * test, so lists are unchanged).  nb6_last_regf_refined() returns the number of
+ * rows of the last call's FP64 pass (it synchronises the device). */
+int nb6_set_regf_refine(int on);
+int nb6_last_regf_refined(void);
+
 /* Ensemble of independent clusters in one device j-set (no counterpart in the
  * reference, which holds one j-set per library instance).
  *   nb6_ens_send_device: offsets[nclusters+1] (host) index the concatenated SoA
@@ -341,6 +350,9 @@ typedef struct nb6_regstate {
     /* written */
     double *xn, *xndot;               /* [nmax][3] XN, XNDOT */
     double *d0, *d0r, *d1r, *d2r, *d3r;  /* [nmax][3] D0, D0R, D1R, D2R, D3R */
+    /* read when KZ(37) = 1 (CHECKL); may be null otherwise */
+    const int *listv;                 /* LISTV: [0] = count, then 1-based high-velocity bodies */
+    const double *r;                  /* [nmax - N] R of KS pairs (c.m. bodies I > N) */
 } nb6_regstate;
 
 typedef struct nb6_regcor_params {
@@ -353,7 +365,9 @@ typedef struct nb6_regcor_params {
     int nnbopt, nnbmax;               /* NNBOPT, NNBMAX */
     int nb_flag, m_flag;              /* from KZ(39) (regcor_gpu.F:20-21) */
     int kz38;                         /* KZ(38): 0 none, 1 always, 2 if DF2 > 1e-4 FR2, 3 losses */
-    int pad;
+    int kz37;                         /* KZ(37): 1 = CHECKL's high-velocity test; > 1 counted
+                                         as unsupported (checkl.F:17-242 not included) */
+    double cmsep2;                    /* CMSEP2 (input.F:184), for c.m. bodies in CHECKL */
 } nb6_regcor_params;
 
 /* REGCOR_GPU for the ni particles d_addr[ni] (1-based I) of one block:
@@ -363,12 +377,16 @@ typedef struct nb6_regcor_params {
  *            in place into the final new list (count, 1-based ascending, no self) as
  *            util_gpu.F:101-110 and regcor_gpu.F:298-350 leave it; lmax must equal
  *            s->lmax and exceed nnbmax + 1;
- *   d_counters int[5] accumulated: NBVOID, NBFLUX, NBSMIN, NBCORR, unsupported
+ *   d_counters int[7] accumulated: NBVOID, NBFLUX, NBSMIN, NBCORR, unsupported
  *            (particles skipped: a negative or too long row, an index outside
- *            [1, nmax], I > N without h; FPCORR terms left out for KS components
- *            J < IFIRST).
- * Not included: external tides beyond TIDAL(4), FCLOUD, CHECKL (KZ(37)).  Particles
- * of one block must be distinct.  Asynchronous on `stream`. */
+ *            [1, nmax], I > N without h; particles with KZ(37) > 1), NBFAST (CHECKL's
+ *            high-velocity intruders, checkl.F:266-269), NCMDER (FPCORR terms of KS
+ *            components J < IFIRST, taken with their c.m. F and FDOT, fpcorr.F:46-61).
+ * CHECKL (regcor_gpu.F:240-244) is done for KZ(37) = 1 (checkl.F:243-340: at most
+ * one body of LISTV inside 2 RS, approaching with impact parameter below RS, joins
+ * the list; c.m. bodies take the resolved component terms of checkl.F:282-336).
+ * Not included: external tides beyond TIDAL(4), FCLOUD, the KZ(37) > 1 parts of
+ * CHECKL.  Particles of one block must be distinct.  Asynchronous on `stream`. */
 int nb6_regcor_device(const nb6_regcor_params *p, const nb6_regstate *s, int ni,
                       const int *d_addr, const double *d_force, int force_stride,
                       int *d_nlist, int *d_counters, void *stream);

@@ -638,6 +638,12 @@ int nb6_device_copy(void *dst, const void *src, unsigned long long bytes, void *
 }
 int nb6_last_regf_chunks(void) { return regf_last_chunks(); }
 int nb6_last_regf_batches(void) { return regf_last_batches(); }
+int nb6_set_regf_refine(int on) {
+    if (on != 0 && on != 1) { set_error(E_ARG, "nb6_set_regf_refine: on must be 0 or 1"); return E_ARG; }
+    regf_config().refine = on;
+    return OK;
+}
+int nb6_last_regf_refined(void) { return regf_last_refined(); }
 int nb6_set_regf_slot_budget(int mib) {
     if (mib < 1) { set_error(E_ARG, "nb6_set_regf_slot_budget: budget must be at least 1 MiB"); return E_ARG; }
     regf_config().slot_budget_mb = mib;

@@ -120,6 +120,7 @@ struct RegfConfig {
     int prec = 0;         // pair arithmetic: 0 FP32, 1 float-float, 2 FP64 (study variants)
     int slot_budget_mb = 32768;   // largest work-array footprint of one call; larger
                                   // calls run in batches of i (same results)
+    int refine = 1;       // FP64 pass over rows whose FP32 error scale is too large
 };
 RegfConfig &regf_config();
 // the configuration a call with ni i-particles runs with (threads resolved)
@@ -131,15 +132,18 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
                 int nnbmax, int m_flag, double *d_out, int *d_list, hipStream_t s);
 int regf_last_chunks();
 int regf_last_batches();
+int regf_last_refined();     // rows of the last call's FP64 pass (synchronises)
 void regf_release();
 
 // Scratch of one regular-force pipeline: FP32 partials, candidate counts and slots.
 struct RegfWork {
     DBuf part, cnt, hslot, sslot;
     DBuf ikey, ival, itmp;           // Morton sort of the i-set
+    DBuf fixrows, nfix, fixpart;     // FP64 pass: flagged rows, their count, chunk sums
     void release() {
         part.release(); cnt.release(); hslot.release(); sslot.release();
         ikey.release(); ival.release(); itmp.release();
+        fixrows.release(); nfix.release(); fixpart.release();
     }
 };
 

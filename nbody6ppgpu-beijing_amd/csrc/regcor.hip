@@ -32,7 +32,7 @@ namespace {
 constexpr double ONE3 = 1.0 / 3.0;
 constexpr double ONE12 = 1.0 / 12.0;
 
-enum { C_VOID, C_FLUX, C_SMIN, C_CORR, C_UNS, NCOUNT };
+enum { C_VOID, C_FLUX, C_SMIN, C_CORR, C_UNS, C_FAST, C_CMDER, NCOUNT };
 
 // force and first derivative of body j (0-based) on i, regcor_gpu.F:334-353
 __device__ __forceinline__ void pair_fj(const nb6_regstate &s, int j, const double *xi,
@@ -56,21 +56,33 @@ __device__ __forceinline__ void pair_fj(const nb6_regstate &s, int j, const doub
     }
 }
 
-// second and third derivative of the force of body j on i, fpcorr.F:24-74
+// second and third derivative of the force of body j on i, fpcorr.F:24-74; a KS
+// component j (jcm >= 0, 0-based c.m. body N + KVEC(J)) takes its own position and
+// velocity with the c.m. body's F and FDOT (fpcorr.F:46-61)
 __device__ __forceinline__ void fp_terms(const nb6_regstate &s, double time, int i, int j,
-                                         const double *xi, const double *vi, double *f2,
-                                         double *f3) {
+                                         int jcm, const double *xi, const double *vi,
+                                         double *f2, double *f3) {
 #pragma clang fp contract(off)
-    const double sj = time - s.t0[j];
-    const double s3 = 3.0 * sj;
     double A[12];
+    if (jcm >= 0) {
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-        const double fj = s.f[3 * (size_t)j + k], fdj = s.fdot[3 * (size_t)j + k];
-        A[k] = s.x[3 * (size_t)j + k] - xi[k];
-        A[k + 3] = (fdj * s3 + 2.0 * fj) * sj + s.x0dot[3 * (size_t)j + k] - vi[k];
-        A[k + 6] = 2.0 * (fdj * s3 + fj - s.f[3 * (size_t)i + k]);
-        A[k + 9] = 6.0 * (fdj - s.fdot[3 * (size_t)i + k]);
+        for (int k = 0; k < 3; k++) {
+            A[k] = s.x[3 * (size_t)j + k] - xi[k];
+            A[k + 3] = s.xdot[3 * (size_t)j + k] - vi[k];
+            A[k + 6] = 2.0 * (s.f[3 * (size_t)jcm + k] - s.f[3 * (size_t)i + k]);
+            A[k + 9] = 6.0 * (s.fdot[3 * (size_t)jcm + k] - s.fdot[3 * (size_t)i + k]);
+        }
+    } else {
+        const double sj = time - s.t0[j];
+        const double s3 = 3.0 * sj;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const double fj = s.f[3 * (size_t)j + k], fdj = s.fdot[3 * (size_t)j + k];
+            A[k] = s.x[3 * (size_t)j + k] - xi[k];
+            A[k + 3] = (fdj * s3 + 2.0 * fj) * sj + s.x0dot[3 * (size_t)j + k] - vi[k];
+            A[k + 6] = 2.0 * (fdj * s3 + fj - s.f[3 * (size_t)i + k]);
+            A[k + 9] = 6.0 * (fdj - s.fdot[3 * (size_t)i + k]);
+        }
     }
     const double a13 = 1.0 / (A[0] * A[0] + A[1] * A[1] + A[2] * A[2]);
     const double a14 = s.body[j] * a13 * sqrt(a13);
@@ -207,6 +219,128 @@ __device__ double rs_update(const nb6_regcor_params &P, const nb6_regstate &s, i
     return rs;
 }
 
+// Insert v into the ascending list nl[0..n) at its place (all lanes of the wavefront:
+// the tail moves up by one, 64 entries at a time from the top).
+__device__ __forceinline__ void list_insert(int *nl, int n, int v, int lane) {
+    int lo = 0, hi_ = n;
+    while (lo < hi_) {
+        const int mid = (lo + hi_) >> 1;
+        if (nl[mid] < v)
+            lo = mid + 1;
+        else
+            hi_ = mid;
+    }
+    for (int c0 = n - 1; c0 >= lo; c0 -= WAVE) {
+        const int idx = c0 - lane;
+        const bool act = idx >= lo;
+        int w = 0;
+        if (act) w = nl[idx];
+        __builtin_amdgcn_wave_barrier();
+        if (act) nl[idx + 1] = w;
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (lane == 0) nl[lo] = v;
+    __builtin_amdgcn_wave_barrier();
+}
+
+// CHECKL for KZ(37) = 1 (checkl.F:243-340): the first body J of LISTV (in LISTV order)
+// with |x_J - x_I|^2 <= 4 RS^2, approaching (dx.dv <= 0), impact parameter^2 <= RS^2
+// and not yet in the list joins it; its force and derivative leave FREG and FDR (the
+// gain loop moves them to the irregular side).  NNB > NNBMAX stops the search.  A c.m.
+// particle I takes the terms between its KS components and J (or J's components, when
+// J is a c.m. body inside its c.m. approximation), mass-weighted, unless J is a single
+// body outside CMSEP2 R^2.  The lanes test 64 LISTV entries at a time; every lane
+// then runs the chosen body with the same values.  Returns the new NNB.
+__device__ int checkl_fast(const nb6_regcor_params &P, const nb6_regstate &s, int I, int i,
+                           const double *xi, const double *vi, double rs2, double *freg,
+                           double *fdr, int *nl, int nnb, int *cnt, int lane) {
+#pragma clang fp contract(off)
+    const int nv = s.listv[0];
+    for (int b = 0; b < nv; b += WAVE) {
+        const int L = b + lane;
+        bool q = false;
+        int J = 0;
+        if (L < nv) {
+            J = s.listv[1 + L];
+            if (J != I && J >= 1 && J <= s.nmax) {
+                const size_t j3 = 3 * (size_t)(J - 1), i3 = 3 * (size_t)i;
+                const double a1 = s.x[j3] - xi[0], a2 = s.x[j3 + 1] - xi[1], a3 = s.x[j3 + 2] - xi[2];
+                const double rij2 = a1 * a1 + a2 * a2 + a3 * a3;
+                const double a4 = s.xdot[j3] - s.xdot[i3], a5 = s.xdot[j3 + 1] - s.xdot[i3 + 1],
+                             a6 = s.xdot[j3 + 2] - s.xdot[i3 + 2];
+                const double a7 = a1 * a4 + a2 * a5 + a3 * a6;
+                const double p2 = rij2 - a7 * a7 / (a4 * a4 + a5 * a5 + a6 * a6);
+                q = !(rij2 > 4.0 * rs2) && !(a7 > 0.0) && !(p2 > rs2) && !lds_contains(nl, nnb, J);
+            }
+        }
+        const unsigned long long m = __ballot(q);
+        if (!m) continue;
+        J = __shfl(J, __builtin_ctzll(m));
+        cnt[C_FAST]++;
+        if (nnb > P.nnbmax) return nnb;
+        const int j = J - 1;
+        bool resolved = false;
+        double rij2c = 0.0;
+        if (I > P.n) {
+            const double d0 = xi[0] - s.x[3 * (size_t)j], d1 = xi[1] - s.x[3 * (size_t)j + 1],
+                         d2 = xi[2] - s.x[3 * (size_t)j + 2];
+            rij2c = d0 * d0 + d1 * d1 + d2 * d2;
+            const double rp = s.r[I - P.n - 1];
+            resolved = !(rij2c > P.cmsep2 * (rp * rp) && J <= P.n);
+        }
+        if (!resolved) {                              // checkl.F:191-215
+            double dx[3], dv[3], dr2 = 0.0, drdv = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                dx[k] = s.x[3 * (size_t)j + k] - xi[k];
+                dv[k] = s.xdot[3 * (size_t)j + k] - vi[k];
+                dr2 = dr2 + dx[k] * dx[k];
+                drdv = drdv + dx[k] * dv[k];
+            }
+            const double dr2i = 1.0 / dr2;
+            const double dr3i = s.body[j] * dr2i * sqrt(dr2i);
+            drdv = 3.0 * drdv * dr2i;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                freg[k] = freg[k] - dx[k] * dr3i;
+                fdr[k] = fdr[k] - (dv[k] - dx[k] * drdv) * dr3i;
+            }
+        } else {                                      // checkl.F:282-336
+            const int i1 = 2 * (I - P.n) - 1;
+            int k1 = J, k2 = 0;
+            if (J > P.n) {
+                const double rq = s.r[J - P.n - 1];
+                if (rij2c < P.cmsep2 * (rq * rq)) {
+                    k1 = 2 * (J - P.n) - 1;
+                    k2 = k1 + 1;
+                }
+            }
+            for (int K = k1; K != 0; K = (K == k1 && k2) ? k2 : 0) {
+                for (int Lc = i1; Lc <= i1 + 1; Lc++) {
+                    const size_t k3 = 3 * (size_t)(K - 1), l3 = 3 * (size_t)(Lc - 1);
+                    const double a1 = s.x[k3] - s.x[l3], a2 = s.x[k3 + 1] - s.x[l3 + 1],
+                                 a3 = s.x[k3 + 2] - s.x[l3 + 2];
+                    const double rij2 = a1 * a1 + a2 * a2 + a3 * a3;
+                    const double dv0 = s.xdot[k3] - s.xdot[l3], dv1 = s.xdot[k3 + 1] - s.xdot[l3 + 1],
+                                 dv2 = s.xdot[k3 + 2] - s.xdot[l3 + 2];
+                    const double dr2i = 1.0 / rij2;
+                    const double dr3i = s.body[K - 1] * s.body[Lc - 1] * dr2i * sqrt(dr2i) / s.body[i];
+                    const double drdv = 3.0 * (a1 * dv0 + a2 * dv1 + a3 * dv2) * dr2i;
+                    freg[0] = freg[0] - a1 * dr3i;
+                    freg[1] = freg[1] - a2 * dr3i;
+                    freg[2] = freg[2] - a3 * dr3i;
+                    fdr[0] = fdr[0] - (dv0 - a1 * drdv) * dr3i;
+                    fdr[1] = fdr[1] - (dv1 - a2 * drdv) * dr3i;
+                    fdr[2] = fdr[2] - (dv2 - a3 * drdv) * dr3i;
+                }
+            }
+        }
+        list_insert(nl, nnb, J, lane);
+        return nnb + 1;
+    }
+    return nnb;
+}
+
 // Hand-over from the list kernel to the finishing kernel, SoA over the block:
 // doubles DFIRR, DFD, F2DOT and F3DOT sums, FREG, FDR (after retention); ints flags
 // (1: corrected, 2: FPCORR applies), NNB before retention, NNB0 and, for NNB <= 3,
@@ -285,6 +419,12 @@ __device__ void regcor_wave(const nb6_regcor_params &P, const nb6_regstate &s, i
     }
     const int nnb_pre = nnb;
     if (lane < 3 && lane < nnb) hi[(HI_J3 + lane) * ni] = nl[lane];
+
+    // CHECKL, regcor_gpu.F:240-244 (after the RS update, which the finishing kernel does
+    // with the NNB and list from before this point)
+    if (nnb > 0 && P.kz37 > 1) cnt[C_UNS]++;
+    if (nnb > 0 && P.kz37 == 1 && s.listv && s.listv[0] > 0)
+        nnb = checkl_fast(P, s, I, i, xi, vi, rs2, freg, fdr, nl, nnb, cnt, lane);
 
     // losses (ascending), gains and JMIN, regcor_gpu.F:252-296
     int nloss = 0, nbgain = 0;
@@ -429,15 +569,13 @@ __device__ void regcor_wave(const nb6_regcor_params &P, const nb6_regstate &s, i
             if (j1) {
                 pair_fj(s, j1 - 1, xi, vi, tv, tv + 3);
                 if (fps) {
-#pragma unroll
-                    for (int k = 6; k < NTERM; k++) tv[k] = 0.0;
-                    if (j1 < P.ifirst)
-                        ks = true;         // KS component: not supported, term left at zero
-                    else
-                        fp_terms(s, time, i, j1 - 1, xi, vi, tv + 6, tv + 9);
+                    // a KS component J < IFIRST: c.m. body N + KVEC(J), KVEC(J) = (J + 1) / 2
+                    ks = j1 < P.ifirst;
+                    const int jcm = ks ? P.n + (j1 + 1) / 2 - 1 : -1;
+                    fp_terms(s, time, i, j1 - 1, jcm, xi, vi, tv + 6, tv + 9);
                 }
             }
-            cnt[C_UNS] += __popcll(__ballot(ks));
+            cnt[C_CMDER] += __popcll(__ballot(ks));
             // the compacted terms pass through LDS 32 at a time, in list order
             const int nt = __popcll(m);
             for (int h = 0; h < nt; h += TCH) {
@@ -642,7 +780,7 @@ __global__ __launch_bounds__(256) void regcor_lists_kernel(nb6_regcor_params P, 
     if (threadIdx.x < NCOUNT) red[threadIdx.x] = 0;
     __syncthreads();
     const int k = blockIdx.x * (blockDim.x / WAVE) + w;
-    int cnt[NCOUNT] = {0, 0, 0, 0, 0};
+    int cnt[NCOUNT] = {};
     if (k < ni) {
         unsigned char *base = lds + (size_t)w * wave_lds_bytes(s.lmax);
         double *T = reinterpret_cast<double *>(base);
@@ -686,8 +824,13 @@ int regcor_device(const nb6_regcor_params *p, const nb6_regstate *s, int ni, con
         return E_ARG;
     }
     if (force_stride < ni) { set_error(E_ARG, "nb6_regcor_device: force_stride < ni"); return E_ARG; }
-    if (p->ifirst < 1 || p->n < 0 || p->kz38 < 0 || p->kz38 > 3 || !(p->time == p->time)) {
-        set_error(E_ARG, "nb6_regcor_device: bad IFIRST, N, KZ(38) or TIME");
+    if (p->ifirst < 1 || p->n < 0 || p->kz38 < 0 || p->kz38 > 3 || p->kz37 < 0 ||
+        !(p->time == p->time)) {
+        set_error(E_ARG, "nb6_regcor_device: bad IFIRST, N, KZ(37), KZ(38) or TIME");
+        return E_ARG;
+    }
+    if (p->kz37 == 1 && s->listv && p->n < s->nmax && !s->r) {
+        set_error(E_ARG, "nb6_regcor_device: KZ(37) = 1 with c.m. bodies needs the KS separations r");
         return E_ARG;
     }
     const void *need[] = {s->x, s->xdot, s->body, s->x0, s->x0dot, s->f, s->fdot, s->t0,

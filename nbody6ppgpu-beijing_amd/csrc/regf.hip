@@ -51,6 +51,9 @@ namespace {
 
 constexpr int RF_FLUSH = 64;          // j per FP32 partial sum
 constexpr int NACC = 7;               // FP32 partials: acc x y z, pot, jerk x y z
+// FP64 partial rows per (i, chunk): acc x y z, jerk x y z, pot, then the three error
+// scales of the FP32 sums (see tile_work): acc flush L1, jerk flush L1, fast records
+constexpr int NPART = 10;
 constexpr int CAP_SOFT = 1024;        // soft (near-field) slots per (i, chunk)
 constexpr int BAND_SLOTS = 16;        // of which kept for band pairs of overflowing rows
 constexpr int NB_SORT_MAX = 2048;     // largest neighbour row the finalize sorts
@@ -80,7 +83,7 @@ struct TileParams {
     int chunk_len;                   // multiple of the block size T and of JTILE
     int nib;                         // i-blocks of the grid (regf_tile_kernel)
     double mmax;
-    double *__restrict__ part;       // [nchunk][7][ni]
+    double *__restrict__ part;       // [nchunk][NPART][ni]
     int *__restrict__ cnt;           // [nchunk][2][ni]: hard, soft
     int *__restrict__ hslot;         // [(chunk*ni + i) * caph + k]
     int caph;
@@ -226,7 +229,7 @@ template <bool MFLAG, bool TEST>
 __device__ __forceinline__ void pair_step(const TileParams &p, const IC &ck, const float4 a,
                                           const float4 b, int jg, int j1, int &nhk, int &nsk,
                                           int *__restrict__ hpk, int *__restrict__ spk,
-                                          float *__restrict__ acc) {
+                                          float *__restrict__ acc, float &efast) {
     {
         const PairBase q = pair_base(ck, a, b);
         // v_rsq_f32 plus one Newton-Raphson step
@@ -291,6 +294,10 @@ __device__ __forceinline__ void pair_step(const TileParams &p, const IC &ck, con
         acc[4] = fmaf(mr3, fmaf(al, q.dx, q.du), acc[4]);
         acc[5] = fmaf(mr3, fmaf(al, q.dy, q.dv), acc[5]);
         acc[6] = fmaf(mr3, fmaf(al, q.dz, q.dw), acc[6]);
+        // error scale of the jerk terms of fast records (b.w = the record's speed
+        // excess, zero for most): their terms can cancel within a flush (the two
+        // members of a tight binary), where the flush sums would not show them
+        efast = fmaf(mr3, b.w, efast);
     }
 }
 
@@ -306,6 +313,7 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
     int *hp[IPL], *sp[IPL];
     float a32[IPL][NACC];
     double a64[IPL][7];
+    float err[IPL][3];               // error scales of the FP32 sums (FP32 is enough)
 
 #pragma unroll
     for (int k = 0; k < IPL; k++) {
@@ -324,6 +332,7 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
         for (int f = 0; f < NACC; f++) a32[k][f] = 0.f;
 #pragma unroll
         for (int f = 0; f < 7; f++) a64[k][f] = 0.0;
+        err[k][0] = err[k][1] = err[k][2] = 0.f;
     }
 
     const int ntile = (j1 - j0 + T - 1) / T;
@@ -370,7 +379,7 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
 #pragma unroll
                     for (int k = 0; k < IPL; k++)
                         pair_step<MFLAG, false>(p, c[k], a, b, jt + jj, j1, nh[k], ns[k], hp[k],
-                                                sp[k], a32[k]);
+                                                sp[k], a32[k], err[k][2]);
                 }
             } else {
 #pragma unroll 2
@@ -379,7 +388,7 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
 #pragma unroll
                     for (int k = 0; k < IPL; k++)
                         pair_step<MFLAG, true>(p, c[k], a, b, jt + jj, j1, nh[k], ns[k], hp[k],
-                                               sp[k], a32[k]);
+                                               sp[k], a32[k], err[k][2]);
                 }
             }
             // flush FP32 partials into FP64
@@ -392,6 +401,9 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
                 a64[k][4] += (double)a32[k][5];
                 a64[k][5] += (double)a32[k][6];
                 a64[k][6] += (double)a32[k][3];
+                // L1 norms of the flushed FP32 partials: the scale of their rounding
+                err[k][0] += fabsf(a32[k][0]) + fabsf(a32[k][1]) + fabsf(a32[k][2]);
+                err[k][1] += fabsf(a32[k][4]) + fabsf(a32[k][5]) + fabsf(a32[k][6]);
 #pragma unroll
                 for (int f = 0; f < NACC; f++) a32[k][f] = 0.f;
             }
@@ -405,9 +417,11 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
     for (int k = 0; k < IPL; k++) {
         const int i = ibase + k * T + tid;
         if (i < iend) {
-            double *pp = p.part + (size_t)chunk * 7 * p.ni + i;
+            double *pp = p.part + (size_t)chunk * NPART * p.ni + i;
 #pragma unroll
             for (int f = 0; f < 7; f++) pp[(size_t)f * p.ni] = a64[k][f];
+#pragma unroll
+            for (int f = 0; f < 3; f++) pp[(size_t)(7 + f) * p.ni] = (double)err[k][f];
             p.cnt[((size_t)chunk * 2 + 0) * p.ni + i] = nh[k];
             p.cnt[((size_t)chunk * 2 + 1) * p.ni + i] = ns[k];
         }
@@ -629,9 +643,11 @@ __global__ __launch_bounds__(256) void regf_var_kernel(TileParams p) {
         }
     }
     if (in) {
-        double *pp = p.part + (size_t)chunk * 7 * p.ni + i;
+        double *pp = p.part + (size_t)chunk * NPART * p.ni + i;
 #pragma unroll
         for (int q = 0; q < 7; q++) pp[(size_t)q * p.ni] = a64[q];
+#pragma unroll
+        for (int q = 7; q < NPART; q++) pp[(size_t)q * p.ni] = 0.0;   // no FP32 sums
         p.cnt[((size_t)chunk * 2 + 0) * p.ni + i] = nh;
         p.cnt[((size_t)chunk * 2 + 1) * p.ni + i] = ns;
     }
@@ -656,7 +672,31 @@ struct FinParams {
     double *__restrict__ out;        // SoA [7][out_stride]
     int out_stride;
     int *__restrict__ list;          // [ni][lmax]
+    int refine;                      // flag rows whose FP32 error scale is too large
+    int row_base;                    // added to r for fix_rows (batches without imap)
+    int *__restrict__ fix_rows;      // flagged caller rows (refine)
+    int *__restrict__ nfix;          // their count (device)
 };
+
+// Error scales of a row's FP32 sums -> does the row need the FP64 pass?  u = 2^-24.
+// e[0], e[1]: sums over the 64-j flushes of the L1 norms of the FP32 acc and jerk
+// partials; e[2]: sum over fast records of m/r^3 times their speed excess.  The
+// FP32 rounding of the sums is about u times these (random signs make it smaller);
+// a row is recomputed in FP64 when the scale, times the factors below, exceeds
+// REFINE_REL of its own |acc| or |jerk|.  Rows whose sums are NaN (band pairs left
+// without a slot) are recomputed too.
+constexpr double REFINE_REL = 1.0e-6;
+constexpr double REFINE_KA = 0.5;
+constexpr double REFINE_KJ = 0.5;
+constexpr double REFINE_KF = 1.0;
+__device__ __forceinline__ bool needs_fp64(const double *s, const double *e, bool nan_row) {
+    if (nan_row) return true;
+    const double a = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    const double j = sqrt(s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
+    const double ea = REFINE_KA * U24 * e[0];
+    const double ej = U24 * (REFINE_KJ * e[1] + REFINE_KF * e[2]);
+    return !(ea <= REFINE_REL * a) || !(ej <= REFINE_REL * j);
+}
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -681,10 +721,8 @@ struct IF64 {
 // FP64 neighbour test of sorted j against i (formula order = the oracle's) and,
 // for a non-neighbour, its exact force added to s[].  test=false: force only.
 template <bool MFLAG>
-__device__ __forceinline__ bool classify(const double *__restrict__ rec, const IF64 &I, bool test,
-                                         double *s) {
-    const double4 r0 = reinterpret_cast<const double4 *>(rec)[0];   // x y z m
-    const double4 r1 = reinterpret_cast<const double4 *>(rec)[1];   // vx vy vz 0
+__device__ __forceinline__ bool pair64(const double4 &r0, const double4 &r1, const IF64 &I,
+                                       bool test, double *s) {
     double dx, dy, dz, du, dv, dw, r2, rmin;
     {
 #pragma clang fp contract(off)
@@ -720,6 +758,14 @@ __device__ __forceinline__ bool classify(const double *__restrict__ rec, const I
         s[6] += mr1;
     }
     return isnb;
+}
+
+template <bool MFLAG>
+__device__ __forceinline__ bool classify(const double *__restrict__ rec, const IF64 &I, bool test,
+                                         double *s) {
+    const double4 r0 = reinterpret_cast<const double4 *>(rec)[0];   // x y z m
+    const double4 r1 = reinterpret_cast<const double4 *>(rec)[1];   // vx vy vz 0
+    return pair64<MFLAG>(r0, r1, I, test, s);
 }
 
 // The FP64 pass over one i's candidates, in the chunks c = c0 + cstep (64 p + lane),
@@ -802,14 +848,18 @@ __global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
     if (r < 0) return;
     int *buf = nbuf[wid];
 
-    // 1. FP64 sum of the chunk partials
-    double s[7];
+    // 1. FP64 sum of the chunk partials (and of the error scales)
+    double s[7], e[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < 7; q++) s[q] = 0.0;
     for (int c = lane; c < f.nchunk; c += 64) {
-        const double *pp = f.part + (size_t)c * 7 * f.ni + i;
+        const double *pp = f.part + (size_t)c * NPART * f.ni + i;
 #pragma unroll
         for (int q = 0; q < 7; q++) s[q] += pp[(size_t)q * f.ni];
+        if (f.refine) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) e[q] += pp[(size_t)(7 + q) * f.ni];
+        }
     }
 
     // 2. FP64 work on the candidates
@@ -829,6 +879,10 @@ __global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
 
 #pragma unroll
     for (int q = 0; q < 7; q++) s[q] = wave_sum(s[q]);
+    if (f.refine) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) e[q] = wave_sum(e[q]);
+    }
     int *row = f.list + (size_t)r * f.lmax;
     const int tot = nnb + dropped;
     // a row whose hard slots overflowed reports -count, as any overflow does, even in
@@ -847,6 +901,8 @@ __global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
         o[3 * os] = s[3]; o[4 * os] = s[4]; o[5 * os] = s[5];
         o[6 * os] = s[6];
         row[0] = ovf ? -tot : tot;
+        if (f.refine && needs_fp64(s, e, unres > 0))
+            f.fix_rows[atomicAdd(f.nfix, 1)] = r + f.row_base;
     }
     if (!ovf && nnb > 0) {
         // ascending row: the rank of each (distinct) index among the row
@@ -870,7 +926,7 @@ __global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
 template <bool MFLAG, int W>
 __global__ __launch_bounds__(64 * W) void regf_finalize_wide_kernel(FinParams f) {
     __shared__ int buf[NB_SORT_MAX];
-    __shared__ double red[W][7];
+    __shared__ double red[W][10];
     __shared__ int cnts[W][3];
     __shared__ int nshared;
     const int lane = threadIdx.x & 63;
@@ -881,13 +937,17 @@ __global__ __launch_bounds__(64 * W) void regf_finalize_wide_kernel(FinParams f)
     if (threadIdx.x == 0) nshared = 0;
     __syncthreads();
 
-    double s[7];
+    double s[10];
 #pragma unroll
-    for (int q = 0; q < 7; q++) s[q] = 0.0;
+    for (int q = 0; q < 10; q++) s[q] = 0.0;
     for (int c = w + W * lane; c < f.nchunk; c += 64 * W) {
-        const double *pp = f.part + (size_t)c * 7 * f.ni + i;
+        const double *pp = f.part + (size_t)c * NPART * f.ni + i;
 #pragma unroll
         for (int q = 0; q < 7; q++) s[q] += pp[(size_t)q * f.ni];
+        if (f.refine) {
+#pragma unroll
+            for (int q = 7; q < 10; q++) s[q] += pp[(size_t)q * f.ni];
+        }
     }
     const int is = f.ib_stride;
     IF64 I;
@@ -900,10 +960,10 @@ __global__ __launch_bounds__(64 * W) void regf_finalize_wide_kernel(FinParams f)
     fin_candidates<MFLAG, true>(f, I, i, jb, w, W, lane, s, buf, &nshared, nnb, dropped, unres,
                                 dropped_any);
 #pragma unroll
-    for (int q = 0; q < 7; q++) s[q] = wave_sum(s[q]);
+    for (int q = 0; q < 10; q++) s[q] = wave_sum(s[q]);
     if (lane == 0) {
 #pragma unroll
-        for (int q = 0; q < 7; q++) red[w][q] = s[q];
+        for (int q = 0; q < 10; q++) red[w][q] = s[q];
         cnts[w][0] = dropped;
         cnts[w][1] = unres;
         cnts[w][2] = dropped_any ? 1 : 0;
@@ -925,14 +985,18 @@ __global__ __launch_bounds__(64 * W) void regf_finalize_wide_kernel(FinParams f)
     if (threadIdx.x == 0) {
         double *o = f.out + r;
         const int os = f.out_stride;
+        double v[10];
 #pragma unroll
-        for (int q = 0; q < 7; q++) {
-            double v = 0.0;   // a fixed order
+        for (int q = 0; q < 10; q++) {
+            v[q] = 0.0;   // a fixed order
 #pragma unroll
-            for (int u = 0; u < W; u++) v += red[u][q];
-            o[q * os] = unr > 0 ? __builtin_nan("") : v;
+            for (int u = 0; u < W; u++) v[q] += red[u][q];
         }
+#pragma unroll
+        for (int q = 0; q < 7; q++) o[q * os] = unr > 0 ? __builtin_nan("") : v[q];
         row[0] = ovf ? -tot : tot;
+        if (f.refine && needs_fp64(v, v + 7, unr > 0))
+            f.fix_rows[atomicAdd(f.nfix, 1)] = r + f.row_base;
     }
     if (!ovf && nall > 0) {
         for (int k = threadIdx.x; k < nall; k += 64 * W) {
@@ -944,9 +1008,100 @@ __global__ __launch_bounds__(64 * W) void regf_finalize_wide_kernel(FinParams f)
     }
 }
 
+// ------------------------------------------------------------------ FP64 pass
+// Rows flagged by the finalize are recomputed over every j in FP64: the same pair
+// function as the finalize's (pair64: the oracle's neighbour test, so exactly the
+// row's neighbours are left out), j tiles staged in LDS and shared by the 256 rows of
+// a work-group.  A persistent grid walks (row block, j-chunk) items; the number of
+// chunks follows from the number of flagged rows alone, and each row's chunk sums are
+// added in chunk order, so a row's result does not depend on which slot it got.
+constexpr int FIX_ITEMS = 2048;       // work items aimed at per launch
+
+struct FixParams {
+    const double *__restrict__ ib;   // SoA [8][ib_stride] (caller rows)
+    int ib_stride;
+    const int *__restrict__ rows;    // flagged caller rows
+    const int *__restrict__ nfix;
+    const double *__restrict__ rec64;
+    int j0, j1;                      // sorted j range
+    double *__restrict__ part;       // [nch][7][nfix]
+    double *__restrict__ out;        // SoA [7][out_stride]
+    int out_stride;
+};
+
+__device__ __forceinline__ void fix_chunks(int nfix, int nj, int &nch, int &tpc) {
+    const int nib = (nfix + 255) / 256;
+    const int ntile = (nj + JTILE - 1) / JTILE;
+    nch = max(1, min(FIX_ITEMS / nib, ntile));
+    tpc = (ntile + nch - 1) / nch;          // tiles per chunk
+    nch = (ntile + tpc - 1) / tpc;
+}
+
+template <bool MFLAG>
+__global__ __launch_bounds__(256) void regf_fix_kernel(FixParams f) {
+    __shared__ double4 sA[JTILE], sB[JTILE];
+    const int nfix = *f.nfix;
+    if (nfix == 0) return;
+    const int nj = f.j1 - f.j0;
+    int nch, tpc;
+    fix_chunks(nfix, nj, nch, tpc);
+    const int nib = (nfix + 255) / 256;
+    const int items = nib * nch;
+    for (int it = blockIdx.x; it < items; it += gridDim.x) {   // uniform per work-group
+        const int b = it % nib, ch = it / nib;
+        const int k = b * 256 + threadIdx.x;
+        const bool ok = k < nfix;
+        const int r = ok ? f.rows[k] : 0;
+        const int is = f.ib_stride;
+        IF64 I;
+        I.x = f.ib[0 * is + r]; I.y = f.ib[1 * is + r]; I.z = f.ib[2 * is + r];
+        I.u = f.ib[3 * is + r]; I.v = f.ib[4 * is + r]; I.w = f.ib[5 * is + r];
+        I.h2 = f.ib[6 * is + r]; I.dtr = f.ib[7 * is + r];
+        double sum[7] = {0, 0, 0, 0, 0, 0, 0};
+        const int ja = f.j0 + ch * tpc * JTILE;
+        const int jb = min(f.j1, ja + tpc * JTILE);
+        for (int jt = ja; jt < jb; jt += JTILE) {
+            __syncthreads();
+            const int j = jt + threadIdx.x;
+            if (j < jb) {
+                sA[threadIdx.x] = reinterpret_cast<const double4 *>(f.rec64)[2 * (size_t)j];
+                sB[threadIdx.x] = reinterpret_cast<const double4 *>(f.rec64)[2 * (size_t)j + 1];
+            }
+            __syncthreads();
+            const int jn = min(JTILE, jb - jt);
+            if (ok)
+                for (int jj = 0; jj < jn; jj++) (void)pair64<MFLAG>(sA[jj], sB[jj], I, true, sum);
+        }
+        if (ok) {
+#pragma unroll
+            for (int q = 0; q < 7; q++) f.part[((size_t)ch * 7 + q) * nfix + k] = sum[q];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void regf_fix_reduce_kernel(FixParams f) {
+    const int nfix = *f.nfix;
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= nfix) return;
+    int nch, tpc;
+    fix_chunks(nfix, f.j1 - f.j0, nch, tpc);
+    double sum[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int ch = 0; ch < nch; ch++) {
+#pragma unroll
+        for (int q = 0; q < 7; q++) sum[q] += f.part[((size_t)ch * 7 + q) * nfix + k];
+    }
+    const int r = f.rows[k];
+#pragma unroll
+    for (int q = 0; q < 7; q++) f.out[(size_t)q * f.out_stride + r] = sum[q];
+}
+
+__global__ void zero_int_kernel(int *p) { *p = 0; }
+
 RegfWork g_work;
 int g_last_chunks = 0;
 int g_last_batches = 0;
+int *g_nfix = nullptr;       // device count of the rows of the last call's FP64 pass
+bool g_last_refine = false;
 
 template <int T, int IPL>
 void launch_tile_t(bool mflag, dim3 grid, const TileParams &tp, hipStream_t s) {
@@ -1058,7 +1213,7 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
     // Work arrays hold one row per (i, chunk): FP64 partials, counts and the hard and
     // soft candidate slots.  Calls whose rows would exceed the slot budget run in
     // batches of i (whole i-blocks); chunks and results do not change.
-    const size_t row_bytes = 7 * sizeof(double) + 2 * sizeof(int) +
+    const size_t row_bytes = NPART * sizeof(double) + 2 * sizeof(int) +
                              ((size_t)caph + (size_t)caps) * sizeof(int);
     const size_t budget = (size_t)std::max(cfg.slot_budget_mb, 1) << 20;
     int nbatch_i = ni;
@@ -1068,12 +1223,21 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
         nbatch_i = std::min(nbatch_i, ni);
     }
     const size_t rows = (size_t)nchunk * nbatch_i;
-    double *part = static_cast<double *>(g_work.part.get(rows * 7 * sizeof(double)));
+    double *part = static_cast<double *>(g_work.part.get(rows * NPART * sizeof(double)));
     int *cnt = static_cast<int *>(g_work.cnt.get(rows * 2 * sizeof(int)));
     int *hslot = static_cast<int *>(g_work.hslot.get(rows * caph * sizeof(int)));
     int *sslot = static_cast<int *>(g_work.sslot.get(rows * caps * sizeof(int)));
     g_last_chunks = nchunk;
     g_last_batches = (ni + nbatch_i - 1) / nbatch_i;
+    // rows the finalize flags for the FP64 pass (FP32 production kernel only)
+    const bool refine = cfg.refine != 0 && cfg.prec == 0;
+    int *fix_rows = nullptr;
+    if (refine) {
+        fix_rows = static_cast<int *>(g_work.fixrows.get((size_t)ni * sizeof(int)));
+        g_nfix = static_cast<int *>(g_work.nfix.get(sizeof(int)));
+        hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, s, g_nfix);
+    }
+    g_last_refine = refine;
 
     // Morton-ordered i: a wavefront's particles are neighbours in space, so their
     // candidates fall on the same j and most tiles are far from all of them
@@ -1119,6 +1283,10 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
         fp.hslot = hslot; fp.caph = caph; fp.sslot = sslot; fp.caps = caps;
         fp.lmax = lmax; fp.nnbmax = nnbeff;
         fp.out = bout; fp.out_stride = ni_stride; fp.list = blist;
+        fp.refine = refine ? 1 : 0;
+        fp.row_base = imap ? 0 : b0;
+        fp.fix_rows = fix_rows;
+        fp.nfix = g_nfix;
         if (b0 == 0) t2.start(s);
         if (fin_form == 8) {
             // few i: eight wavefronts per i
@@ -1142,6 +1310,26 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
         NB6_HIP(hipGetLastError());
         if (b0 + nb >= ni) t2.stop(s);
     }
+    if (refine) {
+        // FP64 pass over the flagged rows (their count stays on the device)
+        const size_t cap = std::max((size_t)FIX_ITEMS * 256, (size_t)ni);
+        FixParams xp;
+        xp.ib = d_i; xp.ib_stride = ni_stride;
+        xp.rows = fix_rows; xp.nfix = g_nfix;
+        xp.rec64 = js.rec64.as<double>(); xp.j0 = 0; xp.j1 = nj;
+        xp.part = static_cast<double *>(g_work.fixpart.get(cap * 7 * sizeof(double)));
+        xp.out = d_out; xp.out_stride = ni_stride;
+        KTimer &t3 = ktimer("regf_fix");
+        t3.start(s);
+        if (m_flag)
+            hipLaunchKernelGGL(regf_fix_kernel<true>, dim3(FIX_ITEMS), dim3(256), 0, s, xp);
+        else
+            hipLaunchKernelGGL(regf_fix_kernel<false>, dim3(FIX_ITEMS), dim3(256), 0, s, xp);
+        NB6_HIP(hipGetLastError());
+        hipLaunchKernelGGL(regf_fix_reduce_kernel, dim3((ni + 255) / 256), dim3(256), 0, s, xp);
+        NB6_HIP(hipGetLastError());
+        t3.stop(s);
+    }
     return OK;
 }
 
@@ -1163,12 +1351,12 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     NB6_HIP(hipStreamIsCapturing(s, &cs));
     const bool capturing = cs != hipStreamCaptureStatusNone;
-    if (capturing && (w.part.bytes < rows * 7 * sizeof(double) || w.hslot.bytes < rows * caph * sizeof(int) ||
+    if (capturing && (w.part.bytes < rows * NPART * sizeof(double) || w.hslot.bytes < rows * caph * sizeof(int) ||
                       w.sslot.bytes < rows * caps * sizeof(int) || w.cnt.bytes < rows * 2 * sizeof(int))) {
         set_error(E_STATE, "regf_items_launch: work buffers must be sized before stream capture");
         return E_STATE;
     }
-    double *part = static_cast<double *>(w.part.get(rows * 7 * sizeof(double)));
+    double *part = static_cast<double *>(w.part.get(rows * NPART * sizeof(double)));
     int *cnt = static_cast<int *>(w.cnt.get(rows * 2 * sizeof(int)));
     int *hslot = static_cast<int *>(w.hslot.get(rows * caph * sizeof(int)));
     int *sslot = static_cast<int *>(w.sslot.get(rows * caps * sizeof(int)));
@@ -1195,6 +1383,7 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
     fp.hslot = hslot; fp.caph = caph; fp.sslot = sslot; fp.caps = caps;
     fp.lmax = lmax; fp.nnbmax = nnbeff;
     fp.out = d_out; fp.out_stride = stride; fp.list = d_list;
+    fp.refine = 0; fp.row_base = 0; fp.fix_rows = nullptr; fp.nfix = nullptr;
     const dim3 fgrid((nslots + 3) / 4);
     if (m_flag)
         hipLaunchKernelGGL(regf_finalize_kernel<true>, fgrid, dim3(256), 0, s, fp);
@@ -1206,6 +1395,13 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
 
 int regf_last_chunks() { return g_last_chunks; }
 int regf_last_batches() { return g_last_batches; }
+int regf_last_refined() {
+    if (!g_last_refine || !g_nfix) return 0;
+    int h = 0;
+    NB6_HIP(hipDeviceSynchronize());   // the call may have run on any stream
+    NB6_HIP(hipMemcpy(&h, g_nfix, sizeof(int), hipMemcpyDeviceToHost));
+    return h;
+}
 
 // Sweep work-group size (= LDS j-tile) 128/256/512, i-particles per lane 1/2 and the
 // minimum number of work-groups (occupancy through j-chunking) on the caller's own
@@ -1240,6 +1436,10 @@ int regf_autotune(const JSet &js, int ni, int ni_stride, const double *d_i, int 
     return OK;
 }
 
-void regf_release() { g_work.release(); }
+void regf_release() {
+    g_work.release();
+    g_nfix = nullptr;
+    g_last_refine = false;
+}
 
 }  // namespace nb6

@@ -60,6 +60,8 @@ def load():
     L.nb6_last_regf_chunks.restype = ctypes.c_int
     L.nb6_last_regf_batches.restype = ctypes.c_int
     L.nb6_set_regf_slot_budget.argtypes = [ctypes.c_int]
+    L.nb6_set_regf_refine.argtypes = [ctypes.c_int]
+    L.nb6_last_regf_refined.restype = ctypes.c_int
     L.nb6_dev_alloc.restype = _vp
     L.nb6_dev_alloc.argtypes = [ctypes.c_ulonglong]
     L.nb6_dev_free.argtypes = [_vp]
@@ -104,16 +106,17 @@ def load():
 
 
 # --------------------------------------------------------------- regular corrector
+NCOUNT = 7   # nb6_regcor_device counters: NBVOID NBFLUX NBSMIN NBCORR unsupported NBFAST NCMDER
 # Field order of nb6_regstate / nb6_regcor_params (include/nb6gpu.h).
 REG3 = ("x", "xdot", "x0", "x0dot", "f", "fdot", "fi", "fidot", "fr", "frdot", "d2", "d3",
         "xn", "xndot", "d0", "d0r", "d1r", "d2r", "d3r")      # [nmax][3] on the device
 REG1 = ("body", "t0", "step", "timenw", "t0r", "stepr", "rs")  # [nmax]
 _STATE_PTRS = ("x", "xdot", "body", "x0", "x0dot", "f", "fdot", "t0", "h", "step", "timenw",
                "fi", "fidot", "fr", "frdot", "d2", "d3", "t0r", "stepr", "rs", "list", "xn",
-               "xndot", "d0", "d0r", "d1r", "d2r", "d3r")
+               "xndot", "d0", "d0r", "d1r", "d2r", "d3r", "listv", "r")
 _PARAM_D = ("time", "dtmin", "smin", "smax", "etar", "rscale", "rc2", "rsfac", "znbmin",
             "bodym", "tidal4")
-_PARAM_I = ("n", "ifirst", "nnbopt", "nnbmax", "nb_flag", "m_flag", "kz38", "pad")
+_PARAM_I = ("n", "ifirst", "nnbopt", "nnbmax", "nb_flag", "m_flag", "kz38", "kz37")
 
 
 class RegState(ctypes.Structure):
@@ -123,7 +126,7 @@ class RegState(ctypes.Structure):
 
 class RegcorParams(ctypes.Structure):
     _fields_ = ([(k, ctypes.c_double) for k in _PARAM_D] + [("rdens", ctypes.c_double * 3)]
-                + [(k, ctypes.c_int) for k in _PARAM_I])
+                + [(k, ctypes.c_int) for k in _PARAM_I] + [("cmsep2", ctypes.c_double)])
 
 
 def regcor_params(P: dict) -> RegcorParams:
@@ -134,6 +137,7 @@ def regcor_params(P: dict) -> RegcorParams:
         p.rdens[q] = float(P["rdens"][q])
     for k in _PARAM_I:
         setattr(p, k, int(P.get(k, 0)))
+    p.cmsep2 = float(P.get("cmsep2", 0.0))
     return p
 
 
@@ -142,7 +146,8 @@ class Regcor:
     nb6_regcor_device (REGCOR_GPU over one block, regcor_gpu.F:1-740).  S holds numpy
     arrays named as in nb6_regstate: [3][nmax] for REG3 (stored on the device as
     [nmax][3], the Fortran X(3,NMAX) layout), [nmax] for REG1, list int32
-    [nmax][lmax], h [nmax - N] (optional)."""
+    [nmax][lmax], h [nmax - N] (optional), listv int32 and r [nmax - N] (optional,
+    CHECKL with KZ(37) = 1).  Counters: int[7] (NCOUNT)."""
 
     def __init__(self, S: dict):
         self.L = load()
@@ -156,6 +161,11 @@ class Regcor:
         self.dev["list"] = DeviceArray.from_host(np.ascontiguousarray(S["list"], np.int32))
         h = S.get("h")
         self.dev["h"] = DeviceArray.from_host(c64(h)) if h is not None and len(h) else None
+        lv = S.get("listv")
+        self.dev["listv"] = (DeviceArray.from_host(np.ascontiguousarray(lv, np.int32))
+                             if lv is not None else None)
+        r = S.get("r")
+        self.dev["r"] = DeviceArray.from_host(c64(r)) if r is not None and len(r) else None
         self.st = RegState(nmax=self.nmax, lmax=self.lmax)
         for k in _STATE_PTRS:
             setattr(self.st, k, self.dev[k].ptr if self.dev.get(k) is not None else None)

@@ -23,11 +23,18 @@ rounds every operation to single), everything else REAL*8 (Python floats, one IE
 rounding per operation in Fortran's left-to-right order).  ONE3 = 1/3 and ONE12 = 1/12
 in double (setup.F:486, zero.f:116).
 
+CHECKL (regcor_gpu.F:240-244) is restated for KZ(37) = 1, the setting of the
+reference's own example input: the high-velocity intruder test of checkl.F:243-340
+(at most one body of LISTV joins the list per particle, with its force moved off the
+regular side; a c.m. particle takes the resolved interactions of checkl.F:282-336
+inside the c.m. approximation).  KZ(37) > 1 (close encounters and disrupted pairs,
+checkl.F:17-242) is counted as unsupported.  A KS component J < IFIRST in FPCORR
+uses its c.m. body's F and FDOT (fpcorr.F:46-61).
+
 Not restated: tides other than the TIDAL(4) rotation of FIDOT (XTRNLF, ETIDE), cloud
-forces (FCLOUD), the high-velocity list check (CHECKL, KZ(37)), KS components in a
-list (J < IFIRST in fpcorr.F:24-35: counted as unsupported), and the print-only
-diagnostic block (regcor_gpu.F:700-722).  NB_FLAG and M_FLAG are inputs (regcor_gpu.F
-sets them from KZ(39) and leaves them unset otherwise).
+forces (FCLOUD), and the print-only diagnostic block (regcor_gpu.F:700-722).  NB_FLAG
+and M_FLAG are inputs (regcor_gpu.F sets them from KZ(39) and leaves them unset
+otherwise).
 
 Order within a block: the reference corrects the particles one after another, so the
 retention test of particle I reads STEP(J) already divided by 8 when J precedes I in
@@ -39,7 +46,9 @@ of I with SMIN < STEP(J) <= 8 SMIN.
 State layout (all numpy, index = particle number - 1): x, xdot, x0, x0dot, f, fdot, fi,
 fidot, fr, frdot, d2, d3, xn, xndot, d0, d0r, d1r, d2r, d3r are [3][nmax]; body, t0,
 step, timenw, t0r, stepr, rs are [nmax]; list is int32 [nmax][lmax] (count, then
-1-based ascending indices); h is [npairs] (KS binding energies, used for I > N).
+1-based ascending indices); h is [npairs] (KS binding energies, used for I > N);
+with KZ(37) = 1 also listv (int, [0] = count, then 1-based LISTV) and r [npairs] (KS
+separations R, for c.m. particles), and P["cmsep2"].
 """
 from __future__ import annotations
 
@@ -94,24 +103,30 @@ def merge_lists(old, new):
     return losses, gains
 
 
-def fpcorr_terms(S, P, i, xi, vi, jlist, nloss, unsupported):
+def fpcorr_terms(S, P, i, xi, vi, jlist, nloss, C):
     """fpcorr.F:11-98: summed F2DOT, F3DOT over the lost (sign -) then gained bodies."""
     save2 = [0.0, 0.0, 0.0]
     save3 = [0.0, 0.0, 0.0]
     for L, j1 in enumerate(jlist):
         j = j1 - 1
-        if j1 < P["ifirst"]:          # KS component (fpcorr.F:24-35)
-            unsupported[0] += 1
-            continue
-        s = P["time"] - S["t0"][j]
-        s3 = 3.0 * s
         A = [0.0] * 12
-        for k in range(3):
-            fj, fdj = S["f"][k][j], S["fdot"][k][j]
-            A[k] = S["x"][k][j] - xi[k]
-            A[k + 3] = (fdj * s3 + 2.0 * fj) * s + S["x0dot"][k][j] - vi[k]
-            A[k + 6] = 2.0 * (fdj * s3 + fj - S["f"][k][i])
-            A[k + 9] = 6.0 * (fdj - S["fdot"][k][i])
+        if j1 < P["ifirst"]:          # KS component: its c.m. values (fpcorr.F:46-61)
+            jcm = P["n"] + (j1 + 1) // 2 - 1          # N + KVEC(J), KVEC(J) = (J + 1) / 2
+            for k in range(3):
+                A[k] = S["x"][k][j] - xi[k]
+                A[k + 3] = S["xdot"][k][j] - vi[k]
+                A[k + 6] = 2.0 * (S["f"][k][jcm] - S["f"][k][i])
+                A[k + 9] = 6.0 * (S["fdot"][k][jcm] - S["fdot"][k][i])
+            C["ncmder"] += 1
+        else:
+            s = P["time"] - S["t0"][j]
+            s3 = 3.0 * s
+            for k in range(3):
+                fj, fdj = S["f"][k][j], S["fdot"][k][j]
+                A[k] = S["x"][k][j] - xi[k]
+                A[k + 3] = (fdj * s3 + 2.0 * fj) * s + S["x0dot"][k][j] - vi[k]
+                A[k + 6] = 2.0 * (fdj * s3 + fj - S["f"][k][i])
+                A[k + 9] = 6.0 * (fdj - S["fdot"][k][i])
         a13 = 1.0 / (A[0] * A[0] + A[1] * A[1] + A[2] * A[2])
         a14 = S["body"][j] * a13 * math.sqrt(a13)
         a15 = (A[0] * A[3] + A[1] * A[4] + A[2] * A[5]) * a13
@@ -141,12 +156,99 @@ def fpcorr_terms(S, P, i, xi, vi, jlist, nloss, unsupported):
     return save2, save3
 
 
+def _pair_mw(xk, vk, xl, vl, mk, ml, mi):
+    """checkl.F:300-329: mass-weighted term of KS component K on component L of I."""
+    a = [xk[0] - xl[0], xk[1] - xl[1], xk[2] - xl[2]]
+    rij2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2]
+    dv = [vk[0] - vl[0], vk[1] - vl[1], vk[2] - vl[2]]
+    dr2i = 1.0 / rij2
+    dr3i = mk * ml * dr2i * math.sqrt(dr2i) / mi
+    drdv = 3.0 * (a[0] * dv[0] + a[1] * dv[1] + a[2] * dv[2]) * dr2i
+    return a, dv, dr3i, drdv
+
+
+def _checkl_fast(S, P, I, xi, vi, rs2, freg, fdr, nlist, nnb, C):
+    """checkl.F:243-340 (KZ(37) = 1): the first body J of LISTV inside 2 RS, approaching,
+    with impact parameter below RS and not yet a neighbour joins the list of I; its
+    force and derivative leave FREG and FDR (the irregular side is done later by the
+    gain loop).  Returns the new NNB; nlist is updated in place."""
+    i = I - 1
+    X, XD, B = S["x"], S["xdot"], S["body"]
+    lv = S["listv"]
+    for L in range(1, int(lv[0]) + 1):
+        J = int(lv[L])
+        if J == I:
+            continue
+        j = J - 1
+        a1, a2, a3 = X[0][j] - xi[0], X[1][j] - xi[1], X[2][j] - xi[2]
+        rij2 = a1 * a1 + a2 * a2 + a3 * a3
+        if rij2 > 4.0 * rs2:
+            continue
+        a4, a5, a6 = XD[0][j] - XD[0][i], XD[1][j] - XD[1][i], XD[2][j] - XD[2][i]
+        a7 = a1 * a4 + a2 * a5 + a3 * a6
+        if a7 > 0.0:
+            continue
+        p2 = rij2 - a7 * a7 / (a4 * a4 + a5 * a5 + a6 * a6)
+        if p2 > rs2:
+            continue
+        if J in nlist:
+            continue
+        C["nbfast"] += 1
+        if nnb > P["nnbmax"]:
+            return nnb
+        resolved = False
+        if I > P["n"]:
+            d = [xi[0] - X[0][j], xi[1] - X[1][j], xi[2] - X[2][j]]
+            rij2c = d[0] * d[0] + d[1] * d[1] + d[2] * d[2]
+            rp = float(S["r"][I - P["n"] - 1])
+            resolved = not (rij2c > P["cmsep2"] * (rp * rp) and J <= P["n"])
+        if not resolved:                              # checkl.F:191-215
+            dx = [X[k][j] - xi[k] for k in range(3)]
+            dv = [XD[k][j] - vi[k] for k in range(3)]
+            dr2 = 0.0
+            drdv = 0.0
+            for k in range(3):
+                dr2 = dr2 + dx[k] * dx[k]
+                drdv = drdv + dx[k] * dv[k]
+            dr2i = 1.0 / dr2
+            dr3i = float(B[j]) * dr2i * math.sqrt(dr2i)
+            drdv = 3.0 * drdv * dr2i
+            for k in range(3):
+                freg[k] = freg[k] - dx[k] * dr3i
+                fdr[k] = fdr[k] - (dv[k] - dx[k] * drdv) * dr3i
+        else:                                         # checkl.F:282-336
+            i1 = 2 * (I - P["n"]) - 1
+            js = [J]
+            if J > P["n"]:
+                rq = float(S["r"][J - P["n"] - 1])
+                if rij2c < P["cmsep2"] * (rq * rq):
+                    js = [2 * (J - P["n"]) - 1, 2 * (J - P["n"])]
+            for K in js:
+                for Lc in (i1, i1 + 1):
+                    k0, l0 = K - 1, Lc - 1
+                    a, dv, dr3i, drdv = _pair_mw([X[q][k0] for q in range(3)],
+                                                 [XD[q][k0] for q in range(3)],
+                                                 [X[q][l0] for q in range(3)],
+                                                 [XD[q][l0] for q in range(3)],
+                                                 float(B[k0]), float(B[l0]), float(B[i]))
+                    for q in range(3):
+                        freg[q] = freg[q] - a[q] * dr3i
+                    for q in range(3):
+                        fdr[q] = fdr[q] - (dv[q] - a[q] * drdv) * dr3i
+        pos = 0
+        while pos < len(nlist) and nlist[pos] < J:
+            pos += 1
+        nlist.insert(pos, J)
+        return nnb + 1
+    return nnb
+
+
 def regcor_block(S, P, addr, force, nlist):
     """Corrects the block in place.  addr: 1-based particle numbers; force: [7][ni]
     regf output (acc, jerk, pot); nlist: int32 [ni][lmax] regf rows (count, 0-based
     j-set indices with self), converted in place.  Returns the counters
-    {nbvoid, nbflux, nbsmin, nbcorr, unsupported}."""
-    C = dict(nbvoid=0, nbflux=0, nbsmin=0, nbcorr=0, unsupported=0)
+    {nbvoid, nbflux, nbsmin, nbcorr, unsupported, nbfast, ncmder}."""
+    C = dict(nbvoid=0, nbflux=0, nbsmin=0, nbcorr=0, unsupported=0, nbfast=0, ncmder=0)
     uns = [0]
     step0 = np.array(S["step"], float)        # STEP(J) of the retention test: block start
     for k_i, I in enumerate(addr):
@@ -255,6 +357,12 @@ def _one(S, P, I, fg, row, C, uns, step0):
                          + d[2] * (vi[2] - float(S["xdot"][2][j]))) / rij
                 a1 = min(a1, rij + rsdot * dtr)
             rs = max(a1, d4(1.1) * rs)
+        # optional neighbour additions, regcor_gpu.F:240-244
+        kz37 = P.get("kz37", 0)
+        if kz37 == 1 and S.get("listv") is not None and int(S["listv"][0]) > 0:
+            nnb = _checkl_fast(S, P, I, xi, vi, rs2, freg, fdr, nlist, nnb, C)
+        elif kz37 > 1:
+            uns[0] += 1
     S["rs"][i] = rs
 
     # losses and gains, regcor_gpu.F:252-350
@@ -359,7 +467,7 @@ def _one(S, P, I, fg, row, C, uns, step0):
         gains = []
         call = True
     if call and nbloss + nbgain > 0:
-        s2, s3 = fpcorr_terms(S, P, i, xi, vi, losses + gains, nbloss, uns)
+        s2, s3 = fpcorr_terms(S, P, i, xi, vi, losses + gains, nbloss, C)
         C["nbcorr"] += 1
         for k in range(3):
             S["d2"][k][i] = float(S["d2"][k][i]) + s2[k]

@@ -94,18 +94,31 @@ def pair_forces(x, v, m, i, js):
 
 def regcor_case(seed=7, n=800, ni=300, lmax=160, nnbopt=20, ncm=0, kz38=1, nb_flag=1,
                 m_flag=0, tidal4=0.0, frac_void=0.04, frac_empty_old=0.05,
-                frac_small_step=0.25, frac_few=0.04, frac_same=0.3, ifirst=1):
+                frac_small_step=0.25, frac_few=0.04, frac_same=0.3, ifirst=1, kz37=0,
+                nfast=0, ks_old=0.0):
     """A regular block for the corrector: state S (oracle layout), parameters P,
     1-based addresses, the regf output force [7][ni] (FP64 regular force and jerk over
     the non-neighbours) and the regf rows [ni][lmax] (0-based, self included).
     Particles n-ncm.. are c.m. bodies with binding energies h.  With ifirst > 1 the
     bodies 1..ifirst-1 stand for KS components: they are outside the j-set (regf rows
     are positions in IFIRST..NTOT), in no list and in no block.  Steps are powers of two
-    commensurate with TIME = 1, so every branch of the step rule is reachable."""
+    commensurate with TIME = 1, so every branch of the step rule is reachable.
+    kz37 = 1 with nfast > 0: LISTV holds nfast fast bodies (singles above the KS
+    components, and some c.m. bodies), each pair has a KS separation r and CMSEP2 =
+    1e4 (CHECKL).  ks_old: that fraction of the old lists also holds a few KS
+    components J < IFIRST (needs IFIRST - 1 <= 2 ncm), which the new lists lose
+    (FPCORR's c.m. branch)."""
     rng = np.random.default_rng(seed)
     m, x, v = ics.plummer(n, seed)
     nmax = n
     time = 1.0
+    listv = None
+    if nfast > 0:
+        lo = max(ifirst, 2 * ncm + 1)                 # above every KS component
+        pool = np.r_[np.arange(lo, n - ncm + 1), np.arange(n - ncm + 1, n + 1)[: ncm // 4]]
+        fast = rng.choice(pool, min(nfast, pool.size), replace=False)
+        v[fast - 1] *= 5.0
+        listv = np.r_[len(fast), fast].astype(np.int32)
     dtmin = 2.0 ** -14
     f0 = ifirst - 1                                   # first j-set body, 0-based
     addr0 = np.sort(f0 + rng.choice(n - f0, ni, replace=False))
@@ -125,8 +138,12 @@ def regcor_case(seed=7, n=800, ni=300, lmax=160, nnbopt=20, ncm=0, kz38=1, nb_fl
     old = [o[o >= f0] for o in old]
     lst = np.zeros((nmax, lmax), np.int32)
     empty_old = rng.random(n) < frac_empty_old
+    ks = (rng.random(n) < ks_old) if ks_old > 0 and ifirst > 1 else np.zeros(n, bool)
     for i in range(n):
         js = old[i][:nnbmax] if not empty_old[i] else old[i][:0]
+        if ks[i] and len(js) > 0:
+            comp = rng.choice(ifirst - 1, rng.integers(1, 4), replace=False)   # 0-based
+            js = np.unique(np.r_[comp, js])[:nnbmax]
         lst[i, 0] = len(js)
         lst[i, 1:1 + len(js)] = js + 1
     # new rows in regf form: 0-based positions, self included; void rows hold self only
@@ -180,7 +197,10 @@ def regcor_case(seed=7, n=800, ni=300, lmax=160, nnbopt=20, ncm=0, kz38=1, nb_fl
     P = dict(time=time, dtmin=dtmin, smin=2.0 * dtmin, smax=0.125, etar=0.02, rscale=0.8,
              rc2=0.05, rsfac=12.0, znbmin=0.2 * nnbopt, bodym=1.0 / n, tidal4=tidal4,
              rdens=[0.01, -0.02, 0.005], n=n - ncm, ifirst=ifirst, nnbopt=nnbopt, nnbmax=nnbmax,
-             nb_flag=nb_flag, m_flag=m_flag, kz38=kz38)
+             nb_flag=nb_flag, m_flag=m_flag, kz38=kz38, kz37=kz37, cmsep2=1.0e4)
+    if listv is not None:
+        S["listv"] = listv
+        S["r"] = 10.0 ** rng.uniform(-3.5, -1.5, max(ncm, 1))
     return S, P, addr0 + 1, force, rows
 
 

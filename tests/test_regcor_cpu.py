@@ -186,12 +186,141 @@ def test_fpcorr_derivatives_match_power_series():
     a = 2 * (S["fdot"][:, j] * 3 * s + S["f"][:, j] - S["f"][:, 0])
     jj = 6 * (S["fdot"][:, j] - S["fdot"][:, 0])
     want2, want3 = _series_force_derivs(r, v, a, jj, S["body"][j])
-    uns = [0]
-    s2, s3 = R.fpcorr_terms(S, P, 0, list(xi), list(vi), [j + 1], 0, uns)
+    C = dict(ncmder=0)
+    s2, s3 = R.fpcorr_terms(S, P, 0, list(xi), list(vi), [j + 1], 0, C)
     assert np.allclose(s2, want2, rtol=1e-10, atol=0)
     assert np.allclose(s3, want3, rtol=1e-10, atol=0)
-    l2, l3 = R.fpcorr_terms(S, P, 0, list(xi), list(vi), [j + 1], 1, uns)   # as a loss
+    l2, l3 = R.fpcorr_terms(S, P, 0, list(xi), list(vi), [j + 1], 1, C)   # as a loss
     assert np.allclose(l2, -np.asarray(want2)) and np.allclose(l3, -np.asarray(want3))
+
+
+def test_fpcorr_ks_component_uses_its_cm_derivatives():
+    """A KS component J < IFIRST (bodies 1, 2 of pair 1; c.m. body N + KVEC(J) = 5):
+    the terms are the 2nd and 3rd derivatives of its pair force along the motion with
+    its own position and velocity and the c.m. body's acceleration 2 F and jerk
+    6 FDOT (fpcorr.F:46-61), without the prediction of the single-body branch."""
+    rng = np.random.default_rng(6)
+    S, P, row = _single(nmax=6)
+    P.update(ifirst=3, n=4)
+    i = 2                                      # particle 3 (a single)
+    S["x"][:, :3] = [[0.05, 0.0502, 0.0], [0.02, 0.0199, 0.0], [0.0, 0.0001, 0.0]]
+    S["t0"][:] = 0.5                           # must not enter the component branch
+    for b in (0, 1, 4, i):
+        S["f"][:, b] = rng.normal(size=3) * 0.3
+        S["fdot"][:, b] = rng.normal(size=3) * 0.2
+    xi, vi = S["x"][:, i], S["xdot"][:, i]
+    for J in (1, 2):
+        jj = J - 1
+        r = S["x"][:, jj] - xi
+        v = S["xdot"][:, jj] - vi
+        a = 2 * (S["f"][:, 4] - S["f"][:, i])
+        jk = 6 * (S["fdot"][:, 4] - S["fdot"][:, i])
+        want2, want3 = _series_force_derivs(r, v, a, jk, S["body"][jj])
+        C = dict(ncmder=0)
+        s2, s3 = R.fpcorr_terms(S, P, i, list(xi), list(vi), [J], 0, C)
+        assert C["ncmder"] == 1
+        assert np.allclose(s2, want2, rtol=1e-10, atol=0)
+        assert np.allclose(s3, want3, rtol=1e-10, atol=0)
+
+
+def _fast_case(J_pos, J_vel, listv, nmax=8):
+    """Particle 1 with neighbours 2, 3 and RS = 0.1; bodies in listv placed at J_pos
+    (relative to particle 1) with velocities J_vel."""
+    S, P, row = _single(nlist=(2, 3), old=(2, 3), nmax=nmax)
+    for J, (dx, dv) in zip(listv, zip(J_pos, J_vel)):
+        S["x"][:, J - 1] = S["x"][:, 0] + np.asarray(dx)
+        S["xdot"][:, J - 1] = S["xdot"][:, 0] + np.asarray(dv)
+    S["listv"] = np.r_[len(listv), listv].astype(np.int32)
+    S["r"] = np.array([1e-3])
+    P.update(kz37=1, cmsep2=1.0e4)
+    return S, P, row
+
+
+def test_checkl_adds_the_first_approaching_intruder():
+    """KZ(37) = 1 (checkl.F:243-340): LISTV = [6 receding, 7 outside 2 RS, 8 impact
+    parameter above RS, 2 already a neighbour, 5 approaching, 4 approaching]: only 5
+    joins the list (one per call), its force leaves FR and reaches FI through the gain,
+    so FI is the direct sum over the final list and FI + FR is unchanged; NBFAST = 1.
+    With KZ(37) = 0 the list stays as it was."""
+    pos = [[0.15, 0, 0], [0.25, 0, 0], [0.15, 0.12, 0], [0.05, 0.02, 0], [0.12, 0.0, 0.03],
+           [0.1, -0.1, 0]]
+    vel = [[1.0, 0, 0], [-1.0, 0, 0], [-1.0, 0, 0], [0.1, 0.1, 0.0], [-2.0, 0.1, 0.0],
+           [-1.0, 1.0, 0.0]]
+    lv = [6, 7, 8, 2, 5, 4]
+    for kz37 in (0, 1):
+        S, P, row = _fast_case(pos, vel, lv)
+        P["kz37"] = kz37
+        if kz37 == 1:
+            S["x"][:, 1] = S["x"][:, 0] + np.array(pos[3])
+            S["xdot"][:, 1] = S["xdot"][:, 0] + np.array(vel[3])
+        x, v, m = S["x"].T.copy(), S["xdot"].T.copy(), S["body"]
+        force = np.zeros((7, 1))
+        force[0:3, 0], force[3:6, 0] = pair_forces(x, v, m, 0, np.array([3, 4, 5, 6, 7]))
+        fi0 = S["fi"][:, 0].copy()
+        C = R.regcor_block(S, P, [1], force, row)
+        final = S["list"][0, 1:S["list"][0, 0] + 1] if C["nbflux"] else np.array([2, 3])
+        if kz37 == 0:
+            assert C["nbfast"] == 0 and list(row[0, :3]) == [2, 2, 3]
+            continue
+        assert C["nbfast"] == 1
+        assert list(row[0, :4]) == [3, 2, 3, 5]
+        assert list(final) == [2, 3, 5]
+        f5, j5 = pair_forces(x, v, m, 0, np.array([4]))
+        sf, sj = np.abs(force[0:3, 0]).max(), np.abs(force[3:6, 0]).max()
+        assert np.allclose(S["fr"][:, 0], force[0:3, 0] - f5, rtol=0, atol=1e-14 * sf)
+        assert np.allclose(S["frdot"][:, 0], force[3:6, 0] - j5, rtol=0, atol=1e-13 * sj)
+        assert np.allclose(S["fi"][:, 0], fi0 + f5, rtol=0, atol=1e-14 * sf)
+
+
+def test_checkl_cm_particle_takes_component_terms():
+    """A c.m. particle I > N near an intruder J (single, inside CMSEP2 R^2): the force
+    leaving FREG is the mass-weighted sum over I's two KS components
+    m_J m_L / m_I (x_J - x_L) / r^3 (checkl.F:282-336), not the point-mass term."""
+    nmax = 8
+    S, P, row = _single(nlist=(6,), old=(6,), nmax=nmax)
+    # bodies 1, 2: components of pair 1 (c.m. = body 7 = N + 1 with N = 6), I = 7
+    P.update(n=6, ifirst=3, kz37=1, cmsep2=1.0e4)
+    m1, m2 = 1.2e-3, 0.8e-3
+    S["body"][[0, 1, 6]] = [m1, m2, m1 + m2]
+    cm = np.array([0.2, 0.1, -0.05])
+    d = np.array([2e-3, -1e-3, 5e-4])
+    S["x"][:, 6] = cm
+    S["x"][:, 0] = cm + m2 / (m1 + m2) * d
+    S["x"][:, 1] = cm - m1 / (m1 + m2) * d
+    S["xdot"][:, 6] = [0.1, 0.0, 0.0]
+    S["xdot"][:, 0] = S["xdot"][:, 6] + [0.0, 1.0, 0.0]
+    S["xdot"][:, 1] = S["xdot"][:, 6] - m1 / m2 * np.array([0.0, 1.0, 0.0])
+    S["x"][:, 5] = cm + [0.05, 0.0, 0.0]            # the old and new neighbour 6
+    S["x"][:, 4] = cm + [0.08, 0.01, 0.0]           # intruder 5, approaching
+    S["xdot"][:, 4] = S["xdot"][:, 6] + [-1.0, 0.0, 0.0]
+    S["listv"] = np.array([1, 5], np.int32)
+    S["r"] = np.array([np.linalg.norm(d)])          # CMSEP2 R^2 = 0.05 > 0.08^2
+    S["rs"][6] = 0.1
+    S["h"] = np.array([-5.0])
+    S["list"][:, :] = 0
+    S["list"][6, 0:2] = [1, 6]
+    row = np.zeros((1, S["list"].shape[1]), np.int32)
+    row[0, 0:3] = [2, 3, 4]             # regf form: positions from IFIRST = 3: bodies 6, 7 (self)
+    force = np.zeros((7, 1))
+    force[0:3, 0] = [0.3, -0.2, 0.1]
+    force[3:6, 0] = [0.01, 0.02, -0.03]
+    C = R.regcor_block(S, P, [7], force, row)
+    assert C["nbfast"] == 1
+    assert list(row[0, :3]) == [2, 5, 6]
+    want_f = np.zeros(3)
+    want_j = np.zeros(3)
+    for L, mL in ((0, m1), (1, m2)):
+        a = S["x"][:, 4] - S["x"][:, L]
+        dv = S["xdot"][:, 4] - S["xdot"][:, L]
+        r2 = a @ a
+        w = S["body"][4] * mL / (m1 + m2) / (r2 * np.sqrt(r2))
+        want_f += a * w
+        want_j += (dv - 3 * (a @ dv) / r2 * a) * w
+    assert np.allclose(S["fr"][:, 6], force[0:3, 0] - want_f, rtol=0, atol=1e-13 * np.abs(want_f).max())
+    assert np.allclose(S["frdot"][:, 6], force[3:6, 0] - want_j, rtol=0,
+                       atol=1e-12 * np.abs(want_j).max())
+    pf, pj = pair_forces(S["x"].T, S["xdot"].T, S["body"], 6, np.array([4]))
+    assert not np.allclose(pf, want_f, rtol=1e-6)      # the point mass would differ
 
 
 def test_fpcorr_moves_derivatives_between_parts():

@@ -1,7 +1,9 @@
 """GPU regular corrector (nb6_regcor_device) against the oracle's restatement of
 regcor_gpu.F + fpcorr.F: bit-identical state, lists and counters on blocks that
 exercise every branch (losses, gains, SMIN retention, empty old and new lists,
-NNB <= 3, c.m. bodies, KZ(38) = 0..3, NB_FLAG, M_FLAG, TIDAL(4)), fed by CPU-built
+NNB <= 3, c.m. bodies, KZ(38) = 0..3, NB_FLAG, M_FLAG, TIDAL(4), CHECKL's
+high-velocity intruders with KZ(37) = 1 for single and c.m. bodies, KS components in
+FPCORR), fed by CPU-built
 rows or straight from nb6_regf_device, and on a sample of a 32768-particle block of a
 262,144-body system."""
 import numpy as np
@@ -19,12 +21,12 @@ FIELDS = lib.REG3 + lib.REG1
 def _gpu(S, P, addr, d_force, force_stride, d_rows, ni):
     rc = lib.Regcor(S)
     d_addr = lib.DeviceArray.from_host(np.asarray(addr, np.int32))
-    d_cnt = lib.DeviceArray.from_host(np.zeros(5, np.int32))
+    d_cnt = lib.DeviceArray.from_host(np.zeros(lib.NCOUNT, np.int32))
     try:
         rc.correct(P, ni, d_addr, d_force, force_stride, d_rows, d_cnt)
         assert lib.load().nb6_sync() == 0
         out = rc.download()
-        cnt = d_cnt.to_host(np.int32, (5,))
+        cnt = d_cnt.to_host(np.int32, (lib.NCOUNT,))
     finally:
         rc.free()
         d_addr.free()
@@ -33,7 +35,8 @@ def _gpu(S, P, addr, d_force, force_stride, d_rows, ni):
 
 
 def _counters(C):
-    return [C["nbvoid"], C["nbflux"], C["nbsmin"], C["nbcorr"], C["unsupported"]]
+    return [C["nbvoid"], C["nbflux"], C["nbsmin"], C["nbcorr"], C["unsupported"], C["nbfast"],
+            C["ncmder"]]
 
 
 def _same(got, want, label, idx=None):
@@ -59,6 +62,10 @@ CASES = [
     dict(seed=12, kz38=1, ncm=60, ni=400),
     dict(seed=13, kz38=2, frac_void=0.3, frac_empty_old=0.3, frac_few=0.2),
     dict(seed=14, kz38=1, ncm=50, ifirst=101, ni=350),
+    dict(seed=15, kz38=1, kz37=1, nfast=40),
+    dict(seed=16, kz38=1, kz37=1, nfast=40, ncm=60, ni=400, ks_old=0.5, ifirst=41),
+    dict(seed=18, kz38=2, kz37=1, nfast=120, ncm=80, ni=500, ifirst=61, ks_old=0.3,
+         frac_small_step=0.4),
 ]
 
 
@@ -70,6 +77,10 @@ def test_regcor_matches_oracle_bitwise(kw):
     rows_o = rows.copy()
     C = R.regcor_block(So, P, addr, force, rows_o)
     assert C["unsupported"] == 0
+    if kw.get("kz37"):
+        assert C["nbfast"] > 0
+    if kw.get("ks_old"):
+        assert C["ncmder"] > 0
     d_force = lib.DeviceArray.from_host(np.ascontiguousarray(force))
     d_rows = lib.DeviceArray.from_host(rows.astype(np.int32))
     try:
