@@ -46,17 +46,9 @@ def log(*a):
 
 def make_workload(nj=N_J, ni=N_I, seed=SEED):
     t = time.time()
-    mass = ics.kroupa_masses(nj, seed)
-    m, x, v = ics.king(nj, seed + 1, w0=6.0, masses=mass)
-    m, x, v = ics.add_binaries(m, x, v, 0.5, seed + 2)
-    rng = np.random.Generator(np.random.PCG64(seed + 3))
-    idx = np.sort(rng.choice(nj, size=ni, replace=False))
-    counts = ics.lognormal_counts(ni, seed + 4, median=80.0)
-    rs = ics.rs_for_counts(x, idx, counts)
-    h2 = rs * rs
-    dtr = np.exp(rng.uniform(np.log(2.0 ** -10), np.log(2.0 ** -4), ni))
+    w = ics.precision_study(nj, ni, seed)
     log(f"workload built in {time.time() - t:.1f} s")
-    return m, x, v, idx, h2, dtr
+    return w
 
 
 def traffic_from_profiles():
@@ -279,15 +271,32 @@ def bench_learned_rs(train_samples, eval_steps, no_cpu):
     net.close()
     r, q = res["rule"], res["net"]
     ratio = r["overflow_events"] / max(q["overflow_events"], 1)
+
+    def split(st):
+        # a particle's first regular step after start-up (radius from start-up, the same
+        # for both policies' inputs) against its later steps (radius from the policy)
+        return {"overflow_first_step": st["overflow_first"],
+                "overflow_later_steps": st["overflow_later"],
+                "later_samples": st["samples_later"],
+                "later_miss30_frac": st["miss30_later"] / max(st["samples_later"], 1),
+                "later_mean_abs_ln_count_ratio": st["absdev_later"] / max(st["samples_later"], 1)}
+    steady = {"rule": split(r), "learned": split(q),
+              "overflow_ratio_later_steps": (r["overflow_later"] / q["overflow_later"]
+                                             if q["overflow_later"] else None),
+              "miss30_ratio_later_steps": (split(r)["later_miss30_frac"] /
+                                           max(split(q)["later_miss30_frac"], 1e-12))}
     return {"metric": "neighbour-list overflow retries: reference RS rule / learned RS "
                       "(King W0=7 N=262144, NNBOPT=100)",
             "value": ratio, "unit": "x fewer gpunb_regf re-runs", "n_gpus": 1,
+            "start_up_vs_steady_state": steady,
             "higher_is_better": True, "dtype": "bf16 weights / fp32 accumulate",
             "data": "synthetic (seeded King W0=7, Kroupa masses)",
             "config": {"workload": "learned-rs", "n": n, "nnbopt": nnbopt, "lmax": lmax,
                        "nnbmax": lmax - 50, "eval_block_steps": eval_steps},
             "training": {"pairs": int(tgt.size), "block_steps": nsteps, "data_s": t_data,
-                         "train_s": t_train, "loss": hist, "target_var": float(tgt.var())},
+                         "train_s": t_train, "device_epochs_s": net.train_s,
+                         "samples_per_s": 4 * tgt.size / max(net.train_s, 1e-9),
+                         "loss": hist, "target_var": float(tgt.var())},
             "rule": r, "learned": q}
 
 
@@ -570,18 +579,25 @@ def bench_regcor(steps, warmup, no_cpu):
         L.nb6_device_copy(d_rows.ptr, d_rows0.ptr, d_rows0.nbytes, None)
         L.nb6_memcpy_htod(d_cnt.ptr, np.zeros(lib.NCOUNT, np.int32).ctypes.data, 4 * lib.NCOUNT)
 
-    for _ in range(warmup):
-        restore()
-        rc.correct(P, ni, d_addr, d_force, ni, d_rows, d_cnt)
-    L.nb6_sync()
-    kms = []
-    for _ in range(steps):
-        restore()
-        rc.correct(P, ni, d_addr, d_force, ni, d_rows, d_cnt)
-        kms.append(L.nb6_kernel_ms(b"regcor"))
-    L.nb6_sync()
+    def timed(mode):
+        # mode 0: the default (packed neighbour records for a block this large), 2: the
+        # neighbours' state read from the state arrays directly
+        L.nb6_set_regcor_pack(mode)
+        for _ in range(warmup):
+            restore()
+            rc.correct(P, ni, d_addr, d_force, ni, d_rows, d_cnt)
+        L.nb6_sync()
+        kms = []
+        for _ in range(steps):
+            restore()
+            rc.correct(P, ni, d_addr, d_force, ni, d_rows, d_cnt)
+            kms.append(L.nb6_kernel_ms(b"regcor"))
+        L.nb6_sync()
+        L.nb6_set_regcor_pack(0)
+        return float(np.mean(kms)) * 1e-3
+    kt_direct = timed(2)
+    kt = timed(0)
     cnt = d_cnt.to_host(np.int32, (lib.NCOUNT,))
-    kt = float(np.mean(kms)) * 1e-3
     out = rc.download()
     rows_g = d_rows.to_host(np.int32, (ni, LMAX))
     # algorithmic bytes of one call: per particle 276 B read (address, FREG/FDR, X/XDOT,
@@ -630,8 +646,8 @@ def bench_regcor(steps, warmup, no_cpu):
                        "mean_nnb": float(nnb.mean()), "changed_fraction": float(changed.mean())},
             "roofline": {"bound": "hbm", "achieved": round(alg / kt / 1e9, 1), "peak": 8000.0,
                          "unit": "GB/s", "frac": round(alg / kt / 8e12, 4),
-                         "algorithmic_bytes": int(alg), "kernel": "regcor_kernel",
-                         "avg_launch_ms": kt * 1e3},
+                         "algorithmic_bytes": int(alg), "kernel": "regcor_lists + finish",
+                         "avg_launch_ms": kt * 1e3, "unpacked_ms": kt_direct * 1e3},
             "cpu_baseline": cpu, "check_vs_oracle": check}
 
 
@@ -861,6 +877,24 @@ def reference_vs_fp64(res, ref, rl):
             "lists_identical": same_rows(rlst, rl[:k])}
 
 
+def parity_vs_reference(res, out, lst, ref64, m, x, v, idx, h2, dtr):
+    """This library's step against the reference's own reg.avx.cpp output on the CPU
+    sample (checker, oracle/refband.py): list entries that differ, outside one FP32
+    ulp of the threshold and outside the reference's derived FP32 rounding band, and
+    the per-particle rule |ours - ref| <= 1e-6 |ref| + |ref - FP64|."""
+    import refband as RB
+    racc, rjrk, rpot, rlst = res
+    k = len(rpot)
+    c = RB.classify_rows(lst[:k], rlst, x, v, m, x[idx[:k]], v[idx[:k]], h2[:k], dtr[:k], 0)
+    rule = {}
+    for name, ours, ref, f64 in (("acc", out[0:3, :k].T, racc, ref64[0:3, :k].T),
+                                 ("jerk", out[3:6, :k].T, rjrk, ref64[3:6, :k].T),
+                                 ("pot", out[6, :k, None], rpot[:, None], ref64[6, :k, None])):
+        nb, worst = RB.force_rule(ours, ref, f64)
+        rule[name] = {"particles_breaking": nb, "worst_ratio": worst}
+    return {"n_i": int(k), "lists": c, "force_rule_1e-6_plus_ref_error": rule}
+
+
 def all_i_vs_fp64_kernel(L, ni, di, do, dl, out, lst, lmax=LMAX, nnbmax=NNBMAX):
     """Every i of the step against the FP64 kernel (nb6_set_regf_precision(2), which
     matches the CPU oracle to 1e-14): per-particle vector relative errors of acc, jerk
@@ -955,20 +989,23 @@ def main():
         rc = L.nb6_regf_device(ni, ni, di.ptr, LMAX, NNBMAX, 0, do.ptr, dl.ptr, None)
         if rc != 0:
             lib.check_error()
-        return L.nb6_kernel_ms(b"regf_tile"), L.nb6_kernel_ms(b"regf_finalize")
+        return (L.nb6_kernel_ms(b"regf_tile"), L.nb6_kernel_ms(b"regf_finalize"),
+                L.nb6_kernel_ms(b"regf_fix"))
 
     for _ in range(args.warmup):
         step()
     L.nb6_sync()
-    tile_ms, fin_ms = [], []
+    tile_ms, fin_ms, fix_ms = [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        a, b = step()
+        a, b, c = step()
         tile_ms.append(a)
         fin_ms.append(b)
+        fix_ms.append(c)
     L.nb6_sync()
     wall = time.perf_counter() - t0
     chunks = L.nb6_last_regf_chunks()
+    refined = L.nb6_last_regf_refined()
 
     out = do.to_host(np.float64, (7, ni))
     lst = dl.to_host(np.int32, (ni, LMAX))
@@ -985,7 +1022,9 @@ def main():
             "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
             "traffic": traffic_from_profiles(),
             "kernel": "regf_tile_kernel", "avg_launch_ms": round(t_tile * 1e3, 3),
-            "finalize_avg_ms": round(float(np.mean(fin_ms)), 3), "j_chunks": chunks}
+            "finalize_avg_ms": round(float(np.mean(fin_ms)), 3),
+            "fp64_pass_avg_ms": round(float(np.mean(fix_ms)), 3), "fp64_pass_rows": refined,
+            "j_chunks": chunks}
     # CPU leg: the reference's AVX back end timed on a sample, and the FP64 oracle
     # as the checker of the GPU result on the first 256 i-particles
     cpu, acc = None, None
@@ -994,6 +1033,8 @@ def main():
         if base is not None:
             cpu, res = base
             full["reference_cpu_vs_fp64_kernel"] = reference_vs_fp64(res, ref64, rl64)
+            full["parity_vs_reference"] = parity_vs_reference(res, out, lst, ref64, m, x, v,
+                                                              idx, h2, dtr)
         acc = accuracy(m, x, v, idx, h2, dtr, out, lst)
     rec = {"metric": "regf pair-interactions/s at N=1M (%FP32 peak) at <=1e-6 rel err vs FP64",
            "value": value, "unit": "pair-interactions/s", "n_gpus": 1, "steps": args.steps,

@@ -166,13 +166,31 @@ int nb6_set_regf_slot_budget(int mib);
 int nb6_last_regf_batches(void);
 
 /* FP64 pass of the regular force (default on; no counterpart in the reference).
- * The FP32 kernel also sums error scales of its FP32 partial sums; the finalize
- * flags every row whose scale could put its acc or jerk more than 1e-6 (relative)
- * from FP64, and those rows are recomputed over all j in FP64 (same neighbour
- * test, so lists are unchanged).  nb6_last_regf_refined() returns the number of
- * rows of the last call's FP64 pass (it synchronises the device). */
+ * The FP32 kernel also sums, per row and j-chunk, error scales of its FP32 partial
+ * sums and of the FP32 rounding of its separations; the finalize flags every row
+ * whose bound could put its acc or jerk more than 1e-6 (relative) from FP64, and the
+ * chunks of that row whose bound exceeds 1/nchunk of the allowed error are recomputed
+ * in FP64 (same neighbour test, so lists are unchanged); rows whose FP32 sums are
+ * incomplete (NaN) are recomputed over all j.  nb6_last_regf_refined() returns the
+ * rows of the last call's FP64 pass, nb6_last_regf_refined_chunks() its (row, chunk)
+ * items; both synchronise the device. */
 int nb6_set_regf_refine(int on);
 int nb6_last_regf_refined(void);
+int nb6_last_regf_refined_chunks(void);
+/* Weights of the four error scales that flag a row for the FP64 pass (flush sums
+ * of acc and jerk, position and velocity rounding of the pair terms); larger
+ * weights flag more rows.  Defaults 0.5, 0.5, 2, 0. */
+int nb6_set_regf_refine_weights(double k0, double k1, double k2, double k3);
+
+/* Near field of the FP32 regular-force kernel (no counterpart in the reference):
+ * pairs with r < f RS are left out of the FP32 sums and summed in FP64 by the
+ * finalize.  f in [1, 4]; default 2.  Lists do not depend on it. */
+int nb6_set_regf_near(double f);
+
+/* Diagnostic: while d_bounds (device, double[4][out_stride] of later calls) is set,
+ * every regf call writes each row's four error scales there (the inputs of the FP64
+ * pass's decision); NULL switches it off. */
+int nb6_set_regf_debug_bounds(double *d_bounds);
 
 /* Ensemble of independent clusters in one device j-set (no counterpart in the
  * reference, which holds one j-set per library instance).
@@ -193,6 +211,9 @@ int nb6_ens_send_device(int nclusters, const int *offsets, const double *d_soa, 
 int nb6_ens_regf_device(const double *d_i, int stride, int lmax, int nnbmax, int m_flag,
                         double *d_out, int *d_list, int use_graph, void *stream);
 int nb6_ens_items(void);
+/* Rows of the last batched step that went through the FP64 pass (as
+ * nb6_last_regf_refined, over each row's own cluster; synchronises the device). */
+int nb6_ens_last_refined(void);
 
 /* Autotuners (run on the caller's own, already sent, inputs; outputs are
  * overwritten).  They time every combination of work-group size = LDS j-tile
@@ -289,6 +310,15 @@ int nb6_rsnet_grad_device(int n, const float *d_feat, const float *d_t, float *d
  * may be null) receives the minibatch loss before the step. */
 int nb6_rsnet_train_step_device(int n, const float *d_feat, const float *d_t, float lr,
                                 float *h_loss, void *stream);
+/* One epoch of minibatch Adam on the device (no counterpart in the reference): the n
+ * samples (SoA float[8][n], targets float[n], both on the device) in an order
+ * shuffled on the device from `seed`, n / batch steps of batch samples each, queued
+ * without host synchronisation; *h_loss (host, may be null; the call then returns
+ * without waiting) receives the mean squared residual over the epoch, each residual
+ * taken before its step. */
+int nb6_rsnet_train_epoch_device(int n, const float *d_feat, const float *d_t, int batch,
+                                 float lr, unsigned long long seed, float *h_loss,
+                                 void *stream);
 /* Features of n particles (SoA double arrays with `stride`; x, acc, jrk [3][stride]):
  * log10 of (nnb+1)/(4/3 pi RS^3), nnb+1, |acc|, |jrk|, |x - centre|, RS, dtr, mass;
  * nnb = previous neighbour count (int[n]), centre = density centre (host, 3). */
@@ -390,6 +420,12 @@ typedef struct nb6_regcor_params {
 int nb6_regcor_device(const nb6_regcor_params *p, const nb6_regstate *s, int ni,
                       const int *d_addr, const double *d_force, int force_stride,
                       int *d_nlist, int *d_counters, void *stream);
+
+/* How nb6_regcor_device reads the neighbours' state: 0 (default) packs every
+ * particle's X, XDOT, BODY, STEP, T0, F, FDOT, X0DOT into one record first when the
+ * block holds at least nmax/32 particles, 1 always packs, 2 never.  Results do not
+ * depend on it. */
+int nb6_set_regcor_pack(int mode);
 
 /* ---------------------------------------------------------------------------
  * Part 5 (extension): total-energy terms on the GPU.  Replaces the GPU branch of

@@ -611,6 +611,7 @@ int nb6_ens_regf_device(const double *d_i, int stride, int lmax, int nnbmax, int
                            stream ? static_cast<hipStream_t>(stream) : d.stream);
 }
 int nb6_ens_items(void) { return ens_items(); }
+int nb6_ens_last_refined(void) { return regf_items_refined(); }
 int nb6_ens_autotune(const double *d_i, int stride, int lmax, int nnbmax, int m_flag,
                      double *d_out, int *d_list, int *best, void *stream) {
     Device &d = device();
@@ -644,6 +645,30 @@ int nb6_set_regf_refine(int on) {
     return OK;
 }
 int nb6_last_regf_refined(void) { return regf_last_refined(); }
+int nb6_last_regf_refined_chunks(void) {
+    int c = 0;
+    (void)regf_last_refined(&c);
+    return c;
+}
+int nb6_set_regf_refine_weights(double k0, double k1, double k2, double k3) {
+    const double k[4] = {k0, k1, k2, k3};
+    for (int q = 0; q < 4; q++)
+        if (!(k[q] >= 0.0 && k[q] <= 1e6)) {
+            set_error(E_ARG, "nb6_set_regf_refine_weights: weights outside [0, 1e6]");
+            return E_ARG;
+        }
+    for (int q = 0; q < 4; q++) regf_config().kref[q] = (float)k[q];
+    return OK;
+}
+int nb6_set_regf_debug_bounds(double *d_bounds) {
+    regf_set_debug_bounds(d_bounds);
+    return OK;
+}
+int nb6_set_regf_near(double f) {
+    if (!(f >= 1.0 && f <= 4.0)) { set_error(E_ARG, "nb6_set_regf_near: factor outside [1, 4]"); return E_ARG; }
+    regf_config().near = (float)f;
+    return OK;
+}
 int nb6_set_regf_slot_budget(int mib) {
     if (mib < 1) { set_error(E_ARG, "nb6_set_regf_slot_budget: budget must be at least 1 MiB"); return E_ARG; }
     regf_config().slot_budget_mb = mib;
@@ -723,6 +748,7 @@ int nb6_energy_device(const nb6_energy_in *e, double *h_out, void *stream) {
     return energy_device(e, h_out, static_cast<hipStream_t>(stream));   // validates first
 }
 
+int nb6_set_regcor_pack(int mode) { return regcor_set_pack(mode); }
 int nb6_regcor_device(const nb6_regcor_params *p, const nb6_regstate *s, int ni,
                       const int *d_addr, const double *d_force, int force_stride,
                       int *d_nlist, int *d_counters, void *stream) {
@@ -749,6 +775,11 @@ int nb6_rsnet_train_step_device(int n, const float *d_feat, const float *d_t, fl
                                 float *h_loss, void *stream) {
     if (!device().ready) return E_NODEV;
     return rsnet_train_step(n, d_feat, d_t, lr, h_loss, pick(stream));
+}
+int nb6_rsnet_train_epoch_device(int n, const float *d_feat, const float *d_t, int batch,
+                                 float lr, unsigned long long seed, float *h_loss, void *stream) {
+    if (!device().ready) return E_NODEV;
+    return rsnet_train_epoch(n, d_feat, d_t, batch, lr, seed, h_loss, pick(stream));
 }
 int nb6_rs_features_device(int n, int stride, const double *d_x, const double *d_acc,
                            const double *d_jrk, const double *d_mass, const double *d_rs,

@@ -135,16 +135,16 @@ __global__ void ens_zero_kernel(int *q) { *q = 0; }
 
 struct GraphKey {
     const void *ptr[4] = {nullptr, nullptr, nullptr, nullptr};   // d_i, d_out, d_list, items
-    long long val[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    long long val[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     bool operator==(const GraphKey &o) const {
-        return std::equal(ptr, ptr + 4, o.ptr) && std::equal(val, val + 12, o.val);
+        return std::equal(ptr, ptr + 4, o.ptr) && std::equal(val, val + 14, o.val);
     }
 };
 
 struct Ens {
     bool open = false;
     int max_clusters = 0, max_particles = 0;
-    int nc = 0, ntot = 0, npad = 0;
+    int nc = 0, ntot = 0, npad = 0, maxseg = 0;   // maxseg: longest padded cluster
     std::vector<int> off, padoff;
     JSet js;                         // padded, sorted j-set of all clusters
     DBuf d_off, d_padoff, d_cid, d_box, d_keys, d_vals, d_skeys, d_svals, d_tmp, d_jbase;
@@ -238,6 +238,7 @@ int ens_close() {
     e.d_keys.release(); e.d_skeys.release(); e.d_vals.release(); e.d_svals.release();
     e.d_tmp.release(); e.d_jbase.release(); e.d_items.release(); e.d_queue.release();
     e.work.release();
+    regf_items_forget();
     e.nc = 0;
     e.per_cu = 0;
     e.items_bi = 0;
@@ -268,6 +269,8 @@ int ens_send_device(int nc, const int *offsets, const double *d_soa, int stride,
         e.padoff[c + 1] = e.padoff[c] + (n + JTILE - 1) / JTILE * JTILE;
     }
     e.npad = e.padoff[nc];
+    e.maxseg = 0;
+    for (int c = 0; c < nc; c++) e.maxseg = std::max(e.maxseg, e.padoff[c + 1] - e.padoff[c]);
     std::vector<int> cid(e.ntot), jb(e.ntot);
     for (int c = 0; c < nc; c++)
         for (int j = offsets[c]; j < offsets[c + 1]; j++) {
@@ -337,8 +340,8 @@ int ens_regf_device(const double *d_i, int stride, int lmax, int nnbmax, int m_f
         hipLaunchKernelGGL(ens_zero_kernel, dim3(1), dim3(1), 0, st, e.d_queue.as<int>());
         return regf_items_launch(e.js, e.npad, e.js.perm.as<int>(), stride, d_i,
                                  e.d_items.as<int4>(), e.nitems, e.d_queue.as<int>(),
-                                 e.d_jbase.as<int>(), lmax, nnbmax, m_flag, d_out, d_list, grid,
-                                 e.work, st);
+                                 e.d_jbase.as<int>(), e.d_padoff.as<int>(), e.nc, e.maxseg, lmax,
+                                 nnbmax, m_flag, d_out, d_list, grid, e.work, st);
     };
     if (!use_graph) return step(s);
     // The graph holds kernel arguments by value: pointers, sizes, the launch shape and
@@ -347,9 +350,11 @@ int ens_regf_device(const double *d_i, int stride, int lmax, int nnbmax, int m_f
     key.ptr[0] = d_i; key.ptr[1] = d_out; key.ptr[2] = d_list; key.ptr[3] = e.d_items.p;
     long long mbits = 0;
     if (m_flag) std::memcpy(&mbits, &e.js.mmax, sizeof(mbits));
-    const long long v[12] = {stride, lmax, nnbmax, m_flag, cfg.threads, cfg.ipl, e.nitems, grid,
-                             e.ntot, e.npad, mbits, cfg.far_test};
-    std::copy(v, v + 12, key.val);
+    int nbits = 0;
+    std::memcpy(&nbits, &cfg.near, sizeof(nbits));
+    const long long v[14] = {stride, lmax, nnbmax, m_flag, cfg.threads, cfg.ipl, e.nitems, grid,
+                             e.ntot, e.npad, mbits, cfg.far_test, cfg.refine, nbits};
+    std::copy(v, v + 14, key.val);
     const bool same = e.exec && key == e.gkey;
     if (!same) {
         drop_graph();

@@ -121,6 +121,8 @@ struct RegfConfig {
     int slot_budget_mb = 32768;   // largest work-array footprint of one call; larger
                                   // calls run in batches of i (same results)
     int refine = 1;       // FP64 pass over rows whose FP32 error scale is too large
+    float near = 2.0f;    // near field (FP64 in the finalize): r < near RS
+    float kref[4] = {0.5f, 0.5f, 2.0f, 0.0f};   // weights of the refine error scales
 };
 RegfConfig &regf_config();
 // the configuration a call with ni i-particles runs with (threads resolved)
@@ -132,18 +134,23 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
                 int nnbmax, int m_flag, double *d_out, int *d_list, hipStream_t s);
 int regf_last_chunks();
 int regf_last_batches();
-int regf_last_refined();     // rows of the last call's FP64 pass (synchronises)
+// rows of the last call's FP64 pass, and its (row, chunk) items (synchronises)
+int regf_last_refined(int *chunk_items = nullptr);
 void regf_release();
 
 // Scratch of one regular-force pipeline: FP32 partials, candidate counts and slots.
 struct RegfWork {
     DBuf part, cnt, hslot, sslot;
     DBuf ikey, ival, itmp;           // Morton sort of the i-set
-    DBuf fixrows, nfix, fixpart;     // FP64 pass: flagged rows, their count, chunk sums
+    DBuf fixrows, nfix, fixpart;     // FP64 pass: whole rows, counters, chunk sums
+    DBuf items, delta, rowsel;       // FP64 pass: (row, chunk) items, candidate terms, rows
+    DBuf segitems, part64;           // FP64 pass: items by chunk, FP64 sums per sub-range
     void release() {
         part.release(); cnt.release(); hslot.release(); sslot.release();
         ikey.release(); ival.release(); itmp.release();
         fixrows.release(); nfix.release(); fixpart.release();
+        items.release(); delta.release(); rowsel.release();
+        segitems.release(); part64.release();
     }
 };
 
@@ -155,11 +162,16 @@ const int *jset_sort_i(const JSet &js, int ni, int stride, const double *d_i, Re
 // One persistent launch over work items (slot begin, slot end, jbeg, jend) of a padded
 // j-set; nslots rows of work arrays, imap[slot] = row of d_i / d_out / d_list (< 0:
 // empty slot); queue is a device counter the caller zeroes first; jbase[row] maps j
-// back to the caller's indexing.  Safe under stream capture once w is sized.
+// back to the caller's indexing; padoff[nc + 1] are the clusters' slot ranges, maxseg the
+// longest (the FP64 pass of flagged rows runs over the row's own cluster).  Safe under stream capture once
+// w is sized.
 int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, const double *d_i,
-                      const int4 *items, int nitems, int *queue, const int *jbase, int lmax,
-                      int nnbmax, int m_flag, double *d_out, int *d_list, int grid, RegfWork &w,
-                      hipStream_t s);
+                      const int4 *items, int nitems, int *queue, const int *jbase,
+                      const int *padoff, int nc, int maxseg, int lmax, int nnbmax, int m_flag,
+                      double *d_out, int *d_list, int grid, RegfWork &w, hipStream_t s);
+int regf_items_refined();     // rows of the last batched step's FP64 pass (synchronises)
+void regf_items_forget();     // the ensemble's work buffers are gone
+void regf_set_debug_bounds(double *d);   // rows' error scales to d[4][out_stride] (or null)
 
 // streaming copy of bytes (multiple of 16) on stream s
 void device_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
@@ -174,6 +186,8 @@ int rsnet_set_norm(const float *mu, const float *sigma);
 int rsnet_forward(int n, const float *d_feat, float *d_y, hipStream_t s);
 int rsnet_grad(int n, const float *d_feat, const float *d_t, float *d_grad, float *d_loss,
                hipStream_t s);
+int rsnet_train_epoch(int n, const float *d_feat, const float *d_t, int batch, float lr,
+                      unsigned long long seed, float *h_loss, hipStream_t s);
 int rsnet_train_step(int n, const float *d_feat, const float *d_t, float lr, float *h_loss,
                      hipStream_t s);
 int rs_features(int n, int stride, const double *x, const double *acc, const double *jrk,
@@ -213,6 +227,7 @@ struct nb6_regstate;
 struct nb6_energy_in;
 namespace nb6 {
 int energy_device(const ::nb6_energy_in *e, double *h_out, hipStream_t s);
+int regcor_set_pack(int mode);
 int regcor_device(const ::nb6_regcor_params *p, const ::nb6_regstate *s, int ni,
                   const int *d_addr, const double *d_force, int force_stride, int *d_nlist,
                   int *d_counters, hipStream_t stream);

@@ -34,19 +34,57 @@ constexpr double ONE12 = 1.0 / 12.0;
 
 enum { C_VOID, C_FLUX, C_SMIN, C_CORR, C_UNS, C_FAST, C_CMDER, NCOUNT };
 
+// Packed neighbour records: what the list kernel reads of a neighbour J, in one record
+// of PK_STRIDE doubles per particle (192 bytes: at most two 128-byte lines) instead of
+// eight arrays (eight lines).  Written at the start of a call from the state arrays,
+// which the call does not change before the list kernel has read them, so the values
+// are the same.
+enum { PK_X = 0, PK_XDOT = 3, PK_BODY = 6, PK_STEP = 7, PK_T0 = 8, PK_F = 9, PK_FDOT = 12,
+       PK_X0DOT = 15, PK_STRIDE = 24 };
+
+struct NbSrc {
+    const nb6_regstate &s;
+    const double *pk;   // packed records or nullptr (read the state arrays)
+    __device__ double x(int j, int k) const { return pk ? pk[(size_t)j * PK_STRIDE + PK_X + k] : s.x[3 * (size_t)j + k]; }
+    __device__ double xdot(int j, int k) const { return pk ? pk[(size_t)j * PK_STRIDE + PK_XDOT + k] : s.xdot[3 * (size_t)j + k]; }
+    __device__ double body(int j) const { return pk ? pk[(size_t)j * PK_STRIDE + PK_BODY] : s.body[j]; }
+    __device__ double step(int j) const { return pk ? pk[(size_t)j * PK_STRIDE + PK_STEP] : s.step[j]; }
+    __device__ double t0(int j) const { return pk ? pk[(size_t)j * PK_STRIDE + PK_T0] : s.t0[j]; }
+    __device__ double f(int j, int k) const { return pk ? pk[(size_t)j * PK_STRIDE + PK_F + k] : s.f[3 * (size_t)j + k]; }
+    __device__ double fdot(int j, int k) const { return pk ? pk[(size_t)j * PK_STRIDE + PK_FDOT + k] : s.fdot[3 * (size_t)j + k]; }
+    __device__ double x0dot(int j, int k) const { return pk ? pk[(size_t)j * PK_STRIDE + PK_X0DOT + k] : s.x0dot[3 * (size_t)j + k]; }
+};
+
+__global__ __launch_bounds__(256) void regcor_pack_kernel(nb6_regstate s, double *__restrict__ pk) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= s.nmax) return;
+    double *o = pk + (size_t)j * PK_STRIDE;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        o[PK_X + k] = s.x[3 * (size_t)j + k];
+        o[PK_XDOT + k] = s.xdot[3 * (size_t)j + k];
+        o[PK_F + k] = s.f[3 * (size_t)j + k];
+        o[PK_FDOT + k] = s.fdot[3 * (size_t)j + k];
+        o[PK_X0DOT + k] = s.x0dot[3 * (size_t)j + k];
+    }
+    o[PK_BODY] = s.body[j];
+    o[PK_STEP] = s.step[j];
+    o[PK_T0] = s.t0[j];
+}
+
 // force and first derivative of body j (0-based) on i, regcor_gpu.F:334-353
-__device__ __forceinline__ void pair_fj(const nb6_regstate &s, int j, const double *xi,
+__device__ __forceinline__ void pair_fj(const NbSrc &s, int j, const double *xi,
                                         const double *vi, double *fa, double *fj) {
 #pragma clang fp contract(off)
     double a[3], dv[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        a[k] = s.x[3 * (size_t)j + k] - xi[k];
-        dv[k] = s.xdot[3 * (size_t)j + k] - vi[k];
+        a[k] = s.x(j, k) - xi[k];
+        dv[k] = s.xdot(j, k) - vi[k];
     }
     const double rij2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
     const double dr2i = 1.0 / rij2;
-    const double dr3i = s.body[j] * dr2i * sqrt(dr2i);
+    const double dr3i = s.body(j) * dr2i * sqrt(dr2i);
     const double drdv = a[0] * dv[0] + a[1] * dv[1] + a[2] * dv[2];
     const double drdp = 3.0 * drdv * dr2i;
 #pragma unroll
@@ -59,7 +97,7 @@ __device__ __forceinline__ void pair_fj(const nb6_regstate &s, int j, const doub
 // second and third derivative of the force of body j on i, fpcorr.F:24-74; a KS
 // component j (jcm >= 0, 0-based c.m. body N + KVEC(J)) takes its own position and
 // velocity with the c.m. body's F and FDOT (fpcorr.F:46-61)
-__device__ __forceinline__ void fp_terms(const nb6_regstate &s, double time, int i, int j,
+__device__ __forceinline__ void fp_terms(const NbSrc &s, double time, int i, int j,
                                          int jcm, const double *xi, const double *vi,
                                          double *f2, double *f3) {
 #pragma clang fp contract(off)
@@ -67,25 +105,25 @@ __device__ __forceinline__ void fp_terms(const nb6_regstate &s, double time, int
     if (jcm >= 0) {
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            A[k] = s.x[3 * (size_t)j + k] - xi[k];
-            A[k + 3] = s.xdot[3 * (size_t)j + k] - vi[k];
-            A[k + 6] = 2.0 * (s.f[3 * (size_t)jcm + k] - s.f[3 * (size_t)i + k]);
-            A[k + 9] = 6.0 * (s.fdot[3 * (size_t)jcm + k] - s.fdot[3 * (size_t)i + k]);
+            A[k] = s.x(j, k) - xi[k];
+            A[k + 3] = s.xdot(j, k) - vi[k];
+            A[k + 6] = 2.0 * (s.f(jcm, k) - s.f(i, k));
+            A[k + 9] = 6.0 * (s.fdot(jcm, k) - s.fdot(i, k));
         }
     } else {
-        const double sj = time - s.t0[j];
+        const double sj = time - s.t0(j);
         const double s3 = 3.0 * sj;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            const double fj = s.f[3 * (size_t)j + k], fdj = s.fdot[3 * (size_t)j + k];
-            A[k] = s.x[3 * (size_t)j + k] - xi[k];
-            A[k + 3] = (fdj * s3 + 2.0 * fj) * sj + s.x0dot[3 * (size_t)j + k] - vi[k];
-            A[k + 6] = 2.0 * (fdj * s3 + fj - s.f[3 * (size_t)i + k]);
-            A[k + 9] = 6.0 * (fdj - s.fdot[3 * (size_t)i + k]);
+            const double fj = s.f(j, k), fdj = s.fdot(j, k);
+            A[k] = s.x(j, k) - xi[k];
+            A[k + 3] = (fdj * s3 + 2.0 * fj) * sj + s.x0dot(j, k) - vi[k];
+            A[k + 6] = 2.0 * (fdj * s3 + fj - s.f(i, k));
+            A[k + 9] = 6.0 * (fdj - s.fdot(i, k));
         }
     }
     const double a13 = 1.0 / (A[0] * A[0] + A[1] * A[1] + A[2] * A[2]);
-    const double a14 = s.body[j] * a13 * sqrt(a13);
+    const double a14 = s.body(j) * a13 * sqrt(a13);
     const double a15 = (A[0] * A[3] + A[1] * A[4] + A[2] * A[5]) * a13;
     const double a16 = a15 * a15;
     const double a17 = 3.0 * a15;
@@ -352,11 +390,12 @@ enum { HI_FLAGS = 0, HI_NNB = 1, HI_NNB0 = 2, HI_J3 = 3, NHI = 6 };
 // every lane computes from the same inputs are uniform, list work is spread over the
 // lanes, lane 0 alone runs the rare SMIN retention.  hd, hi: this particle's column of
 // the hand-over (stride ni).
-__device__ void regcor_wave(const nb6_regcor_params &P, const nb6_regstate &s, int I,
-                            const double *fg, int fstride, int *row, int *cnt, double *T,
+__device__ void regcor_wave(const nb6_regcor_params &P, const nb6_regstate &s, const double *pk,
+                            int I, const double *fg, int fstride, int *row, int *cnt, double *T,
                             int *nl, int *ol, unsigned *lb, unsigned *cb, int lane, double *hd,
                             int *hi, int ni) {
 #pragma clang fp contract(off)
+    const NbSrc nb{s, pk};
     const int lmax = s.lmax;
     const double time = P.time;
     if (I < 1 || I > s.nmax) { cnt[C_UNS]++; return; }
@@ -453,12 +492,11 @@ __device__ void regcor_wave(const nb6_regcor_params &P, const nb6_regstate &s, i
                 const int t = ol[l];
                 lost = !lds_contains(nl, nnb, t);
                 if (lost) {
-                    const double st = s.step[t - 1];
+                    const double st = nb.step(t - 1);
                     jm = jm || st < P.smin;
                     if (!(st > P.smin || t < P.ifirst || t > P.n)) {
-                        const size_t j3 = 3 * (size_t)(t - 1);
-                        const double e0 = xi[0] - s.x[j3], e1 = xi[1] - s.x[j3 + 1],
-                                     e2 = xi[2] - s.x[j3 + 2];
+                        const double e0 = xi[0] - nb.x(t - 1, 0), e1 = xi[1] - nb.x(t - 1, 1),
+                                     e2 = xi[2] - nb.x(t - 1, 2);
                         cand = !(e0 * e0 + e1 * e1 + e2 * e2 > 4.0 * rs2);
                     }
                 }
@@ -525,7 +563,7 @@ __device__ void regcor_wave(const nb6_regcor_params &P, const nb6_regstate &s, i
                 nbloss--;
                 cnt[C_SMIN]++;
                 double fa[3], fj[3];
-                pair_fj(s, vo - 1, xi, vi, fa, fj);
+                pair_fj(nb, vo - 1, xi, vi, fa, fj);
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
                     freg[k] = freg[k] - fa[k];
@@ -567,12 +605,12 @@ __device__ void regcor_wave(const nb6_regcor_params &P, const nb6_regstate &s, i
             double tv[NTERM];
             const int pos = lanes_below(m);
             if (j1) {
-                pair_fj(s, j1 - 1, xi, vi, tv, tv + 3);
+                pair_fj(nb, j1 - 1, xi, vi, tv, tv + 3);
                 if (fps) {
                     // a KS component J < IFIRST: c.m. body N + KVEC(J), KVEC(J) = (J + 1) / 2
                     ks = j1 < P.ifirst;
                     const int jcm = ks ? P.n + (j1 + 1) / 2 - 1 : -1;
-                    fp_terms(s, time, i, j1 - 1, jcm, xi, vi, tv + 6, tv + 9);
+                    fp_terms(nb, time, i, j1 - 1, jcm, xi, vi, tv + 6, tv + 9);
                 }
             }
             cnt[C_CMDER] += __popcll(__ballot(ks));
@@ -768,6 +806,7 @@ __device__ void regcor_finish(const nb6_regcor_params &P, const nb6_regstate &s,
 }
 
 __global__ __launch_bounds__(256) void regcor_lists_kernel(nb6_regcor_params P, nb6_regstate s,
+                                                           const double *__restrict__ pk,
                                                            int ni, const int *__restrict__ addr,
                                                            const double *__restrict__ force,
                                                            int fstride, int *__restrict__ nlist,
@@ -789,8 +828,8 @@ __global__ __launch_bounds__(256) void regcor_lists_kernel(nb6_regcor_params P, 
         unsigned *lb = reinterpret_cast<unsigned *>(ol + (s.lmax + 1));
         unsigned *cb = lb + loss_words(s.lmax);
         if (lane == 0) hi[HI_FLAGS * ni + k] = 0;
-        regcor_wave(P, s, addr[k], force + k, fstride, nlist + (size_t)k * s.lmax, cnt, T, nl, ol,
-                    lb, cb, lane, hd + k, hi + k, ni);
+        regcor_wave(P, s, pk, addr[k], force + k, fstride, nlist + (size_t)k * s.lmax, cnt, T, nl,
+                    ol, lb, cb, lane, hd + k, hi + k, ni);
     }
     if (lane == 0) {
 #pragma unroll
@@ -809,9 +848,17 @@ __global__ __launch_bounds__(256) void regcor_finish_kernel(nb6_regcor_params P,
     if (k < ni && (hi[HI_FLAGS * ni + k] & 1)) regcor_finish(P, s, addr[k], hd + k, hi + k, ni);
 }
 
-DBuf g_hand;
+DBuf g_hand, g_pack;
+constexpr size_t PACK_RATIO = 32;   // pack when ni >= nmax / 32
+int g_pack_mode = 0;                // 0: by block size, 1: always, 2: never
 
 }  // namespace
+
+int regcor_set_pack(int mode) {
+    if (mode < 0 || mode > 2) { set_error(E_ARG, "nb6_set_regcor_pack: mode must be 0, 1 or 2"); return E_ARG; }
+    g_pack_mode = mode;
+    return OK;
+}
 
 int regcor_device(const nb6_regcor_params *p, const nb6_regstate *s, int ni, const int *d_addr,
                   const double *d_force, int force_stride, int *d_nlist, int *d_counters,
@@ -858,8 +905,19 @@ int regcor_device(const nb6_regcor_params *p, const nb6_regstate *s, int ni, con
     char *h = static_cast<char *>(g_hand.get((size_t)ni * (NHD * 8 + NHI * 4)));
     double *hd = reinterpret_cast<double *>(h);
     int *hi = reinterpret_cast<int *>(h + (size_t)ni * NHD * 8);
+    // large blocks: one packed record per particle first (a pass over all nmax
+    // particles, about 300 bytes each), so each gathered neighbour costs two cache
+    // lines instead of eight; small blocks gather from the state arrays directly
+    double *pk = nullptr;
+    if (g_pack_mode == 1 || (g_pack_mode == 0 && (size_t)ni * PACK_RATIO >= (size_t)s->nmax)) {
+        pk = static_cast<double *>(g_pack.get((size_t)s->nmax * PK_STRIDE * sizeof(double)));
+        hipLaunchKernelGGL(regcor_pack_kernel, dim3((s->nmax + 255) / 256), dim3(256), 0, stream,
+                           *s, pk);
+        NB6_HIP(hipGetLastError());
+    }
     hipLaunchKernelGGL(regcor_lists_kernel, dim3((ni + wpb - 1) / wpb), dim3(WAVE * wpb), lds,
-                       stream, *p, *s, ni, d_addr, d_force, force_stride, d_nlist, d_counters, hd, hi);
+                       stream, *p, *s, pk, ni, d_addr, d_force, force_stride, d_nlist, d_counters,
+                       hd, hi);
     NB6_HIP(hipGetLastError());
     hipLaunchKernelGGL(regcor_finish_kernel, dim3((ni + 255) / 256), dim3(256), 0, stream, *p, *s,
                        ni, d_addr, hd, hi);

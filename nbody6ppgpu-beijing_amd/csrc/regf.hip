@@ -51,9 +51,10 @@ namespace {
 
 constexpr int RF_FLUSH = 64;          // j per FP32 partial sum
 constexpr int NACC = 7;               // FP32 partials: acc x y z, pot, jerk x y z
-// FP64 partial rows per (i, chunk): acc x y z, jerk x y z, pot, then the three error
-// scales of the FP32 sums (see tile_work): acc flush L1, jerk flush L1, fast records
-constexpr int NPART = 10;
+// FP64 partial rows per (i, chunk): acc x y z, jerk x y z, pot, then the four error
+// scales of the FP32 sums (see tile_work)
+constexpr int NERR = 4;
+constexpr int NPART = 7 + NERR;
 constexpr int CAP_SOFT = 1024;        // soft (near-field) slots per (i, chunk)
 constexpr int BAND_SLOTS = 16;        // of which kept for band pairs of overflowing rows
 constexpr int NB_SORT_MAX = 2048;     // largest neighbour row the finalize sorts
@@ -67,7 +68,6 @@ constexpr double H_REL = 64.0 * U24;              // relative slack on h2
 constexpr float D_ABS = (float)(64.0 * U24);      // absolute slack factor (x2 spare)
 constexpr float Q_REL = (float)(192.0 * U24);     // cheap threshold over the full one
                                                   // (includes 1/(1-KAPPA_Q))
-constexpr float F_NEAR = 2.0f;                    // near field: r < 2 RS
 constexpr float TINY = 1.1754944e-38f;            // keeps r2 = 0 a candidate
 
 struct TileParams {
@@ -89,13 +89,14 @@ struct TileParams {
     int caph;
     int *__restrict__ sslot;         // [(chunk*ni + i) * caps + k]
     int caps;
+    float fnear;                     // near field: r < fnear RS (soft candidates)
 };
 
 // per-lane constants that do not depend on the tile
 struct ILane {
     double x, y, z, u, v, w;         // FP64 position / velocity
     float dt1, dt2;                  // dtr, 2 dtr
-    float h2H, h2S;                  // (1+slack) h2 and F_NEAR^2 (1+slack) h2 (MFLAG: slopes)
+    float h2H, h2S;                  // (1+slack) h2 and fnear^2 (1+slack) h2 (MFLAG: slopes)
     float h2L;                       // (1-slack) h2 (MFLAG: slope)
     float rsH, rsS;                  // radii used in the absolute error bands
     bool ok;
@@ -108,11 +109,13 @@ struct IC {
     float hH, dH, hS, dS, hQ, dQ;    // thresholds (MFLAG: slopes h*, offsets d*)
     float hL;                        // below it a pair is an FP64 neighbour for certain
                                      // (MFLAG: slope, offset -2 dH)
+    float eP, eV;                    // bounds of |x_i - P| + |x_j - P| and of
+                                     // |v_i - V| + |v_j - V| over the tile's records
 };
 
 template <bool MFLAG>
 __device__ __forceinline__ ILane make_lane(const double *__restrict__ ib, int is, int i, bool ok,
-                                           double mmax) {
+                                           double mmax, float fnear) {
     ILane L;
     const int ii = ok ? i : 0;
     L.x = ib[0 * is + ii]; L.y = ib[1 * is + ii]; L.z = ib[2 * is + ii];
@@ -125,17 +128,18 @@ __device__ __forceinline__ ILane make_lane(const double *__restrict__ ib, int is
     const double hh = h2 > 0.0 ? h2 : 0.0;
     const double hk = hh * (1.0 + H_REL) * (1.0 + 4.0 * U24);
     L.h2H = (float)hk;
-    L.h2S = (float)(hk * (double)(F_NEAR * F_NEAR));
+    L.h2S = (float)(hk * (double)(fnear * fnear));
     L.h2L = (float)(hh * (1.0 - H_REL) * (1.0 - 12.0 * U24));
     const double rs = MFLAG ? sqrt(hh * mmax) : sqrt(hh);
     L.rsH = (float)(rs * (1.0 + 4.0 * U24));
-    L.rsS = F_NEAR * L.rsH;
+    L.rsS = fnear * L.rsH;
     L.ok = ok;
     return L;
 }
 
 template <bool MFLAG>
-__device__ __forceinline__ IC tile_ic(const ILane &L, const double *__restrict__ T) {
+__device__ __forceinline__ IC tile_ic(const ILane &L, const double *__restrict__ T,
+                                      const float *__restrict__ TB) {
     IC c;
     c.x = (float)(L.x - T[0]); c.y = (float)(L.y - T[1]); c.z = (float)(L.z - T[2]);
     c.u = (float)(L.u - T[3]); c.v = (float)(L.v - T[4]); c.w = (float)(L.w - T[5]);
@@ -147,6 +151,20 @@ __device__ __forceinline__ IC tile_ic(const ILane &L, const double *__restrict__
     const float g = A + fabsf(c.dt1) * B;
     const float dH = D_ABS * L.rsH * (g + L.rsH) + TINY;
     const float dS = D_ABS * L.rsS * (g + L.rsS) + TINY;
+    // the tile's records: |x_j - P| and |v_j - V| are at most the largest corner of
+    // its position and velocity boxes (per component)
+    float xm = 0.f, vm = 0.f;
+    if (TB[13] > 0.f) {                  // a tile without records has no bounds
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const float xq = fmaxf(fabsf(TB[q]), fabsf(TB[3 + q]));
+            const float vq = fmaxf(fabsf(TB[6 + q]), fabsf(TB[9 + q]));
+            xm = fmaf(xq, xq, xm);
+            vm = fmaf(vq, vq, vm);
+        }
+    }
+    c.eP = A + sqrtf(xm) * 1.0001f;
+    c.eV = B + sqrtf(vm) * 1.0001f;
     if (MFLAG) {
         c.hH = L.h2H; c.dH = dH;
         c.hS = L.h2S; c.dS = dS;
@@ -229,7 +247,7 @@ template <bool MFLAG, bool TEST>
 __device__ __forceinline__ void pair_step(const TileParams &p, const IC &ck, const float4 a,
                                           const float4 b, int jg, int j1, int &nhk, int &nsk,
                                           int *__restrict__ hpk, int *__restrict__ spk,
-                                          float *__restrict__ acc, float &efast) {
+                                          float *__restrict__ acc, float &s3) {
     {
         const PairBase q = pair_base(ck, a, b);
         // v_rsq_f32 plus one Newton-Raphson step
@@ -294,10 +312,9 @@ __device__ __forceinline__ void pair_step(const TileParams &p, const IC &ck, con
         acc[4] = fmaf(mr3, fmaf(al, q.dx, q.du), acc[4]);
         acc[5] = fmaf(mr3, fmaf(al, q.dy, q.dv), acc[5]);
         acc[6] = fmaf(mr3, fmaf(al, q.dz, q.dw), acc[6]);
-        // error scale of the jerk terms of fast records (b.w = the record's speed
-        // excess, zero for most): their terms can cancel within a flush (the two
-        // members of a tight binary), where the flush sums would not show them
-        efast = fmaf(mr3, b.w, efast);
+        // sum of m/r^3: times the tile's bounds of the FP32 rounding of positions
+        // and velocities it bounds the rounding error of the pair terms
+        s3 += mr3;
     }
 }
 
@@ -313,7 +330,8 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
     int *hp[IPL], *sp[IPL];
     float a32[IPL][NACC];
     double a64[IPL][7];
-    float err[IPL][3];               // error scales of the FP32 sums (FP32 is enough)
+    float err[IPL][NERR];            // error scales of the FP32 sums (FP32 is enough)
+    float s3[IPL];                   // sum of m/r^3 over the current 64 j
 
 #pragma unroll
     for (int k = 0; k < IPL; k++) {
@@ -322,7 +340,7 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
         int r = i;
         if (in && p.imap) r = p.imap[i];
         const bool ok = in && r >= 0;
-        L[k] = make_lane<MFLAG>(p.ib, p.ib_stride, r, ok, p.mmax);
+        L[k] = make_lane<MFLAG>(p.ib, p.ib_stride, r, ok, p.mmax, p.fnear);
         nh[k] = 0;
         ns[k] = 0;
         const size_t row = (size_t)chunk * p.ni + (in ? i : 0);
@@ -332,7 +350,9 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
         for (int f = 0; f < NACC; f++) a32[k][f] = 0.f;
 #pragma unroll
         for (int f = 0; f < 7; f++) a64[k][f] = 0.0;
-        err[k][0] = err[k][1] = err[k][2] = 0.f;
+#pragma unroll
+        for (int f = 0; f < NERR; f++) err[k][f] = 0.f;
+        s3[k] = 0.f;
     }
 
     const int ntile = (j1 - j0 + T - 1) / T;
@@ -367,7 +387,7 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
                 bool far = p.far_test != 0;
 #pragma unroll
                 for (int k = 0; k < IPL; k++) {
-                    c[k] = tile_ic<MFLAG>(L[k], p.torg + tt * 8);
+                    c[k] = tile_ic<MFLAG>(L[k], p.torg + tt * 8, p.tbox + tt * TBOX);
                     if (p.far_test) far = far && lane_far<MFLAG>(c[k], L[k].ok, p.tbox + tt * TBOX);
                 }
                 allfar = __ballot(!far) == 0ull;
@@ -379,7 +399,7 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
 #pragma unroll
                     for (int k = 0; k < IPL; k++)
                         pair_step<MFLAG, false>(p, c[k], a, b, jt + jj, j1, nh[k], ns[k], hp[k],
-                                                sp[k], a32[k], err[k][2]);
+                                                sp[k], a32[k], s3[k]);
                 }
             } else {
 #pragma unroll 2
@@ -388,7 +408,7 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
 #pragma unroll
                     for (int k = 0; k < IPL; k++)
                         pair_step<MFLAG, true>(p, c[k], a, b, jt + jj, j1, nh[k], ns[k], hp[k],
-                                               sp[k], a32[k], err[k][2]);
+                                               sp[k], a32[k], s3[k]);
                 }
             }
             // flush FP32 partials into FP64
@@ -401,9 +421,13 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
                 a64[k][4] += (double)a32[k][5];
                 a64[k][5] += (double)a32[k][6];
                 a64[k][6] += (double)a32[k][3];
-                // L1 norms of the flushed FP32 partials: the scale of their rounding
+                // L1 norms of the flushed FP32 partials: the scale of their rounding;
+                // sum of m/r^3 times the position and velocity rounding bounds
                 err[k][0] += fabsf(a32[k][0]) + fabsf(a32[k][1]) + fabsf(a32[k][2]);
                 err[k][1] += fabsf(a32[k][4]) + fabsf(a32[k][5]) + fabsf(a32[k][6]);
+                err[k][2] = fmaf(s3[k], c[k].eP, err[k][2]);
+                err[k][3] = fmaf(s3[k], c[k].eV, err[k][3]);
+                s3[k] = 0.f;
 #pragma unroll
                 for (int f = 0; f < NACC; f++) a32[k][f] = 0.f;
             }
@@ -421,7 +445,7 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
 #pragma unroll
             for (int f = 0; f < 7; f++) pp[(size_t)f * p.ni] = a64[k][f];
 #pragma unroll
-            for (int f = 0; f < 3; f++) pp[(size_t)(7 + f) * p.ni] = (double)err[k][f];
+            for (int f = 0; f < NERR; f++) pp[(size_t)(7 + f) * p.ni] = (double)err[k][f];
             p.cnt[((size_t)chunk * 2 + 0) * p.ni + i] = nh[k];
             p.cnt[((size_t)chunk * 2 + 1) * p.ni + i] = ns[k];
         }
@@ -674,28 +698,65 @@ struct FinParams {
     int *__restrict__ list;          // [ni][lmax]
     int refine;                      // flag rows whose FP32 error scale is too large
     int row_base;                    // added to r for fix_rows (batches without imap)
-    int *__restrict__ fix_rows;      // flagged caller rows (refine)
-    int *__restrict__ nfix;          // their count (device)
+    int fix_slot;                    // fix_rows holds slots i instead of caller rows
+    float kf[NERR];                  // weights of the error scales (row_bounds)
+    int *__restrict__ fix_rows;      // rows for the whole-row FP64 pass: caller rows, or
+    int *__restrict__ nfix;          // slots (fix_slot); their count (device)
+    // chunk-selective FP64 pass (plain calls): per flagged row the chunks to recompute,
+    // as items k = (slot, chunk) stored contiguously per row in chunk order, each also
+    // listed in its chunk's bucket (seg_items[c * ni + pos] = k, seg_cnt[c] entries), and
+    // one entry {caller row, first item, items} per row
+    int2 *__restrict__ items;
+    int *__restrict__ nitems;
+    int item_cap;
+    int *__restrict__ seg_items;
+    int *__restrict__ seg_cnt;
+    int4 *__restrict__ rowsel;
+    int *__restrict__ nrows;
+    const int *__restrict__ padoff;  // batched form: segments are the clusters [nseg + 1]
+    int nseg;
+    double *__restrict__ dbg;        // diagnostic: the error scales of each row [NERR][out_stride]
 };
 
 // Error scales of a row's FP32 sums -> does the row need the FP64 pass?  u = 2^-24.
 // e[0], e[1]: sums over the 64-j flushes of the L1 norms of the FP32 acc and jerk
-// partials; e[2]: sum over fast records of m/r^3 times their speed excess.  The
-// FP32 rounding of the sums is about u times these (random signs make it smaller);
-// a row is recomputed in FP64 when the scale, times the factors below, exceeds
-// REFINE_REL of its own |acc| or |jerk|.  Rows whose sums are NaN (band pairs left
-// without a slot) are recomputed too.
+// partials (the rounding of the sums themselves); e[2], e[3]: sums over the 64-j
+// segments of sum(m/r^3) times the bound of |x_i - P| + |x_j - P| and of
+// |v_i - V| + |v_j - V| in the segment's tile (the rounding of the FP32 separations
+// and velocity differences: a relative error u of each moves an acc term by at most
+// 2 m/r^3 u |x| and a jerk term by m/r^3 u |v| plus its position part).  A row is
+// recomputed in FP64 when k0 u e[0] + k2 u e[2] exceeds REFINE_REL |acc| or
+// k1 u e[1] + k3 u e[3] + k2 u e[2] |jerk| / |acc| exceeds REFINE_REL |jerk|.  Rows
+// whose sums are NaN (band pairs left without a slot) are recomputed too.
 constexpr double REFINE_REL = 1.0e-6;
-constexpr double REFINE_KA = 0.5;
-constexpr double REFINE_KJ = 0.5;
-constexpr double REFINE_KF = 1.0;
-__device__ __forceinline__ bool needs_fp64(const double *s, const double *e, bool nan_row) {
-    if (nan_row) return true;
+struct RowRef {
+    double ta, tj;       // allowed acc and jerk error: REFINE_REL |acc|, REFINE_REL |jerk|
+    double jpa;          // |jerk| / |acc|
+};
+__device__ __forceinline__ RowRef row_ref(const double *s) {
+    RowRef q;
     const double a = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
     const double j = sqrt(s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
-    const double ea = REFINE_KA * U24 * e[0];
-    const double ej = U24 * (REFINE_KJ * e[1] + REFINE_KF * e[2]);
-    return !(ea <= REFINE_REL * a) || !(ej <= REFINE_REL * j);
+    q.ta = REFINE_REL * a;
+    q.tj = REFINE_REL * j;
+    q.jpa = j / (a + 1e-300);
+    return q;
+}
+// bound of the acc and jerk errors from the error scales e[NERR] (of a row or a chunk)
+__device__ __forceinline__ void err_bound(const double *e, const float *k, const RowRef &q,
+                                          double &ea, double &ej) {
+    const double ep = U24 * (double)k[2] * e[2];
+    ea = U24 * (double)k[0] * e[0] + ep;
+    // the position part of a jerk term's error relative to the jerk: as for acc
+    ej = U24 * ((double)k[1] * e[1] + (double)k[3] * e[3]) + ep * q.jpa;
+}
+__device__ __forceinline__ bool needs_fp64(const double *s, const double *e, bool nan_row,
+                                           const float *k) {
+    if (nan_row) return true;
+    const RowRef q = row_ref(s);
+    double ea, ej;
+    err_bound(e, k, q, ea, ej);
+    return !(ea <= q.ta) || !(ej <= q.tj);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -837,6 +898,94 @@ __device__ __forceinline__ void fin_candidates(const FinParams &f, const IF64 &I
 
 }
 
+// A flagged row whose sums are finite: the chunks to recompute in FP64, chosen so that
+// the bounds of the chunks left in FP32 add up to at most the row's allowed acc and jerk
+// errors: greedily by largest share of the allowance (at most 64 chunks, one per lane),
+// or, with more chunks, every chunk above 1/nchunk of it.  Run by one whole wavefront
+// (s: the row's sums and error scales, the same in every lane).  The row's items are
+// contiguous and in chunk order; a row whose items do not fit the item buffer takes the
+// whole-row pass instead.
+__device__ void emit_chunks(const FinParams &f, int i, int r, const double *s, int lane) {
+    if (f.padoff) {
+        // batched form: one item, the row's whole cluster (slot i in [padoff[c], padoff[c+1]))
+        if (lane == 0) {
+            int lo = 0, hi = f.nseg;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (f.padoff[mid] <= i) lo = mid; else hi = mid;
+            }
+            const int k = atomicAdd(f.nitems, 1);
+            if (k >= f.item_cap) {
+                f.fix_rows[atomicAdd(f.nfix, 1)] = f.fix_slot ? i : r + f.row_base;
+            } else {
+                f.items[k] = make_int2(i, lo);
+                const int pos = atomicAdd(&f.seg_cnt[lo], 1);
+                f.seg_items[(size_t)f.padoff[lo] + pos] = k;
+                f.rowsel[atomicAdd(f.nrows, 1)] = make_int4(r, k, 1, 0);
+            }
+        }
+        return;
+    }
+    const RowRef q = row_ref(s);
+    auto bound = [&](int c, double &ea, double &ej) {
+        ea = ej = 0.0;
+        if (c >= f.nchunk) return;
+        const double *pp = f.part + (size_t)c * NPART * f.ni + i;
+        double e[NERR];
+#pragma unroll
+        for (int k = 0; k < NERR; k++) e[k] = pp[(size_t)(7 + k) * f.ni];
+        err_bound(e, f.kf, q, ea, ej);
+    };
+    bool mine = false;                    // (greedy form) this lane's chunk is selected
+    if (f.nchunk <= 64) {
+        double ea, ej;
+        bound(lane, ea, ej);
+        const double w = fmax(ea / (q.ta + 1e-300), ej / (q.tj + 1e-300));
+        for (int it = 0; it < f.nchunk; it++) {
+            const double ra = wave_sum(mine ? 0.0 : ea), rj = wave_sum(mine ? 0.0 : ej);
+            if (ra <= q.ta && rj <= q.tj) break;
+            // the unselected chunk with the largest share (lowest lane on ties)
+            double best = mine || lane >= f.nchunk ? -1.0 : w;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) best = fmax(best, __shfl_xor(best, o, 64));
+            const unsigned long long m = __ballot(!mine && lane < f.nchunk && w == best);
+            if (!m) break;
+            mine = mine || lane == __builtin_ctzll(m);
+        }
+    }
+    auto sel = [&](int c) {
+        if (c >= f.nchunk) return false;
+        if (f.nchunk <= 64) return mine;
+        double ea, ej;
+        bound(c, ea, ej);
+        return !(ea <= q.ta / f.nchunk) || !(ej <= q.tj / f.nchunk);
+    };
+    int cnt = 0;
+    for (int cb = 0; cb < f.nchunk; cb += 64) cnt += __popcll(__ballot(sel(cb + lane)));
+    if (cnt == 0) return;    // within the bound with every chunk in FP32
+    int start = 0;
+    if (lane == 0) start = atomicAdd(f.nitems, cnt);
+    start = __shfl(start, 0, 64);
+    if (start + cnt > f.item_cap) {
+        if (lane == 0) f.fix_rows[atomicAdd(f.nfix, 1)] = r + f.row_base;
+        return;
+    }
+    int k = start;
+    for (int cb = 0; cb < f.nchunk; cb += 64) {
+        const int c = cb + lane;
+        const bool b = sel(c);
+        const unsigned long long m = __ballot(b);
+        if (b) {
+            const int kk = k + lane_prefix(m);
+            f.items[kk] = make_int2(i, c);
+            const int pos = atomicAdd(&f.seg_cnt[c], 1);
+            f.seg_items[(size_t)c * f.ni + pos] = kk;
+        }
+        k += __popcll(m);
+    }
+    if (lane == 0) f.rowsel[atomicAdd(f.nrows, 1)] = make_int4(r, start, cnt, 0);
+}
+
 template <bool MFLAG>
 __global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
     __shared__ int nbuf[4][NB_SORT_MAX];
@@ -849,16 +998,16 @@ __global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
     int *buf = nbuf[wid];
 
     // 1. FP64 sum of the chunk partials (and of the error scales)
-    double s[7], e[3] = {0.0, 0.0, 0.0};
+    double s[7], e[NERR] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < 7; q++) s[q] = 0.0;
     for (int c = lane; c < f.nchunk; c += 64) {
         const double *pp = f.part + (size_t)c * NPART * f.ni + i;
 #pragma unroll
         for (int q = 0; q < 7; q++) s[q] += pp[(size_t)q * f.ni];
-        if (f.refine) {
+        if (f.refine || f.dbg) {
 #pragma unroll
-            for (int q = 0; q < 3; q++) e[q] += pp[(size_t)(7 + q) * f.ni];
+            for (int q = 0; q < NERR; q++) e[q] += pp[(size_t)(7 + q) * f.ni];
         }
     }
 
@@ -879,9 +1028,9 @@ __global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
 
 #pragma unroll
     for (int q = 0; q < 7; q++) s[q] = wave_sum(s[q]);
-    if (f.refine) {
+    if (f.refine || f.dbg) {
 #pragma unroll
-        for (int q = 0; q < 3; q++) e[q] = wave_sum(e[q]);
+        for (int q = 0; q < NERR; q++) e[q] = wave_sum(e[q]);
     }
     int *row = f.list + (size_t)r * f.lmax;
     const int tot = nnb + dropped;
@@ -901,8 +1050,23 @@ __global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
         o[3 * os] = s[3]; o[4 * os] = s[4]; o[5 * os] = s[5];
         o[6 * os] = s[6];
         row[0] = ovf ? -tot : tot;
-        if (f.refine && needs_fp64(s, e, unres > 0))
-            f.fix_rows[atomicAdd(f.nfix, 1)] = r + f.row_base;
+    }
+    if (f.dbg && lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NERR; q++) f.dbg[(size_t)q * f.out_stride + r] = e[q];
+    }
+    // s and e are wave sums: every lane takes the same decision
+    if (f.refine && needs_fp64(s, e, unres > 0, f.kf)) {
+        if (unres > 0 || !f.items) {
+            if (lane == 0) f.fix_rows[atomicAdd(f.nfix, 1)] = f.fix_slot ? i : r + f.row_base;
+        } else {
+            double se[7 + NERR];
+#pragma unroll
+            for (int q = 0; q < 7; q++) se[q] = s[q];
+#pragma unroll
+            for (int q = 0; q < NERR; q++) se[7 + q] = e[q];
+            emit_chunks(f, i, r, se, lane);
+        }
     }
     if (!ovf && nnb > 0) {
         // ascending row: the rank of each (distinct) index among the row
@@ -926,27 +1090,32 @@ __global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
 template <bool MFLAG, int W>
 __global__ __launch_bounds__(64 * W) void regf_finalize_wide_kernel(FinParams f) {
     __shared__ int buf[NB_SORT_MAX];
-    __shared__ double red[W][10];
+    __shared__ double red[W][NPART];
     __shared__ int cnts[W][3];
     __shared__ int nshared;
+    __shared__ int flag;             // 1: chunk items, 2: whole-row pass
+    __shared__ double vtot[NPART];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int i = blockIdx.x;   // slot
     const int r = f.imap ? f.imap[i] : i;
     if (r < 0) return;   // the whole work-group leaves together
-    if (threadIdx.x == 0) nshared = 0;
+    if (threadIdx.x == 0) {
+        nshared = 0;
+        flag = 0;
+    }
     __syncthreads();
 
-    double s[10];
+    double s[NPART];
 #pragma unroll
-    for (int q = 0; q < 10; q++) s[q] = 0.0;
+    for (int q = 0; q < NPART; q++) s[q] = 0.0;
     for (int c = w + W * lane; c < f.nchunk; c += 64 * W) {
         const double *pp = f.part + (size_t)c * NPART * f.ni + i;
 #pragma unroll
         for (int q = 0; q < 7; q++) s[q] += pp[(size_t)q * f.ni];
-        if (f.refine) {
+        if (f.refine || f.dbg) {
 #pragma unroll
-            for (int q = 7; q < 10; q++) s[q] += pp[(size_t)q * f.ni];
+            for (int q = 7; q < NPART; q++) s[q] += pp[(size_t)q * f.ni];
         }
     }
     const int is = f.ib_stride;
@@ -960,10 +1129,10 @@ __global__ __launch_bounds__(64 * W) void regf_finalize_wide_kernel(FinParams f)
     fin_candidates<MFLAG, true>(f, I, i, jb, w, W, lane, s, buf, &nshared, nnb, dropped, unres,
                                 dropped_any);
 #pragma unroll
-    for (int q = 0; q < 10; q++) s[q] = wave_sum(s[q]);
+    for (int q = 0; q < NPART; q++) s[q] = wave_sum(s[q]);
     if (lane == 0) {
 #pragma unroll
-        for (int q = 0; q < 10; q++) red[w][q] = s[q];
+        for (int q = 0; q < NPART; q++) red[w][q] = s[q];
         cnts[w][0] = dropped;
         cnts[w][1] = unres;
         cnts[w][2] = dropped_any ? 1 : 0;
@@ -985,9 +1154,9 @@ __global__ __launch_bounds__(64 * W) void regf_finalize_wide_kernel(FinParams f)
     if (threadIdx.x == 0) {
         double *o = f.out + r;
         const int os = f.out_stride;
-        double v[10];
+        double v[NPART];
 #pragma unroll
-        for (int q = 0; q < 10; q++) {
+        for (int q = 0; q < NPART; q++) {
             v[q] = 0.0;   // a fixed order
 #pragma unroll
             for (int u = 0; u < W; u++) v[q] += red[u][q];
@@ -995,8 +1164,21 @@ __global__ __launch_bounds__(64 * W) void regf_finalize_wide_kernel(FinParams f)
 #pragma unroll
         for (int q = 0; q < 7; q++) o[q * os] = unr > 0 ? __builtin_nan("") : v[q];
         row[0] = ovf ? -tot : tot;
-        if (f.refine && needs_fp64(v, v + 7, unr > 0))
-            f.fix_rows[atomicAdd(f.nfix, 1)] = r + f.row_base;
+        flag = f.refine && needs_fp64(v, v + 7, unr > 0, f.kf) ? (unr > 0 || !f.items ? 2 : 1) : 0;
+        if (f.dbg) {
+#pragma unroll
+            for (int q = 0; q < NERR; q++) f.dbg[(size_t)q * f.out_stride + r] = v[7 + q];
+        }
+        if (flag == 2) f.fix_rows[atomicAdd(f.nfix, 1)] = f.fix_slot ? i : r + f.row_base;
+#pragma unroll
+        for (int q = 0; q < NPART; q++) vtot[q] = v[q];
+    }
+    __syncthreads();
+    if (flag == 1 && w == 0) {
+        double se[NPART];
+#pragma unroll
+        for (int q = 0; q < NPART; q++) se[q] = vtot[q];
+        emit_chunks(f, i, r, se, lane);
     }
     if (!ovf && nall > 0) {
         for (int k = threadIdx.x; k < nall; k += 64 * W) {
@@ -1027,12 +1209,13 @@ struct FixParams {
     double *__restrict__ part;       // [nch][7][nfix]
     double *__restrict__ out;        // SoA [7][out_stride]
     int out_stride;
+    int items;                       // work items aimed at (the launch's grid size)
 };
 
-__device__ __forceinline__ void fix_chunks(int nfix, int nj, int &nch, int &tpc) {
+__device__ __forceinline__ void fix_chunks(int nfix, int nj, int items, int &nch, int &tpc) {
     const int nib = (nfix + 255) / 256;
     const int ntile = (nj + JTILE - 1) / JTILE;
-    nch = max(1, min(FIX_ITEMS / nib, ntile));
+    nch = max(1, min(items / nib, ntile));
     tpc = (ntile + nch - 1) / nch;          // tiles per chunk
     nch = (ntile + tpc - 1) / tpc;
 }
@@ -1044,7 +1227,7 @@ __global__ __launch_bounds__(256) void regf_fix_kernel(FixParams f) {
     if (nfix == 0) return;
     const int nj = f.j1 - f.j0;
     int nch, tpc;
-    fix_chunks(nfix, nj, nch, tpc);
+    fix_chunks(nfix, nj, f.items, nch, tpc);
     const int nib = (nfix + 255) / 256;
     const int items = nib * nch;
     for (int it = blockIdx.x; it < items; it += gridDim.x) {   // uniform per work-group
@@ -1084,7 +1267,7 @@ __global__ __launch_bounds__(256) void regf_fix_reduce_kernel(FixParams f) {
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= nfix) return;
     int nch, tpc;
-    fix_chunks(nfix, f.j1 - f.j0, nch, tpc);
+    fix_chunks(nfix, f.j1 - f.j0, f.items, nch, tpc);
     double sum[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int ch = 0; ch < nch; ch++) {
 #pragma unroll
@@ -1095,12 +1278,243 @@ __global__ __launch_bounds__(256) void regf_fix_reduce_kernel(FixParams f) {
     for (int q = 0; q < 7; q++) f.out[(size_t)q * f.out_stride + r] = sum[q];
 }
 
+// Chunk-selective FP64 pass.  Item k = (slot i, chunk c) replaces the FP32 sums of
+// chunk c in row i by FP64: all of the chunk's pairs that are not FP64 neighbours
+// (pair64, the finalize's test), minus the candidates the finalize already added in
+// FP64 (the same slots and tests as fin_candidates), minus the chunk's FP32 partials.
+//  * regf_cand_kernel: one wavefront per item, the candidate terms -> cand[k];
+//  * regf_segfix_kernel: work units (chunk c, up to 256 of its items, sub-range s of
+//    FIX_SUB j): the sub-range's records pass through LDS once for all the unit's items,
+//    one thread per item -> part64[k][s];
+//  * regf_segapply_kernel: each row adds, in item and sub-range order,
+//    sum_s part64 - cand - FP32 partial of every one of its items.
+// Every value depends only on (i, c, s), so results do not depend on the order in which
+// items were emitted.
+constexpr int FIX_SUB = 4096;          // j per sub-range
+
+struct ChunkFixParams {
+    const double *__restrict__ ib;   // SoA [8][ib_stride] (as the finalize of the batch)
+    int ib_stride;
+    const int *__restrict__ imap;    // slot -> caller row (nullptr: identity)
+    const double *__restrict__ rec64;
+    int nj, clen, nchunk, ni, nsub;
+    const double *__restrict__ part;
+    const int *__restrict__ cnt;
+    const int *__restrict__ hslot;
+    int caph;
+    const int *__restrict__ sslot;
+    int caps;
+    const int2 *__restrict__ items;
+    const int *__restrict__ nitems;
+    int item_cap;
+    const int *__restrict__ seg_items;
+    const int *__restrict__ seg_cnt;
+    double *__restrict__ cand;       // [item_cap][7]
+    double *__restrict__ part64;     // [item_cap][nsub][7]
+    const int4 *__restrict__ rowsel;
+    const int *__restrict__ nrows;
+    double *__restrict__ out;        // SoA [7][out_stride] (of the batch)
+    int out_stride;
+    const int *__restrict__ padoff;  // batched form: segment c = cluster [padoff[c],
+    int maxseg;                      // padoff[c + 1]), work-array chunk 0; maxseg its
+};                                   // longest segment
+
+__device__ __forceinline__ double pick7(const double *v, int q) {
+    double r = v[0];
+#pragma unroll
+    for (int k = 1; k < 7; k++) r = q == k ? v[k] : r;
+    return r;
+}
+
+__device__ __forceinline__ IF64 load_if64(const double *__restrict__ ib, int is, int r) {
+    IF64 I;
+    I.x = ib[0 * is + r]; I.y = ib[1 * is + r]; I.z = ib[2 * is + r];
+    I.u = ib[3 * is + r]; I.v = ib[4 * is + r]; I.w = ib[5 * is + r];
+    I.h2 = ib[6 * is + r]; I.dtr = ib[7 * is + r];
+    return I;
+}
+
+// one wavefront per row entry (only rows whose items were all stored), its items in turn
+template <bool MFLAG>
+__global__ __launch_bounds__(256) void regf_cand_kernel(ChunkFixParams f) {
+    const int lane = threadIdx.x & 63;
+    const int nr = *f.nrows;
+    for (int e = blockIdx.x * 4 + (threadIdx.x >> 6); e < nr; e += gridDim.x * 4)
+    for (int k = f.rowsel[e].y; k < f.rowsel[e].y + f.rowsel[e].z; k++) {
+        const int2 it = f.items[k];
+        const int i = it.x, c = f.padoff ? 0 : it.y;   // work-array chunk
+        const IF64 I = load_if64(f.ib, f.ib_stride, f.imap ? f.imap[i] : i);
+        double cd[7] = {0, 0, 0, 0, 0, 0, 0};
+        const size_t rowc = (size_t)c * f.ni + i;
+        const int nh = min(f.cnt[((size_t)c * 2 + 0) * f.ni + i], f.caph);
+        const int nsr = min(f.cnt[((size_t)c * 2 + 1) * f.ni + i], f.caps);
+        for (int q = lane; q < nh; q += 64) {
+            const int sj = f.hslot[rowc * f.caph + q];
+            if ((unsigned)sj < (unsigned)f.nj) (void)classify<MFLAG>(f.rec64 + (size_t)sj * 8, I, true, cd);
+        }
+        for (int q = lane; q < nsr; q += 64) {
+            const int e = f.sslot[rowc * f.caps + q];
+            const bool band = e < 0;
+            const int sj = band ? ~e : e;
+            if ((unsigned)sj < (unsigned)f.nj) (void)classify<MFLAG>(f.rec64 + (size_t)sj * 8, I, band, cd);
+        }
+#pragma unroll
+        for (int q = 0; q < 7; q++) cd[q] = wave_sum(cd[q]);
+        if (lane < 7) f.cand[(size_t)k * 7 + lane] = pick7(cd, lane);
+    }
+}
+
+template <bool MFLAG>
+__global__ __launch_bounds__(256) void regf_segfix_kernel(ChunkFixParams f) {
+    __shared__ double4 sA[JTILE], sB[JTILE];
+    const int t = threadIdx.x;
+    const int gmax = ((f.padoff ? f.maxseg : f.ni) + 255) / 256;
+    const long long units = (long long)f.nchunk * gmax * f.nsub;
+    for (long long u = blockIdx.x; u < units; u += gridDim.x) {   // uniform per work-group
+        const int c = (int)(u / ((long long)gmax * f.nsub));
+        const int rem = (int)(u - (long long)c * gmax * f.nsub);
+        const int g = rem / f.nsub, sub = rem - g * f.nsub;
+        const int nc = f.seg_cnt[c];
+        if (g * 256 >= nc) continue;
+        // (the last chunk or a short cluster: trailing sub-ranges are empty and write 0)
+        const int sb = f.padoff ? f.padoff[c] : c * f.clen;
+        const int se = f.padoff ? f.padoff[c + 1] : min(f.nj, (c + 1) * f.clen);
+        const int j0 = sb + sub * FIX_SUB;
+        const int j1 = min(se, j0 + FIX_SUB);
+        const bool ok = g * 256 + t < nc;
+        const size_t bucket = f.padoff ? (size_t)sb : (size_t)c * f.ni;
+        const int k = ok ? f.seg_items[bucket + g * 256 + t] : 0;
+        IF64 I;
+        if (ok) {
+            const int i = f.items[k].x;
+            I = load_if64(f.ib, f.ib_stride, f.imap ? f.imap[i] : i);
+        }
+        double sum[7] = {0, 0, 0, 0, 0, 0, 0};
+        for (int jt = j0; jt < j1; jt += JTILE) {
+            __syncthreads();
+            const int j = jt + t;
+            if (j < j1) {
+                sA[t] = reinterpret_cast<const double4 *>(f.rec64)[2 * (size_t)j];
+                sB[t] = reinterpret_cast<const double4 *>(f.rec64)[2 * (size_t)j + 1];
+            }
+            __syncthreads();
+            const int jn = min(JTILE, j1 - jt);
+            if (ok)
+                for (int jj = 0; jj < jn; jj++) (void)pair64<MFLAG>(sA[jj], sB[jj], I, true, sum);
+        }
+        if (ok) {
+            double *o = f.part64 + ((size_t)k * f.nsub + sub) * 7;
+#pragma unroll
+            for (int q = 0; q < 7; q++) o[q] = sum[q];
+        }
+    }
+}
+
+// each flagged row: its items' corrections in item and sub-range order
+__global__ __launch_bounds__(256) void regf_segapply_kernel(ChunkFixParams f) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= *f.nrows) return;
+    const int4 e = f.rowsel[k];
+    double d[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int m = 0; m < e.z; m++) {
+        const int kk = e.y + m;
+        const int2 it = f.items[kk];
+        const double *pp = f.part + (size_t)(f.padoff ? 0 : it.y) * NPART * f.ni + it.x;
+        double v[7] = {0, 0, 0, 0, 0, 0, 0};
+        for (int sb = 0; sb < f.nsub; sb++) {
+#pragma unroll
+            for (int q = 0; q < 7; q++) v[q] += f.part64[((size_t)kk * f.nsub + sb) * 7 + q];
+        }
+#pragma unroll
+        for (int q = 0; q < 7; q++) d[q] += (v[q] - f.cand[(size_t)kk * 7 + q]) - pp[(size_t)q * f.ni];
+    }
+#pragma unroll
+    for (int q = 0; q < 7; q++) f.out[(size_t)q * f.out_stride + e.x] += d[q];
+}
+
+// Ensemble rows: one work-group per flagged slot, over the padded j range of the
+// slot's cluster (padoff: cluster starts in slot order, found by binary search).
+// Padding records are massless and far, so they add nothing.  Thread t sums
+// j = lo + t + 256 k; the 256 sums are added by a fixed tree, so a row's result does
+// not depend on the work-group that takes it.
+struct FixRowParams {
+    const double *__restrict__ ib;   // SoA [8][ib_stride] (caller rows)
+    int ib_stride;
+    const int *__restrict__ imap;    // slot -> caller row
+    const int *__restrict__ slots;   // flagged slots
+    const int *__restrict__ nfix;
+    const int *__restrict__ padoff;  // [nc + 1]
+    int nc;
+    const double *__restrict__ rec64;
+    double *__restrict__ out;        // SoA [7][out_stride]
+    int out_stride;
+};
+
+template <bool MFLAG>
+__global__ __launch_bounds__(256) void regf_fix_rows_kernel(FixRowParams f) {
+    __shared__ double red[7][256];
+    const int nfix = *f.nfix;
+    const int t = threadIdx.x;
+    for (int k = blockIdx.x; k < nfix; k += gridDim.x) {   // uniform per work-group
+        const int slot = f.slots[k];
+        const int r = f.imap[slot];
+        int lo = 0, hi = f.nc;   // the cluster c with padoff[c] <= slot < padoff[c + 1]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (f.padoff[mid] <= slot) lo = mid; else hi = mid;
+        }
+        const int j0 = f.padoff[lo], j1 = f.padoff[lo + 1];
+        const int is = f.ib_stride;
+        IF64 I;
+        I.x = f.ib[0 * is + r]; I.y = f.ib[1 * is + r]; I.z = f.ib[2 * is + r];
+        I.u = f.ib[3 * is + r]; I.v = f.ib[4 * is + r]; I.w = f.ib[5 * is + r];
+        I.h2 = f.ib[6 * is + r]; I.dtr = f.ib[7 * is + r];
+        double sum[7] = {0, 0, 0, 0, 0, 0, 0};
+        for (int j = j0 + t; j < j1; j += 256) {
+            const double4 r0 = reinterpret_cast<const double4 *>(f.rec64)[2 * (size_t)j];
+            const double4 r1 = reinterpret_cast<const double4 *>(f.rec64)[2 * (size_t)j + 1];
+            (void)pair64<MFLAG>(r0, r1, I, true, sum);
+        }
+#pragma unroll
+        for (int q = 0; q < 7; q++) red[q][t] = sum[q];
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (t < o) {
+#pragma unroll
+                for (int q = 0; q < 7; q++) red[q][t] += red[q][t + o];
+            }
+            __syncthreads();
+        }
+        if (t < 7) f.out[(size_t)t * f.out_stride + r] = red[t][0];
+        __syncthreads();   // red is reused by the next row
+    }
+}
+
 __global__ void zero_int_kernel(int *p) { *p = 0; }
+__global__ void zero_ints_kernel(int *p, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0;
+}
+// between batches: rows and items of the batch so far into [3], [4]; per-batch
+// counters (and the nchunk bucket counts after them) cleared
+__global__ void next_batch_kernel(int *c, int nchunk) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        c[3] += c[2];
+        c[4] += c[1];
+        c[1] = 0;
+        c[2] = 0;
+    }
+    for (int k = threadIdx.x; k < nchunk; k += blockDim.x) c[6 + k] = 0;
+}
+constexpr int CHUNKFIX_GROUPS = 2048;
 
 RegfWork g_work;
 int g_last_chunks = 0;
 int g_last_batches = 0;
 int *g_nfix = nullptr;       // device count of the rows of the last call's FP64 pass
+int *g_ens_nfix = nullptr;   // the same for the last batched (ensemble) step
+double *g_dbg = nullptr;     // diagnostic output of the rows' error scales (or null)
 bool g_last_refine = false;
 
 template <int T, int IPL>
@@ -1178,8 +1592,46 @@ RegfConfig regf_effective(int ni) {
     return c;
 }
 
+// Carves the caller's work memory (no allocation): regions are 256-byte aligned, so a
+// set of regions needs up to 255 bytes each beyond their sizes.
+constexpr size_t ARENA_SLACK = 8 * 256;
+struct Arena {
+    unsigned char *p;
+    size_t left;
+    void *take(size_t n) {
+        n = (n + 255) & ~(size_t)255;
+        if (n > left) return nullptr;
+        void *r = p;
+        p += n;
+        left -= n;
+        return r;
+    }
+};
+
+// na == nullptr: the library's own, growing work buffers.  With na, every work array
+// comes from na (at most rows_bytes of it for the per-(i, chunk) rows), the i-set is
+// not Morton-ordered (an ordering only, results do not depend on it), the number of
+// j-chunks is capped so one i-block of rows fits, no kernel timers are recorded, and
+// nothing is allocated.
+struct NoAlloc {
+    Arena *arena;
+    size_t rows_bytes;
+    int fix_items;
+};
+
+int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, int lmax,
+                     int nnbmax, int m_flag, double *d_out, int *d_list, hipStream_t s,
+                     const NoAlloc *na);
+
 int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lmax,
                 int nnbmax, int m_flag, double *d_out, int *d_list, hipStream_t s) {
+    return regf_launch_impl(js, ni, ni_stride, d_i, lmax, nnbmax, m_flag, d_out, d_list, s,
+                            nullptr);
+}
+
+int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, int lmax,
+                     int nnbmax, int m_flag, double *d_out, int *d_list, hipStream_t s,
+                     const NoAlloc *na) {
     if (ni <= 0) return OK;
     if (js.n <= 0) {
         set_error(E_STATE, "gpunb_regf: no j-particles sent");
@@ -1195,14 +1647,26 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
     const int nj = js.n;
     const int unit = std::max(cfg.threads, JTILE);       // chunk granularity (j)
     const int nunit = (nj + unit - 1) / unit;
+    const int nnbeff = std::max(0, std::min(nnbmax, lmax - 1));
     // small i-sets get chunks down to one unit each, so that even ni = 1 spreads over
     // the whole GPU
-    const int want = (cfg.min_groups + nib - 1) / nib;
+    int want = (cfg.min_groups + nib - 1) / nib;
+    if (na) {
+        // one i-block of rows (largest row size, FP64-pass items included) must fit the
+        // caller's work memory
+        const size_t rb = NPART * sizeof(double) + 2 * sizeof(int) +
+                          ((size_t)nnbeff + 8 + (size_t)CAP_SOFT) * sizeof(int) + sizeof(int);
+        const size_t cmax = (na->rows_bytes - ARENA_SLACK) / ((size_t)nbi * rb);
+        if (cmax < 1) {
+            set_error(E_ARG, "regf: work memory too small for one i-block");
+            return E_ARG;
+        }
+        want = (int)std::min<size_t>((size_t)want, cmax);
+    }
     int nchunk = std::max(1, std::min(want, nunit));
     const int upc = (nunit + nchunk - 1) / nchunk;        // units per chunk
     const int clen = upc * unit;
     nchunk = (nunit + upc - 1) / upc;
-    const int nnbeff = std::max(0, std::min(nnbmax, lmax - 1));
     // a chunk cannot hold more candidates than records
     const int caph = std::min(nnbeff + 8, clen);
     const int caps = std::min(CAP_SOFT, clen);
@@ -1213,9 +1677,13 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
     // Work arrays hold one row per (i, chunk): FP64 partials, counts and the hard and
     // soft candidate slots.  Calls whose rows would exceed the slot budget run in
     // batches of i (whole i-blocks); chunks and results do not change.
+    // FP64 pass: per (i, chunk) a bucket entry
+    const bool refine = cfg.refine != 0 && cfg.prec == 0;
+    const size_t item_bytes = refine ? sizeof(int) : 0;
     const size_t row_bytes = NPART * sizeof(double) + 2 * sizeof(int) +
-                             ((size_t)caph + (size_t)caps) * sizeof(int);
-    const size_t budget = (size_t)std::max(cfg.slot_budget_mb, 1) << 20;
+                             ((size_t)caph + (size_t)caps) * sizeof(int) + item_bytes;
+    const size_t budget = na ? na->rows_bytes - ARENA_SLACK
+                             : (size_t)std::max(cfg.slot_budget_mb, 1) << 20;
     int nbatch_i = ni;
     if ((size_t)nchunk * (size_t)ni * row_bytes > budget) {
         const size_t rows_max = budget / row_bytes;
@@ -1223,30 +1691,86 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
         nbatch_i = std::min(nbatch_i, ni);
     }
     const size_t rows = (size_t)nchunk * nbatch_i;
-    double *part = static_cast<double *>(g_work.part.get(rows * NPART * sizeof(double)));
-    int *cnt = static_cast<int *>(g_work.cnt.get(rows * 2 * sizeof(int)));
-    int *hslot = static_cast<int *>(g_work.hslot.get(rows * caph * sizeof(int)));
-    int *sslot = static_cast<int *>(g_work.sslot.get(rows * caps * sizeof(int)));
+    // rows the finalize flags for the FP64 pass (FP32 production kernel only)
+    const int fix_items = na ? na->fix_items : FIX_ITEMS;
+    const size_t fix_cap = std::max((size_t)fix_items * 256, (size_t)ni);
+    // items of the chunk-selective pass: (slot, chunk), candidate terms, FP64 sums per
+    // sub-range; at most 4 per row of a batch (more go to the whole-row pass)
+    const int nsub = (clen + FIX_SUB - 1) / FIX_SUB;
+    const size_t per_item = sizeof(int2) + 7 * sizeof(double) * (1 + (size_t)nsub);
+    size_t item_cap = std::min(rows, std::max<size_t>(4 * (size_t)nbatch_i, 65536));
+    double *part, *fixpart = nullptr, *cand = nullptr, *part64 = nullptr;
+    int *cnt, *hslot, *sslot, *fix_rows = nullptr, *nfix = nullptr, *ctr = nullptr;
+    int *seg_items = nullptr;
+    int2 *items = nullptr;
+    int4 *rowsel = nullptr;
+    if (na) {
+        Arena rows_ar{na->arena->p, std::min(na->rows_bytes, na->arena->left)};
+        part = static_cast<double *>(rows_ar.take(rows * NPART * sizeof(double)));
+        cnt = static_cast<int *>(rows_ar.take(rows * 2 * sizeof(int)));
+        hslot = static_cast<int *>(rows_ar.take(rows * caph * sizeof(int)));
+        sslot = static_cast<int *>(rows_ar.take(rows * caps * sizeof(int)));
+        if (refine) seg_items = static_cast<int *>(rows_ar.take(rows * sizeof(int)));
+        na->arena->take(na->rows_bytes);
+        if (refine) {
+            fix_rows = static_cast<int *>(na->arena->take((size_t)ni * sizeof(int)));
+            ctr = static_cast<int *>(na->arena->take((6 + (size_t)nchunk) * sizeof(int)));
+            fixpart = static_cast<double *>(na->arena->take(fix_cap * 7 * sizeof(double)));
+            rowsel = static_cast<int4 *>(na->arena->take((size_t)nbatch_i * sizeof(int4)));
+            // the items take what is left
+            const size_t left = na->arena->left > ARENA_SLACK ? na->arena->left - ARENA_SLACK : 0;
+            item_cap = std::min(item_cap, left / per_item);
+            items = static_cast<int2 *>(na->arena->take(item_cap * sizeof(int2)));
+            cand = static_cast<double *>(na->arena->take(item_cap * 7 * sizeof(double)));
+            part64 = static_cast<double *>(na->arena->take(item_cap * nsub * 7 * sizeof(double)));
+        }
+        if (!part || !cnt || !hslot || !sslot ||
+            (refine && (!fix_rows || !ctr || !fixpart || !rowsel || !seg_items || !items ||
+                        !cand || !part64))) {
+            set_error(E_ARG, "regf: the call does not fit the work memory");
+            return E_ARG;
+        }
+    } else {
+        part = static_cast<double *>(g_work.part.get(rows * NPART * sizeof(double)));
+        cnt = static_cast<int *>(g_work.cnt.get(rows * 2 * sizeof(int)));
+        hslot = static_cast<int *>(g_work.hslot.get(rows * caph * sizeof(int)));
+        sslot = static_cast<int *>(g_work.sslot.get(rows * caps * sizeof(int)));
+        if (refine) {
+            fix_rows = static_cast<int *>(g_work.fixrows.get((size_t)ni * sizeof(int)));
+            ctr = static_cast<int *>(g_work.nfix.get((6 + (size_t)nchunk) * sizeof(int)));
+            fixpart = static_cast<double *>(g_work.fixpart.get(fix_cap * 7 * sizeof(double)));
+            seg_items = static_cast<int *>(g_work.segitems.get(rows * sizeof(int)));
+            items = static_cast<int2 *>(g_work.items.get(item_cap * sizeof(int2)));
+            cand = static_cast<double *>(g_work.delta.get(item_cap * 7 * sizeof(double)));
+            part64 = static_cast<double *>(g_work.part64.get(item_cap * nsub * 7 * sizeof(double)));
+            rowsel = static_cast<int4 *>(g_work.rowsel.get((size_t)nbatch_i * sizeof(int4)));
+        }
+    }
+    int *seg_cnt = ctr ? ctr + 6 : nullptr;   // items per chunk of the current batch
+    // counters: [0] whole-row passes, [1] chunk items, [2] rows with items (this batch),
+    // [3] rows with items and [4] items of the earlier batches
+    int *nitems = ctr ? ctr + 1 : nullptr, *nrows = ctr ? ctr + 2 : nullptr;
     g_last_chunks = nchunk;
     g_last_batches = (ni + nbatch_i - 1) / nbatch_i;
-    // rows the finalize flags for the FP64 pass (FP32 production kernel only)
-    const bool refine = cfg.refine != 0 && cfg.prec == 0;
-    int *fix_rows = nullptr;
     if (refine) {
-        fix_rows = static_cast<int *>(g_work.fixrows.get((size_t)ni * sizeof(int)));
-        g_nfix = static_cast<int *>(g_work.nfix.get(sizeof(int)));
-        hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, s, g_nfix);
+        nfix = ctr;
+        g_nfix = ctr;
+        hipLaunchKernelGGL(zero_ints_kernel, dim3((6 + nchunk + 255) / 256), dim3(256), 0, s, ctr,
+                           6 + nchunk);
     }
     g_last_refine = refine;
 
     // Morton-ordered i: a wavefront's particles are neighbours in space, so their
     // candidates fall on the same j and most tiles are far from all of them
-    const int *imap = cfg.isort_min > 0 && ni >= cfg.isort_min
+    const int *imap = !na && cfg.isort_min > 0 && ni >= cfg.isort_min
                           ? jset_sort_i(js, ni, ni_stride, d_i, g_work, s)
                           : nullptr;
 
-    KTimer &t1 = ktimer("regf_tile");
-    KTimer &t2 = ktimer("regf_finalize");
+    KTimer dummy;   // never started: with caller work memory no events are recorded
+    KTimer &t1 = na ? dummy : ktimer("regf_tile");
+    KTimer &t2 = na ? dummy : ktimer("regf_finalize");
+    KTimer &t4 = na ? dummy : ktimer("regf_chunkfix");
+    const bool timed = !na;
     for (int b0 = 0; b0 < ni; b0 += nbatch_i) {
         const int nb = std::min(nbatch_i, ni - b0);
         // slots [b0, b0 + nb): through imap the rows are absolute; without it the
@@ -1268,11 +1792,12 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
         tp.mmax = js.mmax;
         tp.part = part; tp.cnt = cnt;
         tp.hslot = hslot; tp.caph = caph; tp.sslot = sslot; tp.caps = caps;
+        tp.fnear = cfg.near;
         dim3 grid(bnib * nchunk);
-        if (b0 == 0) t1.start(s);
+        if (timed && b0 == 0) t1.start(s);
         launch_tile(cfg, m_flag != 0, grid, tp, s);
         NB6_HIP(hipGetLastError());
-        if (b0 + nb >= ni) t1.stop(s);
+        if (timed && b0 + nb >= ni) t1.stop(s);
 
         FinParams fp;
         fp.ib = bi; fp.ni = nb; fp.ib_stride = ni_stride; fp.imap = bmap;
@@ -1285,9 +1810,19 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
         fp.out = bout; fp.out_stride = ni_stride; fp.list = blist;
         fp.refine = refine ? 1 : 0;
         fp.row_base = imap ? 0 : b0;
+        fp.fix_slot = 0;
+        for (int q = 0; q < NERR; q++) fp.kf[q] = cfg.kref[q];
         fp.fix_rows = fix_rows;
-        fp.nfix = g_nfix;
-        if (b0 == 0) t2.start(s);
+        fp.nfix = nfix;
+        fp.items = items; fp.nitems = nitems;
+        fp.item_cap = (int)std::min<size_t>(item_cap, INT32_MAX);
+        fp.seg_items = seg_items; fp.seg_cnt = seg_cnt;
+        fp.padoff = nullptr; fp.nseg = 0;
+        fp.dbg = g_dbg ? (imap ? g_dbg : g_dbg + b0) : nullptr;
+        fp.rowsel = rowsel; fp.nrows = nrows;
+        if (refine && b0 > 0)   // item counters per batch (the rows' total is kept in [3])
+            hipLaunchKernelGGL(next_batch_kernel, dim3(1), dim3(256), 0, s, ctr, nchunk);
+        if (timed && b0 == 0) t2.start(s);
         if (fin_form == 8) {
             // few i: eight wavefronts per i
             if (m_flag)
@@ -1308,27 +1843,55 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
                 hipLaunchKernelGGL(regf_finalize_kernel<false>, fgrid, dim3(256), 0, s, fp);
         }
         NB6_HIP(hipGetLastError());
-        if (b0 + nb >= ni) t2.stop(s);
+        if (timed && b0 + nb >= ni) t2.stop(s);
+        if (refine) {
+            // chunk-selective FP64 pass of this batch's flagged rows, while its work
+            // arrays (partials, slots) are still there
+            ChunkFixParams cp;
+            cp.ib = bi; cp.ib_stride = ni_stride; cp.imap = bmap;
+            cp.rec64 = js.rec64.as<double>(); cp.nj = nj; cp.clen = clen; cp.nchunk = nchunk;
+            cp.ni = nb; cp.nsub = nsub;
+            cp.part = part; cp.cnt = cnt; cp.hslot = hslot; cp.caph = caph;
+            cp.sslot = sslot; cp.caps = caps;
+            cp.items = items; cp.nitems = nitems; cp.item_cap = fp.item_cap;
+            cp.seg_items = seg_items; cp.seg_cnt = seg_cnt;
+            cp.cand = cand; cp.part64 = part64;
+            cp.rowsel = rowsel; cp.nrows = nrows;
+            cp.out = bout; cp.out_stride = ni_stride;
+            cp.padoff = nullptr; cp.maxseg = 0;
+            if (timed && b0 == 0) t4.start(s);
+            if (m_flag) {
+                hipLaunchKernelGGL(regf_cand_kernel<true>, dim3(CHUNKFIX_GROUPS), dim3(256), 0, s, cp);
+                hipLaunchKernelGGL(regf_segfix_kernel<true>, dim3(CHUNKFIX_GROUPS), dim3(256), 0, s, cp);
+            } else {
+                hipLaunchKernelGGL(regf_cand_kernel<false>, dim3(CHUNKFIX_GROUPS), dim3(256), 0, s, cp);
+                hipLaunchKernelGGL(regf_segfix_kernel<false>, dim3(CHUNKFIX_GROUPS), dim3(256), 0, s, cp);
+            }
+            NB6_HIP(hipGetLastError());
+            hipLaunchKernelGGL(regf_segapply_kernel, dim3((nb + 255) / 256), dim3(256), 0, s, cp);
+            NB6_HIP(hipGetLastError());
+            if (timed && b0 + nb >= ni) t4.stop(s);
+        }
     }
     if (refine) {
-        // FP64 pass over the flagged rows (their count stays on the device)
-        const size_t cap = std::max((size_t)FIX_ITEMS * 256, (size_t)ni);
+        // whole-row FP64 pass over the rows with NaN sums (their count stays on the device)
         FixParams xp;
         xp.ib = d_i; xp.ib_stride = ni_stride;
-        xp.rows = fix_rows; xp.nfix = g_nfix;
+        xp.rows = fix_rows; xp.nfix = nfix;
         xp.rec64 = js.rec64.as<double>(); xp.j0 = 0; xp.j1 = nj;
-        xp.part = static_cast<double *>(g_work.fixpart.get(cap * 7 * sizeof(double)));
+        xp.part = fixpart;
         xp.out = d_out; xp.out_stride = ni_stride;
-        KTimer &t3 = ktimer("regf_fix");
-        t3.start(s);
+        xp.items = fix_items;
+        KTimer &t3 = na ? dummy : ktimer("regf_fix");
+        if (timed) t3.start(s);
         if (m_flag)
-            hipLaunchKernelGGL(regf_fix_kernel<true>, dim3(FIX_ITEMS), dim3(256), 0, s, xp);
+            hipLaunchKernelGGL(regf_fix_kernel<true>, dim3(fix_items), dim3(256), 0, s, xp);
         else
-            hipLaunchKernelGGL(regf_fix_kernel<false>, dim3(FIX_ITEMS), dim3(256), 0, s, xp);
+            hipLaunchKernelGGL(regf_fix_kernel<false>, dim3(fix_items), dim3(256), 0, s, xp);
         NB6_HIP(hipGetLastError());
         hipLaunchKernelGGL(regf_fix_reduce_kernel, dim3((ni + 255) / 256), dim3(256), 0, s, xp);
         NB6_HIP(hipGetLastError());
-        t3.stop(s);
+        if (timed) t3.stop(s);
     }
     return OK;
 }
@@ -1336,9 +1899,9 @@ int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lm
 // Batched form (ensemble.hip): one persistent launch over work items, nchunk = 1,
 // work rows indexed by slot, caller rows by imap[slot].
 int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, const double *d_i,
-                      const int4 *items, int nitems, int *queue, const int *jbase, int lmax,
-                      int nnbmax, int m_flag, double *d_out, int *d_list, int grid, RegfWork &w,
-                      hipStream_t s) {
+                      const int4 *items, int nitems, int *queue, const int *jbase,
+                      const int *padoff, int nc, int maxseg, int lmax, int nnbmax, int m_flag,
+                      double *d_out, int *d_list, int grid, RegfWork &w, hipStream_t s) {
     if (lmax < 1 || lmax > NB_SORT_MAX + 1) {
         set_error(E_ARG, "nb6_ens_regf_device: lmax outside [1, 2049]");
         return E_ARG;
@@ -1351,8 +1914,19 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     NB6_HIP(hipStreamIsCapturing(s, &cs));
     const bool capturing = cs != hipStreamCaptureStatusNone;
+    const bool refine = cfg.refine != 0;
+    // FP64 pass: one item per flagged row (its whole cluster), items bucketed by cluster
+    const int nsub = (maxseg + FIX_SUB - 1) / FIX_SUB;
+    const size_t item_cap = std::max<size_t>(rows / 2, 1);
+    const size_t ctr_bytes = (6 + (size_t)nc) * sizeof(int);
     if (capturing && (w.part.bytes < rows * NPART * sizeof(double) || w.hslot.bytes < rows * caph * sizeof(int) ||
-                      w.sslot.bytes < rows * caps * sizeof(int) || w.cnt.bytes < rows * 2 * sizeof(int))) {
+                      w.sslot.bytes < rows * caps * sizeof(int) || w.cnt.bytes < rows * 2 * sizeof(int) ||
+                      (refine && (w.fixrows.bytes < rows * sizeof(int) || w.nfix.bytes < ctr_bytes ||
+                                  w.segitems.bytes < rows * sizeof(int) ||
+                                  w.items.bytes < item_cap * sizeof(int2) ||
+                                  w.delta.bytes < item_cap * 7 * sizeof(double) ||
+                                  w.part64.bytes < item_cap * nsub * 7 * sizeof(double) ||
+                                  w.rowsel.bytes < rows * sizeof(int4))))) {
         set_error(E_STATE, "regf_items_launch: work buffers must be sized before stream capture");
         return E_STATE;
     }
@@ -1360,6 +1934,21 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
     int *cnt = static_cast<int *>(w.cnt.get(rows * 2 * sizeof(int)));
     int *hslot = static_cast<int *>(w.hslot.get(rows * caph * sizeof(int)));
     int *sslot = static_cast<int *>(w.sslot.get(rows * caps * sizeof(int)));
+    int *fix_slots = nullptr, *nfix = nullptr, *ctr = nullptr, *seg_items = nullptr;
+    int2 *fitems = nullptr;
+    double *cand = nullptr, *part64 = nullptr;
+    int4 *rowsel = nullptr;
+    if (refine) {
+        fix_slots = static_cast<int *>(w.fixrows.get(rows * sizeof(int)));
+        ctr = static_cast<int *>(w.nfix.get(ctr_bytes));
+        seg_items = static_cast<int *>(w.segitems.get(rows * sizeof(int)));
+        fitems = static_cast<int2 *>(w.items.get(item_cap * sizeof(int2)));
+        cand = static_cast<double *>(w.delta.get(item_cap * 7 * sizeof(double)));
+        part64 = static_cast<double *>(w.part64.get(item_cap * nsub * 7 * sizeof(double)));
+        rowsel = static_cast<int4 *>(w.rowsel.get(rows * sizeof(int4)));
+        nfix = ctr;   // [0] whole rows, [1] items, [2] rows with items, [6 ..] per cluster
+        hipLaunchKernelGGL(zero_ints_kernel, dim3((6 + nc + 255) / 256), dim3(256), 0, s, ctr, 6 + nc);
+    }
     TileParams tp;
     tp.ib = d_i; tp.ni = nslots; tp.ib_stride = stride; tp.imap = imap;
     tp.rec32 = js.rec32.as<float4>(); tp.torg = js.torg.as<double>(); tp.nj = js.n;
@@ -1370,6 +1959,7 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
     tp.mmax = js.mmax;
     tp.part = part; tp.cnt = cnt;
     tp.hslot = hslot; tp.caph = caph; tp.sslot = sslot; tp.caps = caps;
+    tp.fnear = cfg.near;
     KTimer &t1 = ktimer("regf_items");
     if (!capturing) t1.start(s);
     launch_items(cfg, m_flag != 0, grid, tp, items, nitems, queue, s);
@@ -1383,24 +1973,85 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
     fp.hslot = hslot; fp.caph = caph; fp.sslot = sslot; fp.caps = caps;
     fp.lmax = lmax; fp.nnbmax = nnbeff;
     fp.out = d_out; fp.out_stride = stride; fp.list = d_list;
-    fp.refine = 0; fp.row_base = 0; fp.fix_rows = nullptr; fp.nfix = nullptr;
+    fp.refine = refine ? 1 : 0; fp.row_base = 0; fp.fix_slot = 1;
+    fp.items = fitems; fp.nitems = ctr ? ctr + 1 : nullptr;
+    fp.item_cap = (int)std::min<size_t>(item_cap, INT32_MAX);
+    fp.seg_items = seg_items; fp.seg_cnt = ctr ? ctr + 6 : nullptr;
+    fp.dbg = g_dbg;
+    fp.rowsel = rowsel; fp.nrows = ctr ? ctr + 2 : nullptr;
+    fp.padoff = padoff; fp.nseg = nc;
+    for (int q = 0; q < NERR; q++) fp.kf[q] = cfg.kref[q];
+    fp.fix_rows = fix_slots; fp.nfix = nfix;
     const dim3 fgrid((nslots + 3) / 4);
     if (m_flag)
         hipLaunchKernelGGL(regf_finalize_kernel<true>, fgrid, dim3(256), 0, s, fp);
     else
         hipLaunchKernelGGL(regf_finalize_kernel<false>, fgrid, dim3(256), 0, s, fp);
     NB6_HIP(hipGetLastError());
+    if (refine) {
+        // FP64 pass of the flagged rows: items bucketed by cluster, each cluster's records
+        // through LDS once per 256 of its rows
+        ChunkFixParams cp;
+        cp.ib = d_i; cp.ib_stride = stride; cp.imap = imap;
+        cp.rec64 = js.rec64.as<double>(); cp.nj = js.n; cp.clen = 0; cp.nchunk = nc;
+        cp.ni = nslots; cp.nsub = nsub;
+        cp.part = part; cp.cnt = cnt; cp.hslot = hslot; cp.caph = caph;
+        cp.sslot = sslot; cp.caps = caps;
+        cp.items = fitems; cp.nitems = ctr + 1; cp.item_cap = fp.item_cap;
+        cp.seg_items = seg_items; cp.seg_cnt = ctr + 6;
+        cp.cand = cand; cp.part64 = part64;
+        cp.rowsel = rowsel; cp.nrows = ctr + 2;
+        cp.out = d_out; cp.out_stride = stride;
+        cp.padoff = padoff; cp.maxseg = maxseg;
+        if (m_flag) {
+            hipLaunchKernelGGL(regf_cand_kernel<true>, dim3(CHUNKFIX_GROUPS), dim3(256), 0, s, cp);
+            hipLaunchKernelGGL(regf_segfix_kernel<true>, dim3(CHUNKFIX_GROUPS), dim3(256), 0, s, cp);
+        } else {
+            hipLaunchKernelGGL(regf_cand_kernel<false>, dim3(CHUNKFIX_GROUPS), dim3(256), 0, s, cp);
+            hipLaunchKernelGGL(regf_segfix_kernel<false>, dim3(CHUNKFIX_GROUPS), dim3(256), 0, s, cp);
+        }
+        NB6_HIP(hipGetLastError());
+        hipLaunchKernelGGL(regf_segapply_kernel, dim3((nslots + 255) / 256), dim3(256), 0, s, cp);
+        NB6_HIP(hipGetLastError());
+        // rows whose items did not fit and rows with NaN sums: whole rows
+        FixRowParams xp;
+        xp.ib = d_i; xp.ib_stride = stride; xp.imap = imap;
+        xp.slots = fix_slots; xp.nfix = nfix;
+        xp.padoff = padoff; xp.nc = nc;
+        xp.rec64 = js.rec64.as<double>();
+        xp.out = d_out; xp.out_stride = stride;
+        const dim3 xgrid(1024);
+        if (m_flag)
+            hipLaunchKernelGGL(regf_fix_rows_kernel<true>, xgrid, dim3(256), 0, s, xp);
+        else
+            hipLaunchKernelGGL(regf_fix_rows_kernel<false>, xgrid, dim3(256), 0, s, xp);
+        NB6_HIP(hipGetLastError());
+        g_ens_nfix = nfix;
+    }
     return OK;
+}
+
+void regf_items_forget() { g_ens_nfix = nullptr; }
+void regf_set_debug_bounds(double *d) { g_dbg = d; }
+
+int regf_items_refined() {
+    if (!g_ens_nfix) return 0;
+    int h[3] = {0, 0, 0};
+    NB6_HIP(hipDeviceSynchronize());
+    NB6_HIP(hipMemcpy(h, g_ens_nfix, sizeof(h), hipMemcpyDeviceToHost));
+    return h[0] + h[2];
 }
 
 int regf_last_chunks() { return g_last_chunks; }
 int regf_last_batches() { return g_last_batches; }
-int regf_last_refined() {
+int regf_last_refined(int *chunk_items) {
+    if (chunk_items) *chunk_items = 0;
     if (!g_last_refine || !g_nfix) return 0;
-    int h = 0;
+    int h[6] = {0, 0, 0, 0, 0, 0};
     NB6_HIP(hipDeviceSynchronize());   // the call may have run on any stream
-    NB6_HIP(hipMemcpy(&h, g_nfix, sizeof(int), hipMemcpyDeviceToHost));
-    return h;
+    NB6_HIP(hipMemcpy(h, g_nfix, sizeof(h), hipMemcpyDeviceToHost));
+    if (chunk_items) *chunk_items = h[1] + h[4];
+    return h[0] + h[2] + h[3];
 }
 
 // Sweep work-group size (= LDS j-tile) 128/256/512, i-particles per lane 1/2 and the

@@ -22,6 +22,8 @@
 //                        weights with a bfloat16 copy for the forward pass.
 #include "nb6_internal.h"
 
+#include <hipcub/hipcub.hpp>
+
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -56,6 +58,7 @@ __host__ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
 struct NetState {
     bool open = false;
     DBuf master, m1, m2, wbf, grad, part, loss, norm;   // norm: mu[8], 1/sigma[8]
+    DBuf lacc, keys, skeys, vals, perm, sort_tmp;          // epoch: loss sum, shuffle
     int step = 0;
 } g_net;
 
@@ -110,8 +113,12 @@ struct GradLds {
     float dy[GB];
 };
 
+// Minibatch of n samples: sample s is row idx[base + s] (idx null: base + s) of the
+// SoA features float[8][fstride] and of the targets.
 __global__ __launch_bounds__(GB) void rsnet_grad_kernel(int n, const float *__restrict__ feat,
                                                         const float *__restrict__ tgt,
+                                                        const int *__restrict__ idx, int base,
+                                                        int fstride,
                                                         const uint16_t *__restrict__ wbf,
                                                         const float *__restrict__ norm,
                                                         float *__restrict__ part,
@@ -121,8 +128,9 @@ __global__ __launch_bounds__(GB) void rsnet_grad_kernel(int n, const float *__re
     const int t = threadIdx.x;
     const int s = blockIdx.x * GB + t;
     const bool ok = s < n;
+    const int g = ok ? (idx ? idx[base + s] : base + s) : 0;
     float x[NF], h1[NH];
-    if (ok) load_features(feat, n, s, norm, x);
+    if (ok) load_features(feat, fstride, g, norm, x);
     else
         for (int f = 0; f < NF; f++) x[f] = 0.f;
     __syncthreads();
@@ -146,7 +154,7 @@ __global__ __launch_bounds__(GB) void rsnet_grad_kernel(int n, const float *__re
     }
 #pragma unroll
     for (int f = 0; f < NF; f++) L.x[t][f] = x[f];
-    const float r = ok ? y - tgt[s] : 0.f;
+    const float r = ok ? y - tgt[g] : 0.f;
     const float dy = 2.0f * r / (float)n;
     L.dy[t] = dy;
     __syncthreads();
@@ -233,7 +241,8 @@ __global__ __launch_bounds__(GB) void rsnet_grad_kernel(int n, const float *__re
 __global__ __launch_bounds__(256) void rsnet_sum_kernel(const float *__restrict__ part, int nblk,
                                                         float *__restrict__ grad,
                                                         const float *__restrict__ lpart,
-                                                        float *__restrict__ loss, int n) {
+                                                        float *__restrict__ loss, int n,
+                                                        double *__restrict__ lacc) {
     const int p = blockIdx.x * 256 + threadIdx.x;
     if (p < NPARAM) {
         float g = 0.f;
@@ -244,8 +253,25 @@ __global__ __launch_bounds__(256) void rsnet_sum_kernel(const float *__restrict_
         double l = 0.0;
         for (int b = 0; b < nblk; b++) l += lpart[b];
         *loss = (float)(l / n);
+        if (lacc) *lacc += l;   // sum of squared residuals over an epoch
     }
 }
+
+// epoch shuffle: a random 32-bit key per sample (SplitMix64 of seed and index), sorted
+__global__ __launch_bounds__(256) void rsnet_keys_kernel(int n, unsigned long long seed,
+                                                         unsigned *__restrict__ key,
+                                                         int *__restrict__ val) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (unsigned long long)(k + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    key[k] = (unsigned)(z >> 32);
+    val[k] = k;
+}
+
+__global__ void rsnet_zero_kernel(double *p) { *p = 0.0; }
 
 __global__ __launch_bounds__(256) void rsnet_adam_kernel(float *__restrict__ w, float *__restrict__ m1,
                                                          float *__restrict__ m2,
@@ -551,6 +577,8 @@ int rsnet_close() {
     if (!s.open) return OK;
     s.master.release(); s.m1.release(); s.m2.release(); s.wbf.release();
     s.grad.release(); s.part.release(); s.loss.release(); s.norm.release();
+    s.lacc.release(); s.keys.release(); s.skeys.release(); s.vals.release(); s.perm.release();
+    s.sort_tmp.release();
     s.open = false;
     return OK;
 }
@@ -611,12 +639,63 @@ int rsnet_grad(int n, const float *d_feat, const float *d_t, float *d_grad, floa
     const int nblk = (n + GB - 1) / GB;
     float *part = static_cast<float *>(g.part.get(((size_t)nblk * NPARAM + nblk) * sizeof(float)));
     float *lpart = part + (size_t)nblk * NPARAM;
-    hipLaunchKernelGGL(rsnet_grad_kernel, dim3(nblk), dim3(GB), 0, s, n, d_feat, d_t,
-                       g.wbf.as<uint16_t>(), g.norm.as<float>(), part, lpart);
+    hipLaunchKernelGGL(rsnet_grad_kernel, dim3(nblk), dim3(GB), 0, s, n, d_feat, d_t, nullptr, 0,
+                       n, g.wbf.as<uint16_t>(), g.norm.as<float>(), part, lpart);
     NB6_HIP(hipGetLastError());
     hipLaunchKernelGGL(rsnet_sum_kernel, dim3((NPARAM + 1 + 255) / 256), dim3(256), 0, s, part, nblk,
-                       d_grad ? d_grad : g.grad.as<float>(), lpart, d_loss ? d_loss : g.loss.as<float>(), n);
+                       d_grad ? d_grad : g.grad.as<float>(), lpart, d_loss ? d_loss : g.loss.as<float>(), n,
+                       nullptr);
     NB6_HIP(hipGetLastError());
+    return OK;
+}
+
+// One epoch on the device: the n samples (SoA float[8][n], targets[n]) shuffled by a
+// sort of seeded random keys, then n / batch Adam steps on consecutive minibatches of
+// the shuffled order with no host synchronisation until the end; *h_loss = mean
+// squared residual over the epoch (each residual before its step).
+int rsnet_train_epoch(int n, const float *d_feat, const float *d_t, int batch, float lr,
+                      unsigned long long seed, float *h_loss, hipStream_t s) {
+    if (!net_ready()) return E_STATE;
+    if (n <= 0 || batch <= 0 || batch > n) {
+        set_error(E_ARG, "nb6_rsnet_train_epoch_device: need 0 < batch <= n");
+        return E_ARG;
+    }
+    NetState &g = g_net;
+    unsigned *key = static_cast<unsigned *>(g.keys.get((size_t)n * sizeof(unsigned)));
+    unsigned *skey = static_cast<unsigned *>(g.skeys.get((size_t)n * sizeof(unsigned)));
+    int *val = static_cast<int *>(g.vals.get((size_t)n * sizeof(int)));
+    int *perm = static_cast<int *>(g.perm.get((size_t)n * sizeof(int)));
+    hipLaunchKernelGGL(rsnet_keys_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, seed, key, val);
+    NB6_HIP(hipGetLastError());
+    size_t tb = 0;
+    NB6_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key, skey, val, perm, n, 0, 32, s));
+    void *tmp = g.sort_tmp.get(tb);
+    NB6_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, key, skey, val, perm, n, 0, 32, s));
+    double *lacc = static_cast<double *>(g.lacc.get(sizeof(double)));
+    hipLaunchKernelGGL(rsnet_zero_kernel, dim3(1), dim3(1), 0, s, lacc);
+    const int nblk = (batch + GB - 1) / GB;
+    float *part = static_cast<float *>(g.part.get(((size_t)nblk * NPARAM + nblk) * sizeof(float)));
+    float *lpart = part + (size_t)nblk * NPARAM;
+    const int nb = n / batch;
+    for (int b = 0; b < nb; b++) {
+        hipLaunchKernelGGL(rsnet_grad_kernel, dim3(nblk), dim3(GB), 0, s, batch, d_feat, d_t, perm,
+                           b * batch, n, g.wbf.as<uint16_t>(), g.norm.as<float>(), part, lpart);
+        hipLaunchKernelGGL(rsnet_sum_kernel, dim3((NPARAM + 1 + 255) / 256), dim3(256), 0, s, part,
+                           nblk, g.grad.as<float>(), lpart, g.loss.as<float>(), batch, lacc);
+        g.step++;
+        const float c1 = 1.f - std::pow(0.9f, (float)g.step);
+        const float c2 = 1.f - std::pow(0.999f, (float)g.step);
+        hipLaunchKernelGGL(rsnet_adam_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, s,
+                           g.master.as<float>(), g.m1.as<float>(), g.m2.as<float>(),
+                           g.grad.as<float>(), g.wbf.as<uint16_t>(), lr, c1, c2);
+    }
+    NB6_HIP(hipGetLastError());
+    if (h_loss) {
+        double l = 0.0;
+        NB6_HIP(hipMemcpyAsync(&l, lacc, sizeof(double), hipMemcpyDeviceToHost, s));
+        NB6_HIP(hipStreamSynchronize(s));
+        *h_loss = (float)(l / ((double)nb * batch));
+    }
     return OK;
 }
 

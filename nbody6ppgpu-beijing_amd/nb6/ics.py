@@ -275,3 +275,20 @@ def lognormal_counts(n: int, seed: int, median: float = 80.0, sigma: float = 0.5
     rng = _rng(seed)
     c = np.exp(rng.normal(math.log(median), sigma, n))
     return np.clip(np.rint(c), lo, cap).astype(np.int64)
+
+
+def precision_study(nj: int = 1 << 20, ni: int = 1 << 17, seed: int = 20261204):
+    """The N = 1,048,576 precision-study system and its largest block-step i-set:
+    King W0 = 6 with 50 % of the systems split into binaries, Kroupa masses, all nj
+    as j-particles, ni i-particles drawn at random, RS giving log-normal neighbour
+    counts (median 80, cap 500), dtr log-uniform in [2^-10, 2^-4].  Returns m, x, v,
+    idx (sorted i indices), h2 = RS^2 and dtr per i."""
+    mass = kroupa_masses(nj, seed)
+    m, x, v = king(nj, seed + 1, w0=6.0, masses=mass)
+    m, x, v = add_binaries(m, x, v, 0.5, seed + 2)
+    rng = _rng(seed + 3)
+    idx = np.sort(rng.choice(nj, size=ni, replace=False))
+    counts = lognormal_counts(ni, seed + 4, median=80.0)
+    rs = rs_for_counts(x, idx, counts)
+    dtr = np.exp(rng.uniform(np.log(2.0 ** -10), np.log(2.0 ** -4), ni))
+    return m, x, v, idx, rs * rs, dtr

@@ -61,7 +61,13 @@ def load():
     L.nb6_last_regf_batches.restype = ctypes.c_int
     L.nb6_set_regf_slot_budget.argtypes = [ctypes.c_int]
     L.nb6_set_regf_refine.argtypes = [ctypes.c_int]
+    L.nb6_set_regf_near.argtypes = [ctypes.c_double]
+    L.nb6_set_regf_refine_weights.argtypes = [ctypes.c_double] * 4
+    L.nb6_set_regcor_pack.argtypes = [ctypes.c_int]
+    L.nb6_set_regf_debug_bounds.argtypes = [_vp]
     L.nb6_last_regf_refined.restype = ctypes.c_int
+    L.nb6_last_regf_refined_chunks.restype = ctypes.c_int
+    L.nb6_ens_last_refined.restype = ctypes.c_int
     L.nb6_dev_alloc.restype = _vp
     L.nb6_dev_alloc.argtypes = [ctypes.c_ulonglong]
     L.nb6_dev_free.argtypes = [_vp]
@@ -93,6 +99,7 @@ def load():
     L.nb6_rsnet_forward_device.argtypes = [i, _vp, _vp, _vp]
     L.nb6_rsnet_grad_device.argtypes = [i, _vp, _vp, _vp, _vp, _vp]
     L.nb6_rsnet_train_step_device.argtypes = [i, _vp, _vp, f, _vp, _vp]
+    L.nb6_rsnet_train_epoch_device.argtypes = [i, _vp, _vp, i, f, ctypes.c_ulonglong, _vp, _vp]
     L.nb6_rs_features_device.argtypes = [i, i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.nb6_rs_predict_device.argtypes = [i, _vp, _vp, _vp, _vp, _vp, _vp, i, i, d, _vp]
     L.nb6_rs_rule_device.argtypes = [i, i, _vp, _vp, _vp, _vp, _vp, _vp, i, _vp, _vp, i, _vp, i, i,

@@ -78,7 +78,12 @@ class Sim:
         self.t = 0.0
         self.t0 = np.zeros(n)
         self.stats = dict(calls=0, overflow_events=0, extra_pairs=0, steps=0, samples=0,
-                          miss30=0, absdev=0.0, fallback=0)
+                          miss30=0, absdev=0.0, fallback=0,
+                          # split by the particle's regular step: its first after start-up
+                          # (radius from start-up) or a later one (radius from the policy)
+                          overflow_first=0, overflow_later=0, samples_later=0,
+                          miss30_later=0, absdev_later=0.0)
+        self.nreg = np.zeros(n, np.int64)    # regular steps taken by each particle
         r = np.linalg.norm(self.x, axis=1)
         if rs0 is None:
             # radius holding NNBOPT particles at the mean density inside the half-mass radius
@@ -218,6 +223,11 @@ class Sim:
         while todo.size:
             chunk = np.minimum(self.maxthr, ni - (todo // self.maxthr) * self.maxthr)
             self.stats["overflow_events"] += int(todo.size)
+            first = getattr(self, "_first", None)
+            if first is not None and first.size == ni:
+                nf = int(first[todo].sum())
+                self.stats["overflow_first"] += nf
+                self.stats["overflow_later"] += int(todo.size) - nf
             self.stats["extra_pairs"] += int(chunk.sum()) * self.n
             rs[todo] = _overflow(rs[todo], cnt[todo], self.nnbopt, self.nnbmax)
             _, sub = self._regf(xi[todo], vi[todo], rs[todo] ** 2, dtr[todo], self.d.tmp,
@@ -244,11 +254,18 @@ class Sim:
         xi, vi, dtr = xp[act], vp[act], self.dt[act]
 
         rs = self.rs[act].copy()
+        self._first = self.nreg[act] == 0
         nnb_new = self._rs_call(xi, vi, rs, dtr)
         dev = np.abs(nnb_new - self.nnbopt) / self.nnbopt
+        lad = np.abs(np.log((nnb_new + 1.0) / (self.nnbopt + 1.0)))
         self.stats["miss30"] += int((dev > 0.3).sum())
-        self.stats["absdev"] += float(np.abs(np.log((nnb_new + 1.0) / (self.nnbopt + 1.0))).sum())
+        self.stats["absdev"] += float(lad.sum())
         self.stats["samples"] += ni
+        later = ~self._first
+        self.stats["miss30_later"] += int((dev[later] > 0.3).sum())
+        self.stats["absdev_later"] += float(lad[later].sum())
+        self.stats["samples_later"] += int(later.sum())
+        self.nreg[act] += 1
         if len(self.stats.setdefault("overflow_by_step", [])) < 8:
             self.stats["overflow_by_step"].append(self.stats["overflow_events"])
 
@@ -416,32 +433,27 @@ class RSNet:
             lib.check_error()
 
     def train(self, feat, tgt, epochs=4, batch=4096, lr=2e-3, seed=1, log=None):
-        """Adam on minibatches; features [n][8], targets [n].  Returns the loss history."""
+        """Adam on minibatches; features [n][8], targets [n].  The samples go to the
+        device once; each epoch is shuffled and run on the device
+        (nb6_rsnet_train_epoch_device).  Returns the loss history."""
         n = tgt.size
         mu = feat.mean(0)
         sigma = feat.std(0) + 1e-6
         self.set_norm(mu, sigma)
-        fd = lib.DeviceArray(8 * batch * 4)
-        td = lib.DeviceArray(batch * 4)
-        rng = np.random.Generator(np.random.PCG64(seed))
+        fd = lib.DeviceArray.from_host(np.ascontiguousarray(feat.T, np.float32))
+        td = lib.DeviceArray.from_host(np.ascontiguousarray(tgt, np.float32))
         hist = []
         loss = ctypes.c_float()
         t0 = time.time()
+        self.train_s = 0.0
         for ep in range(epochs):
-            perm = rng.permutation(n)
-            tot, cnt = 0.0, 0
-            for b in range(0, n - batch + 1, batch):
-                sel = perm[b:b + batch]
-                fb = np.ascontiguousarray(feat[sel].T, np.float32)
-                tb = np.ascontiguousarray(tgt[sel], np.float32)
-                self.L.nb6_memcpy_htod(fd.ptr, fb.ctypes.data, fb.nbytes)
-                self.L.nb6_memcpy_htod(td.ptr, tb.ctypes.data, tb.nbytes)
-                if self.L.nb6_rsnet_train_step_device(batch, fd.ptr, td.ptr, ctypes.c_float(lr),
-                                                      ctypes.byref(loss), None) != 0:
-                    lib.check_error()
-                tot += loss.value
-                cnt += 1
-            hist.append(tot / max(cnt, 1))
+            t1 = time.time()
+            if self.L.nb6_rsnet_train_epoch_device(n, fd.ptr, td.ptr, batch, ctypes.c_float(lr),
+                                                   seed * 1000003 + ep, ctypes.byref(loss),
+                                                   None) != 0:
+                lib.check_error()
+            self.train_s += time.time() - t1
+            hist.append(loss.value)
             if log:
                 log(f"rsnet epoch {ep}: loss {hist[-1]:.4f} ({time.time() - t0:.1f} s)")
         fd.free()

@@ -1,0 +1,240 @@
+"""The benchmark configurations at their stated sizes, on the GPU.
+
+* N = 65,536 Kroupa Plummer, the whole set as i (ni = 65,536): every particle
+  against the FP64 oracle.
+* N = 1,048,576 King W0 = 6 with 50 % binaries (the headline step, ni = 131,072,
+  the bench's launch shape: 512-wide groups, 16 j-chunks, Morton-sorted i):
+  every particle against the FP64 kernel, a sample against the FP64 oracle, and
+  the FP64 kernel against the oracle on that sample; block sizes drawn from
+  p(n) ~ 1/n on [1, 2^17]; gpupot over all N.
+* 256 Plummer clusters (N log-uniform in 1024-32768): the batched, graph-replayed
+  ensemble step against the FP64 kernel on every particle of every cluster.
+
+Bar (the library's accuracy target): lists identical to the FP64 lists; acc,
+jerk and pot within 1e-6 (per-particle vector relative error) of FP64 for every
+particle.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import c2_inputs, vec_rel_err
+from nb6 import ics, lib
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1.0e-6
+LMAX, NNBMAX = 600, 550
+
+
+def rel_cols(a, b):
+    """Per-column relative error of [k][n] arrays (vector norm over k)."""
+    a, b = np.atleast_2d(a), np.atleast_2d(b)
+    return np.linalg.norm(a - b, axis=0) / np.linalg.norm(b, axis=0)
+
+
+def same_rows(a, b):
+    """Rows equal as written (count, then the ascending neighbours)."""
+    if not np.array_equal(a[:, 0], b[:, 0]):
+        return False
+    c = np.maximum(a[:, 0], 0)
+    mask = np.arange(a.shape[1] - 1)[None, :] < c[:, None]
+    return bool(np.array_equal(np.where(mask, a[:, 1:], 0), np.where(mask, b[:, 1:], 0)))
+
+
+def check_every_particle(label, out, lst, ref, rlst):
+    ea, ej = rel_cols(out[0:3], ref[0:3]), rel_cols(out[3:6], ref[3:6])
+    ep = rel_cols(out[6:7], ref[6:7])
+    print(f"{label}: acc max {ea.max():.2e} | jerk max {ej.max():.2e} p99.9 "
+          f"{np.percentile(ej, 99.9):.2e} above 1e-6 {(ej > TOL).sum()} | pot max {ep.max():.2e}"
+          f" | rows {lst.shape[0]}", flush=True)
+    assert same_rows(lst, rlst), f"{label}: lists differ from FP64"
+    assert ea.max() <= TOL, f"{label}: acc {ea.max()}"
+    assert ej.max() <= TOL, f"{label}: jerk {ej.max()} ({(ej > TOL).sum()} above)"
+    assert ep.max() <= TOL, f"{label}: pot {ep.max()}"
+
+
+class Step:
+    """A device j-set and an i-block, through nb6_send_device / nb6_regf_device."""
+
+    def __init__(self, m, x, v):
+        self.L = lib.load()
+        self.nj = m.size
+        self.g = lib.Gpunb(self.nj)
+        self.dj = lib.DeviceArray.from_host(np.ascontiguousarray(np.concatenate([x.T, v.T, m[None]])))
+        assert self.L.nb6_send_device(self.nj, self.nj, self.dj.ptr, None) == 0
+        self.bufs = None
+
+    def regf(self, xi, vi, h2, dtr, lmax=LMAX, nnbmax=NNBMAX, prec=0, m_flag=0):
+        ni = xi.shape[0]
+        if self.bufs is None or self.bufs[0] < ni:
+            self.free_bufs()
+            self.bufs = (ni, lib.DeviceArray(8 * ni * 8), lib.DeviceArray(7 * ni * 8),
+                         lib.DeviceArray(ni * lmax * 4))
+        _, di, do, dl = self.bufs
+        ib = np.ascontiguousarray(np.concatenate([xi.T, vi.T, h2[None], dtr[None]]))
+        self.L.nb6_memcpy_htod(di.ptr, ib.ctypes.data, ib.nbytes)
+        self.L.nb6_set_regf_precision(prec)
+        try:
+            if self.L.nb6_regf_device(ni, ni, di.ptr, lmax, nnbmax, m_flag, do.ptr, dl.ptr, None):
+                lib.check_error()
+            self.L.nb6_sync()
+        finally:
+            self.L.nb6_set_regf_precision(0)
+        return do.to_host(np.float64, (7, ni)), dl.to_host(np.int32, (ni, lmax))
+
+    def free_bufs(self):
+        if self.bufs is not None:
+            for d in self.bufs[1:]:
+                d.free()
+        self.bufs = None
+
+    def close(self):
+        self.free_bufs()
+        self.dj.free()
+        self.g.close()
+
+
+def oracle_cols(m, x, v, h2, dtr, xi, vi, lmax, nnbmax, m_flag=0):
+    racc, rjrk, rpot, rlst = O.regf(m, x, v, h2, dtr, xi, vi, lmax, nnbmax, m_flag)
+    return np.concatenate([racc.T, rjrk.T, rpot[None]]), rlst
+
+
+# ----------------------------------------------------------------- N = 65,536
+
+
+@pytest.mark.parametrize("m_flag", [0, 1])
+def test_c2_whole_set_as_i(m_flag):
+    """ni = N = 65,536: every particle against the FP64 oracle."""
+    m, x, v, idx, h2, dtr = c2_inputs(ni=65536, m_flag=m_flag)
+    lmax = 500
+    nnbmax = lmax - 50
+    s = Step(m, x, v)
+    try:
+        out, lst = s.regf(x[idx], v[idx], h2, dtr, lmax, nnbmax, m_flag=m_flag)
+    finally:
+        s.close()
+    ref, rlst = oracle_cols(m, x, v, h2, dtr, x[idx], v[idx], lmax, nnbmax, m_flag)
+    check_every_particle(f"N=65536 ni=65536 m_flag={m_flag}", out, lst, ref, rlst)
+
+
+# ----------------------------------------------------------------- N = 2^20
+
+
+@pytest.fixture(scope="module")
+def king1m():
+    m, x, v, idx, h2, dtr = ics.precision_study()
+    s = Step(m, x, v)
+    yield m, x, v, idx, h2, dtr, s
+    s.close()
+
+
+def test_c3_headline_step_every_particle(king1m):
+    """The bench step: ni = 131,072 at the default launch shape.  Every particle
+    against the FP64 kernel; the FP64 kernel and the FP32 kernel against the CPU
+    oracle on 512 sampled particles."""
+    m, x, v, idx, h2, dtr, s = king1m
+    L = s.L
+    out, lst = s.regf(x[idx], v[idx], h2, dtr)
+    chunks, refined = L.nb6_last_regf_chunks(), L.nb6_last_regf_refined()
+    ref, rlst = s.regf(x[idx], v[idx], h2, dtr, prec=2)
+    print(f"headline step: {chunks} j-chunks, {refined} rows through the FP64 pass", flush=True)
+    check_every_particle("N=2^20 ni=131072", out, lst, ref, rlst)
+    k = np.random.default_rng(3).choice(idx.size, 512, replace=False)
+    oref, olst = oracle_cols(m, x, v, h2[k], dtr[k], x[idx[k]], v[idx[k]], LMAX, NNBMAX)
+    e64 = rel_cols(ref[0:3, k], oref[0:3]).max()
+    print(f"FP64 kernel vs oracle on 512 particles: acc {e64:.2e}", flush=True)
+    assert e64 <= 1e-12
+    check_every_particle("N=2^20 sample vs oracle", out[:, k], lst[k], oref, olst)
+
+
+def test_c3_power_law_blocks(king1m):
+    """Block i-sets with sizes drawn from p(n) ~ 1/n on [1, 2^17] at N = 2^20:
+    every particle of every block against the FP64 kernel."""
+    m, x, v, idx, h2, dtr, s = king1m
+    rng = np.random.Generator(np.random.PCG64(20261274))
+    sizes = np.unique(np.exp(rng.uniform(0.0, np.log(2.0 ** 17), 6)).astype(int).clip(1, 2 ** 17))
+    for ni in sizes:
+        sel = np.sort(rng.choice(idx.size, int(ni), replace=False))
+        xi, vi = x[idx[sel]], v[idx[sel]]
+        out, lst = s.regf(xi, vi, h2[sel], dtr[sel])
+        ref, rlst = s.regf(xi, vi, h2[sel], dtr[sel], prec=2)
+        check_every_particle(f"N=2^20 block ni={ni}", out, lst, ref, rlst)
+
+
+def test_c3_gpupot_all_n(king1m):
+    """gpupot over all N = 2^20 against the FP64 kernel's all-pairs potential on
+    4096 sampled bodies."""
+    m, x, v, idx, h2, dtr, s = king1m
+    L, nj = s.L, m.size
+    dx = lib.DeviceArray.from_host(np.ascontiguousarray(np.concatenate([x.T, m[None]])))
+    dp = lib.DeviceArray(nj * 8)
+    try:
+        assert L.nb6_pot_device(nj, nj, dx.ptr, 0, nj, dp.ptr, None) == 0
+        L.nb6_sync()
+        pot = dp.to_host(np.float64, nj)
+    finally:
+        dx.free()
+        dp.free()
+    ks = np.arange(0, nj, nj // 4096)
+    ref, _ = s.regf(x[ks], v[ks], np.zeros(ks.size), np.zeros(ks.size), 2, 1, prec=2)
+    ep = np.abs(pot[ks] - ref[6]) / np.abs(ref[6])
+    print(f"gpupot N=2^20: max rel err {ep.max():.2e} median {np.median(ep):.2e}", flush=True)
+    assert ep.max() <= TOL
+
+
+# ----------------------------------------------------------------- ensemble
+
+
+def test_c4_256_clusters_every_particle():
+    """The 256-cluster ensemble (2.3 M particles) in one graph-replayed batched
+    step: every particle of every cluster against the FP64 kernel run on that
+    cluster alone."""
+    rng = np.random.Generator(np.random.PCG64(20261244))
+    nc = 256
+    sizes = np.exp(rng.uniform(np.log(1024), np.log(32768), nc)).astype(int)
+    ms, xs, vs, h2s, dts = [], [], [], [], []
+    for c, n in enumerate(sizes):
+        m, x, v = ics.plummer(int(n), 20263204 + c, masses=ics.kroupa_masses(int(n), 20264204 + c))
+        rs = ics.rs_for_counts(x, np.arange(n), np.full(n, 50))
+        r2 = (x * x).sum(1)
+        ms.append(m); xs.append(x); vs.append(v); h2s.append(rs * rs)
+        dts.append(np.minimum(0.125 * np.sqrt(1 + r2), 0.125) * 0.05)
+    off = np.r_[0, np.cumsum(sizes)].astype(np.int32)
+    ntot = int(off[-1])
+    lmax, nnbmax = 400, 350
+    m, x, v = np.concatenate(ms), np.concatenate(xs), np.concatenate(vs)
+    h2, dtr = np.concatenate(h2s), np.concatenate(dts)
+    L = lib.load()
+    dj = lib.DeviceArray.from_host(np.ascontiguousarray(np.concatenate([x.T, v.T, m[None]])))
+    di = lib.DeviceArray.from_host(np.ascontiguousarray(np.concatenate([x.T, v.T, h2[None], dtr[None]])))
+    do = lib.DeviceArray(7 * ntot * 8)
+    dl = lib.DeviceArray(ntot * lmax * 4)
+    try:
+        assert L.nb6_ens_open(nc, ntot) == 0
+        assert L.nb6_ens_send_device(nc, off.ctypes.data_as(lib._ip), dj.ptr, ntot, None) == 0
+        for _ in range(3):   # eager sizing + capture, then replays
+            assert L.nb6_ens_regf_device(di.ptr, ntot, lmax, nnbmax, 0, do.ptr, dl.ptr, 1, None) == 0
+        L.nb6_sync()
+        print(f"ensemble: {L.nb6_ens_last_refined()} rows through the FP64 pass", flush=True)
+        out = do.to_host(np.float64, (7, ntot))
+        lst = dl.to_host(np.int32, (ntot, lmax))
+        L.nb6_ens_close()
+        # FP64 kernel, one cluster at a time
+        L.nb6_set_regf_precision(2)
+        try:
+            for c in range(nc):
+                a, n = int(off[c]), int(sizes[c])
+                assert L.nb6_send_device(n, ntot, dj.ptr + 8 * a, None) == 0
+                assert L.nb6_regf_device(n, ntot, di.ptr + 8 * a, lmax, nnbmax, 0, do.ptr + 8 * a,
+                                         dl.ptr + 4 * lmax * a, None) == 0
+        finally:
+            L.nb6_set_regf_precision(0)
+        L.nb6_sync()
+        ref = do.to_host(np.float64, (7, ntot))
+        rlst = dl.to_host(np.int32, (ntot, lmax))
+    finally:
+        L.nb6_ens_close()
+        for d in (dj, di, do, dl):
+            d.free()
+    check_every_particle(f"ensemble {nc} clusters, {ntot} particles", out, lst, ref, rlst)

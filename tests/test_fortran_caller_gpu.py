@@ -156,7 +156,8 @@ def test_fortran_caller_through_the_reference_interface(tmp_path):
                              d["lmax"])
     ea, ej = vec_rel_err(r["ai"], iacc), vec_rel_err(r["ji"], ijrk)
     print(f"irr via Fortran: acc {ea.max():.2e} jrk {ej.max():.2e}", flush=True)
-    assert ea.max() <= 2e-6 and ej.max() <= 2e-6
+    # FP32 records and pair arithmetic of the irregular force (irr.avx.cpp's format)
+    assert ea.max() <= 2e-6 and ej.max() <= 1e-5
     assert np.array_equal(r["nnbid"], inn)
 
     # profile lines (gpunb.velocity.cu:883-903, irr.avx.cpp:407-422): counters of the

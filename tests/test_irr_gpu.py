@@ -9,7 +9,12 @@ from nb6 import lib
 
 pytestmark = pytest.mark.gpu
 
+# The irregular force keeps the reference's record format (irr.avx.cpp:57-233:
+# positions FP64, velocity, a/2 and adot/6 FP32) and FP32 pair arithmetic, so its
+# jerk carries the FP32 rounding of the velocities (up to about 6e-6 measured where
+# the neighbour terms cancel); acc stays within 2e-6.
 REL_TOL = 2.0e-6
+JRK_TOL = 1.0e-5
 
 
 def irr_state(n=1024, seed=5):
@@ -48,7 +53,7 @@ def test_firr_matches_fp64(alias):
     ej = np.linalg.norm(jrk - rjrk, axis=1) / np.linalg.norm(rjrk, axis=1)
     print(f"irr: acc max {ea.max():.3e} jrk max {ej.max():.3e} nnbid agree "
           f"{(nn == rnn).mean():.4f}", flush=True)
-    assert ea.max() <= REL_TOL and ej.max() <= REL_TOL
+    assert ea.max() <= REL_TOL and ej.max() <= JRK_TOL
     assert (nn == rnn).all()
 
 
@@ -148,6 +153,19 @@ def test_firr_fed_from_device_equals_host_path():
         g.close()
     assert np.array_equal(out[:, 0:3], ha) and np.array_equal(out[:, 3:6], hj)
     assert np.array_equal(out[:, 6].astype(int), hn)
+    # the device-fed result against the FP64 oracle on the same inputs: the device
+    # regf rows of the block (+IFIRST, self dropped), the predictor records
+    lists = np.zeros((n, 400), np.int32)
+    for k, i in enumerate(blk):
+        row = [j + 1 for j in lst[k, 1:1 + lst[k, 0]] if j != i]
+        lists[i, 0] = len(row)
+        lists[i, 1:1 + len(row)] = row
+    racc, rjrk, rnn = O.firr(ti, blk + 1, x, v, a2, j6, m, t0, lists, 400)
+    ea = np.linalg.norm(out[:, 0:3] - racc, axis=1) / np.linalg.norm(racc, axis=1)
+    ej = np.linalg.norm(out[:, 3:6] - rjrk, axis=1) / np.linalg.norm(rjrk, axis=1)
+    print(f"device-fed irr vs FP64: acc {ea.max():.2e} jrk {ej.max():.2e}", flush=True)
+    assert ea.max() <= REL_TOL and ej.max() <= JRK_TOL
+    assert np.array_equal(out[:, 6].astype(int), rnn)
 
 
 @pytest.mark.parametrize("lpi", [64, 16])

@@ -69,10 +69,12 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("pack", [1, 2], ids=["packed", "direct"])
 @pytest.mark.parametrize("kw", CASES, ids=lambda kw: "-".join(f"{k}{v}" for k, v in kw.items()))
-def test_regcor_matches_oracle_bitwise(kw):
+def test_regcor_matches_oracle_bitwise(kw, pack):
+    """Both ways of reading the neighbours' state (packed records, state arrays)."""
     S, P, addr, force, rows = regcor_case(**kw)
-    print(f"regcor case {kw} ni={len(addr)}", flush=True)
+    print(f"regcor case {kw} ni={len(addr)} pack={pack}", flush=True)
     So = copy_state(S)
     rows_o = rows.copy()
     C = R.regcor_block(So, P, addr, force, rows_o)
@@ -83,10 +85,12 @@ def test_regcor_matches_oracle_bitwise(kw):
         assert C["ncmder"] > 0
     d_force = lib.DeviceArray.from_host(np.ascontiguousarray(force))
     d_rows = lib.DeviceArray.from_host(rows.astype(np.int32))
+    lib.load().nb6_set_regcor_pack(pack)
     try:
         got, cnt = _gpu(S, P, addr, d_force, force.shape[1], d_rows, len(addr))
         rows_g = d_rows.to_host(np.int32, rows.shape)
     finally:
+        lib.load().nb6_set_regcor_pack(0)
         d_force.free()
         d_rows.free()
     _same(got, So, str(kw))

@@ -326,23 +326,36 @@ def test_regf_overflow_band_pairs_past_the_slots(gpunb, mode):
     assert vec_rel_err(got[1], ref[1]).max() <= JRK_MAX
 
 
-def test_regf_overflow_band_pairs_without_slots_give_nan(gpunb):
+@pytest.mark.parametrize("refine", [0, 1])
+def test_regf_overflow_band_pairs_without_slots(gpunb, refine):
     """When band pairs of an overflowing row find neither a hard nor a soft slot
-    (the near field has already taken the soft slots), the row cannot be summed
-    exactly: its forces come back NaN and its count stays negative, at least the
-    exact count in magnitude.  The other row is untouched."""
+    (the near field has already taken the soft slots), the FP32 pass cannot sum the
+    row exactly.  Without the FP64 pass its forces come back NaN; with it (the
+    default) the row is recomputed over all j in FP64 and equals the oracle.  Its
+    count stays negative, at least the exact count in magnitude.  The other row is
+    untouched."""
     m, x, v, xi, vi, h2, dtr = _crowded_row(32, 1000, 200, 400, 3000)
     lmax, nnbmax = 60, 50
     g = gpunb(len(m))
     g.send(m, x, v)
     L = lib.load()
     L.nb6_set_regf_config(256, 1, 1)
+    L.nb6_set_regf_refine(refine)
     try:
         got = g.regf(h2, dtr, xi, vi, lmax, nnbmax)
+        nfix = L.nb6_last_regf_refined()
     finally:
         L.nb6_set_regf_config(0, 1, 4096)
+        L.nb6_set_regf_refine(1)
     ref = O.regf(m, x, v, h2, dtr, xi, vi, lmax, nnbmax)
-    assert np.isnan(got[0][0]).all() and np.isnan(got[2][0])
+    if refine:
+        assert nfix >= 1
+        assert vec_rel_err(got[0][:1], ref[0][:1]).max() <= 1e-12
+        assert vec_rel_err(got[1][:1], ref[1][:1]).max() <= 1e-12
+        assert abs(got[2][0] - ref[2][0]) <= 1e-12 * abs(ref[2][0])
+    else:
+        assert nfix == 0
+        assert np.isnan(got[0][0]).all() and np.isnan(got[2][0])
     assert got[3][0, 0] <= ref[3][0, 0] < 0
     assert got[3][1, 0] == ref[3][1, 0]
     assert vec_rel_err(got[0][1:], ref[0][1:]).max() <= REL_TOL
