@@ -38,6 +38,8 @@ LMAX, NNBMAX = 600, 550          # configure.ac LMAX for --with-par >= 128k; NNB
 FP32_PEAK_TFLOPS = 157.3         # MI355X FP32 vector peak
 FLOPS_PER_PAIR = 60              # gpunb.velocity.cu:895 convention
 SEED = 20261202 + 2
+PLAIN_REFINE = (0.5, 0.5, 2.0, 0.0)   # default FP64-pass weights of a plain call
+ENS_REFINE = (0.5, 0.0, 1.0, 0.0)     # and of the batched ensemble step
 
 
 def log(*a):
@@ -169,10 +171,12 @@ def bench_ensemble(steps, warmup):
         L.nb6_ens_regf_device(di.ptr, ntot, lmax, nnbmax, 0, do.ptr, dl.ptr, 1, None)
     L.nb6_sync()
     t_ens_regf = (time.perf_counter() - t0) / steps
+    refined_b = L.nb6_ens_last_refined()
     out_b = do.to_host(np.float64, (7, ntot))
     lst_b = dl.to_host(np.int32, (ntot, lmax))
     L.nb6_ens_close()
-    L.nb6_set_regf_config(0, 1, 4096)      # the looped path runs the default configuration
+    L.nb6_set_regf_config(0, 1, 4096)      # the looped path runs the default configuration,
+    L.nb6_set_regf_refine_weights(*ENS_REFINE)   # with the batched step's FP64-pass criterion
 
     # looped: the plain path, one cluster after the other
     g = lib.Gpunb(int(sizes.max()))
@@ -193,8 +197,19 @@ def bench_ensemble(steps, warmup):
         loop_step()
     L.nb6_sync()
     t_loop = (time.perf_counter() - t0) / steps
+    L.nb6_set_regf_refine_weights(*PLAIN_REFINE)
     out_l = do.to_host(np.float64, (7, ntot))
     lst_l = dl.to_host(np.int32, (ntot, lmax))
+    # every particle of every cluster against the FP64 kernel (after the timed regions)
+    L.nb6_set_regf_precision(2)
+    loop_step()
+    L.nb6_sync()
+    L.nb6_set_regf_precision(0)
+    ref = do.to_host(np.float64, (7, ntot))
+    rl = dl.to_host(np.int32, (ntot, lmax))
+    vs64 = {"particles": ntot, "acc": err_stats(rel_cols(out_b[0:3], ref[0:3])),
+            "jerk": err_stats(rel_cols(out_b[3:6], ref[3:6])),
+            "pot": err_stats(rel_cols(out_b[6], ref[6])), "lists_identical": same_rows(lst_b, rl)}
     g.close()
     for d in (dj, di, do, dl):
         d.free()
@@ -219,7 +234,10 @@ def bench_ensemble(steps, warmup):
                        "pairs_per_s": pairs / t_loop},
             "speedup": t_loop / t_ens,
             "batched_vs_looped": {"acc_max_rel": float(da.max()), "pot_max_rel": float(dp.max()),
-                                  "lists_identical": same_lists}}
+                                  "lists_identical": same_lists},
+            "batched_vs_fp64_kernel": vs64,
+            "fp64_pass": {"batched_rows": refined_b,
+                          "weights_batched_and_looped": list(ENS_REFINE)}}
 
 
 def bench_learned_rs(train_samples, eval_steps, no_cpu):
@@ -989,8 +1007,9 @@ def main():
         rc = L.nb6_regf_device(ni, ni, di.ptr, LMAX, NNBMAX, 0, do.ptr, dl.ptr, None)
         if rc != 0:
             lib.check_error()
+        # FP64 pass: segment recomputation (chunkfix) plus whole-row fixes (fix)
         return (L.nb6_kernel_ms(b"regf_tile"), L.nb6_kernel_ms(b"regf_finalize"),
-                L.nb6_kernel_ms(b"regf_fix"))
+                max(L.nb6_kernel_ms(b"regf_chunkfix"), 0.0) + max(L.nb6_kernel_ms(b"regf_fix"), 0.0))
 
     for _ in range(args.warmup):
         step()
@@ -1006,6 +1025,7 @@ def main():
     wall = time.perf_counter() - t0
     chunks = L.nb6_last_regf_chunks()
     refined = L.nb6_last_regf_refined()
+    refined_items = L.nb6_last_regf_refined_chunks()
 
     out = do.to_host(np.float64, (7, ni))
     lst = dl.to_host(np.int32, (ni, LMAX))
@@ -1024,7 +1044,7 @@ def main():
             "kernel": "regf_tile_kernel", "avg_launch_ms": round(t_tile * 1e3, 3),
             "finalize_avg_ms": round(float(np.mean(fin_ms)), 3),
             "fp64_pass_avg_ms": round(float(np.mean(fix_ms)), 3), "fp64_pass_rows": refined,
-            "j_chunks": chunks}
+            "fp64_pass_segments": refined_items, "j_chunks": chunks}
     # CPU leg: the reference's AVX back end timed on a sample, and the FP64 oracle
     # as the checker of the GPU result on the first 256 i-particles
     cpu, acc = None, None

@@ -179,8 +179,13 @@ int nb6_last_regf_refined(void);
 int nb6_last_regf_refined_chunks(void);
 /* Weights of the four error scales that flag a row for the FP64 pass (flush sums
  * of acc and jerk, position and velocity rounding of the pair terms); larger
- * weights flag more rows.  Defaults 0.5, 0.5, 2, 0. */
+ * weights flag more rows.  Defaults 0.5, 0.5, 2, 0, and 0.5, 0.5, 2, 1 for j-sets
+ * of at most 4096 (the velocity term's linear bound over-flags large j-sets, and a
+ * small j-set's rows are cheap to recompute); a call here sets both to its values. */
 int nb6_set_regf_refine_weights(double k0, double k1, double k2, double k3);
+/* The same weights for the batched ensemble step.  Default 0.5, 0, 1, 0: rows are
+ * flagged by the acc bound (and the position part of the jerk bound) only. */
+int nb6_set_ens_refine_weights(double k0, double k1, double k2, double k3);
 
 /* Near field of the FP32 regular-force kernel (no counterpart in the reference):
  * pairs with r < f RS are left out of the FP32 sums and summed in FP64 by the

@@ -657,11 +657,21 @@ int nb6_set_regf_refine_weights(double k0, double k1, double k2, double k3) {
             set_error(E_ARG, "nb6_set_regf_refine_weights: weights outside [0, 1e6]");
             return E_ARG;
         }
-    for (int q = 0; q < 4; q++) regf_config().kref[q] = (float)k[q];
+    for (int q = 0; q < 4; q++) regf_config().kref[q] = regf_config().kref_small[q] = (float)k[q];
     return OK;
 }
 int nb6_set_regf_debug_bounds(double *d_bounds) {
     regf_set_debug_bounds(d_bounds);
+    return OK;
+}
+int nb6_set_ens_refine_weights(double k0, double k1, double k2, double k3) {
+    const double k[4] = {k0, k1, k2, k3};
+    for (int q = 0; q < 4; q++)
+        if (!(k[q] >= 0.0 && k[q] <= 1e6)) {
+            set_error(E_ARG, "nb6_set_ens_refine_weights: weights outside [0, 1e6]");
+            return E_ARG;
+        }
+    for (int q = 0; q < 4; q++) regf_config().kref_ens[q] = (float)k[q];
     return OK;
 }
 int nb6_set_regf_near(double f) {

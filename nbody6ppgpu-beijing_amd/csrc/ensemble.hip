@@ -135,9 +135,9 @@ __global__ void ens_zero_kernel(int *q) { *q = 0; }
 
 struct GraphKey {
     const void *ptr[4] = {nullptr, nullptr, nullptr, nullptr};   // d_i, d_out, d_list, items
-    long long val[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    long long val[18] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     bool operator==(const GraphKey &o) const {
-        return std::equal(ptr, ptr + 4, o.ptr) && std::equal(val, val + 14, o.val);
+        return std::equal(ptr, ptr + 4, o.ptr) && std::equal(val, val + 18, o.val);
     }
 };
 
@@ -350,11 +350,13 @@ int ens_regf_device(const double *d_i, int stride, int lmax, int nnbmax, int m_f
     key.ptr[0] = d_i; key.ptr[1] = d_out; key.ptr[2] = d_list; key.ptr[3] = e.d_items.p;
     long long mbits = 0;
     if (m_flag) std::memcpy(&mbits, &e.js.mmax, sizeof(mbits));
-    int nbits = 0;
+    int nbits = 0, kb[4] = {0, 0, 0, 0};
     std::memcpy(&nbits, &cfg.near, sizeof(nbits));
-    const long long v[14] = {stride, lmax, nnbmax, m_flag, cfg.threads, cfg.ipl, e.nitems, grid,
-                             e.ntot, e.npad, mbits, cfg.far_test, cfg.refine, nbits};
-    std::copy(v, v + 14, key.val);
+    std::memcpy(kb, cfg.kref_ens, sizeof(kb));
+    const long long v[18] = {stride, lmax, nnbmax, m_flag, cfg.threads, cfg.ipl, e.nitems, grid,
+                             e.ntot, e.npad, mbits, cfg.far_test, cfg.refine, nbits,
+                             kb[0], kb[1], kb[2], kb[3]};
+    std::copy(v, v + 18, key.val);
     const bool same = e.exec && key == e.gkey;
     if (!same) {
         drop_graph();

@@ -123,6 +123,12 @@ struct RegfConfig {
     int refine = 1;       // FP64 pass over rows whose FP32 error scale is too large
     float near = 2.0f;    // near field (FP64 in the finalize): r < near RS
     float kref[4] = {0.5f, 0.5f, 2.0f, 0.0f};   // weights of the refine error scales
+    // weights for j-sets of at most small_nj: the velocity-rounding scale (a linear
+    // bound, loose by about the square root of the terms on large j-sets) counts too,
+    // where recomputing a row costs at most small_nj FP64 pairs
+    float kref_small[4] = {0.5f, 0.5f, 2.0f, 1.0f};
+    int small_nj = 4096;
+    float kref_ens[4] = {0.5f, 0.0f, 1.0f, 0.0f};   // the same for the batched form
 };
 RegfConfig &regf_config();
 // the configuration a call with ni i-particles runs with (threads resolved)

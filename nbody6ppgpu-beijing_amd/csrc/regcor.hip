@@ -390,6 +390,7 @@ enum { HI_FLAGS = 0, HI_NNB = 1, HI_NNB0 = 2, HI_J3 = 3, NHI = 6 };
 // every lane computes from the same inputs are uniform, list work is spread over the
 // lanes, lane 0 alone runs the rare SMIN retention.  hd, hi: this particle's column of
 // the hand-over (stride ni).
+template <bool CHECKL>
 __device__ void regcor_wave(const nb6_regcor_params &P, const nb6_regstate &s, const double *pk,
                             int I, const double *fg, int fstride, int *row, int *cnt, double *T,
                             int *nl, int *ol, unsigned *lb, unsigned *cb, int lane, double *hd,
@@ -462,7 +463,7 @@ __device__ void regcor_wave(const nb6_regcor_params &P, const nb6_regstate &s, c
     // CHECKL, regcor_gpu.F:240-244 (after the RS update, which the finishing kernel does
     // with the NNB and list from before this point)
     if (nnb > 0 && P.kz37 > 1) cnt[C_UNS]++;
-    if (nnb > 0 && P.kz37 == 1 && s.listv && s.listv[0] > 0)
+    if (CHECKL && nnb > 0 && P.kz37 == 1 && s.listv && s.listv[0] > 0)
         nnb = checkl_fast(P, s, I, i, xi, vi, rs2, freg, fdr, nl, nnb, cnt, lane);
 
     // losses (ascending), gains and JMIN, regcor_gpu.F:252-296
@@ -805,6 +806,9 @@ __device__ void regcor_finish(const nb6_regcor_params &P, const nb6_regstate &s,
     }
 }
 
+// CHECKL = false (KZ(37) != 1, or no LISTV): the kernel without the high-velocity test,
+// whose registers would halve its occupancy
+template <bool CHECKL>
 __global__ __launch_bounds__(256) void regcor_lists_kernel(nb6_regcor_params P, nb6_regstate s,
                                                            const double *__restrict__ pk,
                                                            int ni, const int *__restrict__ addr,
@@ -828,7 +832,7 @@ __global__ __launch_bounds__(256) void regcor_lists_kernel(nb6_regcor_params P, 
         unsigned *lb = reinterpret_cast<unsigned *>(ol + (s.lmax + 1));
         unsigned *cb = lb + loss_words(s.lmax);
         if (lane == 0) hi[HI_FLAGS * ni + k] = 0;
-        regcor_wave(P, s, pk, addr[k], force + k, fstride, nlist + (size_t)k * s.lmax, cnt, T, nl,
+        regcor_wave<CHECKL>(P, s, pk, addr[k], force + k, fstride, nlist + (size_t)k * s.lmax, cnt, T, nl,
                     ol, lb, cb, lane, hd + k, hi + k, ni);
     }
     if (lane == 0) {
@@ -898,7 +902,9 @@ int regcor_device(const nb6_regcor_params *p, const nb6_regstate *s, int ni, con
     const size_t lds = wl * wpb;
     static size_t lds_set = 0;
     if (lds > lds_set) {
-        NB6_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(regcor_lists_kernel),
+        NB6_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(regcor_lists_kernel<false>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        NB6_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(regcor_lists_kernel<true>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         lds_set = lds;
     }
@@ -915,9 +921,14 @@ int regcor_device(const nb6_regcor_params *p, const nb6_regstate *s, int ni, con
                            *s, pk);
         NB6_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(regcor_lists_kernel, dim3((ni + wpb - 1) / wpb), dim3(WAVE * wpb), lds,
-                       stream, *p, *s, pk, ni, d_addr, d_force, force_stride, d_nlist, d_counters,
-                       hd, hi);
+    if (p->kz37 == 1 && s->listv)
+        hipLaunchKernelGGL(regcor_lists_kernel<true>, dim3((ni + wpb - 1) / wpb), dim3(WAVE * wpb),
+                           lds, stream, *p, *s, pk, ni, d_addr, d_force, force_stride, d_nlist,
+                           d_counters, hd, hi);
+    else
+        hipLaunchKernelGGL(regcor_lists_kernel<false>, dim3((ni + wpb - 1) / wpb), dim3(WAVE * wpb),
+                           lds, stream, *p, *s, pk, ni, d_addr, d_force, force_stride, d_nlist,
+                           d_counters, hd, hi);
     NB6_HIP(hipGetLastError());
     hipLaunchKernelGGL(regcor_finish_kernel, dim3((ni + 255) / 256), dim3(256), 0, stream, *p, *s,
                        ni, d_addr, hd, hi);

@@ -90,6 +90,8 @@ struct TileParams {
     int *__restrict__ sslot;         // [(chunk*ni + i) * caps + k]
     int caps;
     float fnear;                     // near field: r < fnear RS (soft candidates)
+    int nseg_c;                      // partial-sum segments per chunk (part rows chunk * nseg_c
+    int seg_tiles;                   // + s), each seg_tiles T-tiles long (0: the whole chunk)
 };
 
 // per-lane constants that do not depend on the tile
@@ -318,6 +320,28 @@ __device__ __forceinline__ void pair_step(const TileParams &p, const IC &ck, con
     }
 }
 
+// FP64 partial sums and error scales of this lane's i-particles to part row `row`
+// (of the chunk's partial-sum segments), then zeroed for the next segment
+template <int T, int IPL>
+__device__ __forceinline__ void write_part(const TileParams &p, int ibase, int iend, int row,
+                                           double (*a64)[7], float (*err)[NERR]) {
+#pragma unroll
+    for (int k = 0; k < IPL; k++) {
+        const int i = ibase + k * T + (int)threadIdx.x;
+        if (i < iend) {
+            double *pp = p.part + (size_t)row * NPART * p.ni + i;
+#pragma unroll
+            for (int f = 0; f < 7; f++) pp[(size_t)f * p.ni] = a64[k][f];
+#pragma unroll
+            for (int f = 0; f < NERR; f++) pp[(size_t)(7 + f) * p.ni] = (double)err[k][f];
+        }
+#pragma unroll
+        for (int f = 0; f < 7; f++) a64[k][f] = 0.0;
+#pragma unroll
+        for (int f = 0; f < NERR; f++) err[k][f] = 0.f;
+    }
+}
+
 // One work item: i in [ibase, min(ibase + T*IPL, iend)) against sorted j in
 // [j0, j1) (a whole number of tiles), partial-sum row = chunk * ni + i.
 template <int T, int IPL, bool MFLAG>
@@ -356,6 +380,7 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
     }
 
     const int ntile = (j1 - j0 + T - 1) / T;
+    int segw = 0;                    // partial-sum segment being summed
 
     // Tile t goes straight from HBM into LDS buffer t & 1 (global_load_lds_dwordx4:
     // lane l of a wavefront writes 16 bytes at the wavefront's LDS base + 16 l), so the
@@ -435,17 +460,21 @@ __device__ __forceinline__ void tile_work(const TileParams &p, int ibase, int ie
 
         lds_dma_wait();   // this wavefront's part of tile t + 1 is in LDS
         __syncthreads();
+        if (p.seg_tiles > 0 && (t + 1) % p.seg_tiles == 0 && t + 1 < ntile) {
+            // end of a partial-sum segment: its sums to their own row, then afresh
+            write_part<T, IPL>(p, ibase, iend, chunk * p.nseg_c + segw, a64, err);
+            segw++;
+        }
     }
+    // the last segment, and empty segments past the end of a short chunk
+    write_part<T, IPL>(p, ibase, iend, chunk * p.nseg_c + segw, a64, err);
+    for (segw++; segw < p.nseg_c; segw++)
+        write_part<T, IPL>(p, ibase, iend, chunk * p.nseg_c + segw, a64, err);
 
 #pragma unroll
     for (int k = 0; k < IPL; k++) {
         const int i = ibase + k * T + tid;
         if (i < iend) {
-            double *pp = p.part + (size_t)chunk * NPART * p.ni + i;
-#pragma unroll
-            for (int f = 0; f < 7; f++) pp[(size_t)f * p.ni] = a64[k][f];
-#pragma unroll
-            for (int f = 0; f < NERR; f++) pp[(size_t)(7 + f) * p.ni] = (double)err[k][f];
             p.cnt[((size_t)chunk * 2 + 0) * p.ni + i] = nh[k];
             p.cnt[((size_t)chunk * 2 + 1) * p.ni + i] = ns[k];
         }
@@ -686,6 +715,7 @@ struct FinParams {
     const int *__restrict__ jbase;      // per i: index of its cluster's first j (or null)
     int nj;
     int nchunk;
+    int nprow;                       // partial-sum rows per i: nchunk * segments per chunk
     const double *__restrict__ part;
     const int *__restrict__ cnt;
     const int *__restrict__ hslot;
@@ -821,6 +851,58 @@ __device__ __forceinline__ bool pair64(const double4 &r0, const double4 &r1, con
     return isnb;
 }
 
+// pair64's force terms (test on) without a branch, for the FP64 pass's long j loops:
+// the same operations on the pairs that count and exact zeros added for the others,
+// so that unrolled iterations interleave (one pair's chain of dependent FP64
+// operations alone leaves the VALU idle)
+template <bool MFLAG>
+__device__ __forceinline__ void pair64_sum(const double4 &r0, const double4 &r1, const IF64 &I,
+                                           double *s) {
+    double dx, dy, dz, du, dv, dw, r2, rmin;
+    {
+#pragma clang fp contract(off)
+        dx = r0.x - I.x; dy = r0.y - I.y; dz = r0.z - I.z;
+        du = r1.x - I.u; dv = r1.y - I.v; dw = r1.z - I.w;
+        r2 = dx * dx + dy * dy + dz * dz;
+        const double ex = dx + I.dtr * du;
+        const double ey = dy + I.dtr * dv;
+        const double ez = dz + I.dtr * dw;
+        const double r2p = ex * ex + ey * ey + ez * ez;
+        rmin = r2 < r2p ? r2 : r2p;
+    }
+    const double thr = MFLAG ? r0.w * I.h2 : I.h2;
+    const bool use = !(rmin < thr) && r2 > 0.0;
+    const double r2s = use ? r2 : 1.0;
+    const double hr = 0.5 * r2s;
+    double rinv = __builtin_amdgcn_rsq(r2s);
+    rinv = rinv * fma(-hr * rinv, rinv, 1.5);
+    rinv = rinv * fma(-hr * rinv, rinv, 1.5);
+    const double rinv2 = rinv * rinv;
+    const double mr1 = use ? r0.w * rinv : 0.0;
+    const double mr3 = mr1 * rinv2;
+    const double rv = dx * du + dy * dv + dz * dw;
+    const double al = -3.0 * rv * rinv2;
+    s[0] += mr3 * dx;
+    s[1] += mr3 * dy;
+    s[2] += mr3 * dz;
+    s[3] += mr3 * (du + al * dx);
+    s[4] += mr3 * (dv + al * dy);
+    s[5] += mr3 * (dw + al * dz);
+    s[6] += mr1;
+}
+
+// pair64_sum over jn records of an LDS tile, four at a time, in j order
+template <bool MFLAG>
+__device__ __forceinline__ void tile_sum64(const double4 *sA, const double4 *sB, int jn,
+                                           const IF64 &I, double *s) {
+    int jj = 0;
+    for (; jj + 4 <= jn; jj += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) pair64_sum<MFLAG>(sA[jj + u], sB[jj + u], I, s);
+    }
+    for (; jj < jn; jj++) pair64_sum<MFLAG>(sA[jj], sB[jj], I, s);
+}
+
 template <bool MFLAG>
 __device__ __forceinline__ bool classify(const double *__restrict__ rec, const IF64 &I, bool test,
                                          double *s) {
@@ -929,7 +1011,7 @@ __device__ void emit_chunks(const FinParams &f, int i, int r, const double *s, i
     const RowRef q = row_ref(s);
     auto bound = [&](int c, double &ea, double &ej) {
         ea = ej = 0.0;
-        if (c >= f.nchunk) return;
+        if (c >= f.nprow) return;
         const double *pp = f.part + (size_t)c * NPART * f.ni + i;
         double e[NERR];
 #pragma unroll
@@ -937,31 +1019,31 @@ __device__ void emit_chunks(const FinParams &f, int i, int r, const double *s, i
         err_bound(e, f.kf, q, ea, ej);
     };
     bool mine = false;                    // (greedy form) this lane's chunk is selected
-    if (f.nchunk <= 64) {
+    if (f.nprow <= 64) {
         double ea, ej;
         bound(lane, ea, ej);
         const double w = fmax(ea / (q.ta + 1e-300), ej / (q.tj + 1e-300));
-        for (int it = 0; it < f.nchunk; it++) {
+        for (int it = 0; it < f.nprow; it++) {
             const double ra = wave_sum(mine ? 0.0 : ea), rj = wave_sum(mine ? 0.0 : ej);
             if (ra <= q.ta && rj <= q.tj) break;
             // the unselected chunk with the largest share (lowest lane on ties)
-            double best = mine || lane >= f.nchunk ? -1.0 : w;
+            double best = mine || lane >= f.nprow ? -1.0 : w;
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) best = fmax(best, __shfl_xor(best, o, 64));
-            const unsigned long long m = __ballot(!mine && lane < f.nchunk && w == best);
+            const unsigned long long m = __ballot(!mine && lane < f.nprow && w == best);
             if (!m) break;
             mine = mine || lane == __builtin_ctzll(m);
         }
     }
     auto sel = [&](int c) {
-        if (c >= f.nchunk) return false;
-        if (f.nchunk <= 64) return mine;
+        if (c >= f.nprow) return false;
+        if (f.nprow <= 64) return mine;
         double ea, ej;
         bound(c, ea, ej);
-        return !(ea <= q.ta / f.nchunk) || !(ej <= q.tj / f.nchunk);
+        return !(ea <= q.ta / f.nprow) || !(ej <= q.tj / f.nprow);
     };
     int cnt = 0;
-    for (int cb = 0; cb < f.nchunk; cb += 64) cnt += __popcll(__ballot(sel(cb + lane)));
+    for (int cb = 0; cb < f.nprow; cb += 64) cnt += __popcll(__ballot(sel(cb + lane)));
     if (cnt == 0) return;    // within the bound with every chunk in FP32
     int start = 0;
     if (lane == 0) start = atomicAdd(f.nitems, cnt);
@@ -971,7 +1053,7 @@ __device__ void emit_chunks(const FinParams &f, int i, int r, const double *s, i
         return;
     }
     int k = start;
-    for (int cb = 0; cb < f.nchunk; cb += 64) {
+    for (int cb = 0; cb < f.nprow; cb += 64) {
         const int c = cb + lane;
         const bool b = sel(c);
         const unsigned long long m = __ballot(b);
@@ -1001,7 +1083,7 @@ __global__ __launch_bounds__(256) void regf_finalize_kernel(FinParams f) {
     double s[7], e[NERR] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < 7; q++) s[q] = 0.0;
-    for (int c = lane; c < f.nchunk; c += 64) {
+    for (int c = lane; c < f.nprow; c += 64) {
         const double *pp = f.part + (size_t)c * NPART * f.ni + i;
 #pragma unroll
         for (int q = 0; q < 7; q++) s[q] += pp[(size_t)q * f.ni];
@@ -1109,7 +1191,7 @@ __global__ __launch_bounds__(64 * W) void regf_finalize_wide_kernel(FinParams f)
     double s[NPART];
 #pragma unroll
     for (int q = 0; q < NPART; q++) s[q] = 0.0;
-    for (int c = w + W * lane; c < f.nchunk; c += 64 * W) {
+    for (int c = w + W * lane; c < f.nprow; c += 64 * W) {
         const double *pp = f.part + (size_t)c * NPART * f.ni + i;
 #pragma unroll
         for (int q = 0; q < 7; q++) s[q] += pp[(size_t)q * f.ni];
@@ -1252,8 +1334,7 @@ __global__ __launch_bounds__(256) void regf_fix_kernel(FixParams f) {
             }
             __syncthreads();
             const int jn = min(JTILE, jb - jt);
-            if (ok)
-                for (int jj = 0; jj < jn; jj++) (void)pair64<MFLAG>(sA[jj], sB[jj], I, true, sum);
+            if (ok) tile_sum64<MFLAG>(sA, sB, jn, I, sum);
         }
         if (ok) {
 #pragma unroll
@@ -1290,14 +1371,16 @@ __global__ __launch_bounds__(256) void regf_fix_reduce_kernel(FixParams f) {
 //    sum_s part64 - cand - FP32 partial of every one of its items.
 // Every value depends only on (i, c, s), so results do not depend on the order in which
 // items were emitted.
-constexpr int FIX_SUB = 4096;          // j per sub-range
+constexpr int FIX_SUB = 1024;          // j per sub-range
+constexpr int SEG_MIN = 8192;          // shortest partial-sum segment
 
 struct ChunkFixParams {
     const double *__restrict__ ib;   // SoA [8][ib_stride] (as the finalize of the batch)
     int ib_stride;
     const int *__restrict__ imap;    // slot -> caller row (nullptr: identity)
     const double *__restrict__ rec64;
-    int nj, clen, nchunk, ni, nsub;
+    int nj, clen, nchunk, ni, nsub;  // clen, nchunk: the partial-sum segments' length, count
+    int nseg_c;                      // segments per work-array chunk
     const double *__restrict__ part;
     const int *__restrict__ cnt;
     const int *__restrict__ hslot;
@@ -1342,21 +1425,24 @@ __global__ __launch_bounds__(256) void regf_cand_kernel(ChunkFixParams f) {
     for (int e = blockIdx.x * 4 + (threadIdx.x >> 6); e < nr; e += gridDim.x * 4)
     for (int k = f.rowsel[e].y; k < f.rowsel[e].y + f.rowsel[e].z; k++) {
         const int2 it = f.items[k];
-        const int i = it.x, c = f.padoff ? 0 : it.y;   // work-array chunk
+        const int i = it.x, c = f.padoff ? 0 : it.y / f.nseg_c;   // work-array chunk
+        // the segment's j range (the batched form: the whole cluster, any j)
+        const unsigned jlo = f.padoff ? 0u : (unsigned)(it.y * f.clen);
+        const unsigned jhi = f.padoff ? (unsigned)f.nj : (unsigned)min(f.nj, (it.y + 1) * f.clen);
         const IF64 I = load_if64(f.ib, f.ib_stride, f.imap ? f.imap[i] : i);
         double cd[7] = {0, 0, 0, 0, 0, 0, 0};
         const size_t rowc = (size_t)c * f.ni + i;
         const int nh = min(f.cnt[((size_t)c * 2 + 0) * f.ni + i], f.caph);
         const int nsr = min(f.cnt[((size_t)c * 2 + 1) * f.ni + i], f.caps);
         for (int q = lane; q < nh; q += 64) {
-            const int sj = f.hslot[rowc * f.caph + q];
-            if ((unsigned)sj < (unsigned)f.nj) (void)classify<MFLAG>(f.rec64 + (size_t)sj * 8, I, true, cd);
+            const unsigned sj = (unsigned)f.hslot[rowc * f.caph + q];
+            if (sj >= jlo && sj < jhi) (void)classify<MFLAG>(f.rec64 + (size_t)sj * 8, I, true, cd);
         }
         for (int q = lane; q < nsr; q += 64) {
             const int e = f.sslot[rowc * f.caps + q];
             const bool band = e < 0;
-            const int sj = band ? ~e : e;
-            if ((unsigned)sj < (unsigned)f.nj) (void)classify<MFLAG>(f.rec64 + (size_t)sj * 8, I, band, cd);
+            const unsigned sj = (unsigned)(band ? ~e : e);
+            if (sj >= jlo && sj < jhi) (void)classify<MFLAG>(f.rec64 + (size_t)sj * 8, I, band, cd);
         }
 #pragma unroll
         for (int q = 0; q < 7; q++) cd[q] = wave_sum(cd[q]);
@@ -1370,10 +1456,13 @@ __global__ __launch_bounds__(256) void regf_segfix_kernel(ChunkFixParams f) {
     const int t = threadIdx.x;
     const int gmax = ((f.padoff ? f.maxseg : f.ni) + 255) / 256;
     const long long units = (long long)f.nchunk * gmax * f.nsub;
+    // units ordered (item group, chunk, sub-range): the few busy groups of every chunk
+    // come first and spread over the whole grid
+    const int per_g = f.nchunk * f.nsub;
     for (long long u = blockIdx.x; u < units; u += gridDim.x) {   // uniform per work-group
-        const int c = (int)(u / ((long long)gmax * f.nsub));
-        const int rem = (int)(u - (long long)c * gmax * f.nsub);
-        const int g = rem / f.nsub, sub = rem - g * f.nsub;
+        const int g = (int)(u / per_g);
+        const int rem = (int)(u - (long long)g * per_g);
+        const int c = rem / f.nsub, sub = rem - c * f.nsub;
         const int nc = f.seg_cnt[c];
         if (g * 256 >= nc) continue;
         // (the last chunk or a short cluster: trailing sub-ranges are empty and write 0)
@@ -1399,8 +1488,7 @@ __global__ __launch_bounds__(256) void regf_segfix_kernel(ChunkFixParams f) {
             }
             __syncthreads();
             const int jn = min(JTILE, j1 - jt);
-            if (ok)
-                for (int jj = 0; jj < jn; jj++) (void)pair64<MFLAG>(sA[jj], sB[jj], I, true, sum);
+            if (ok) tile_sum64<MFLAG>(sA, sB, jn, I, sum);
         }
         if (ok) {
             double *o = f.part64 + ((size_t)k * f.nsub + sub) * 7;
@@ -1677,13 +1765,26 @@ int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, i
     // Work arrays hold one row per (i, chunk): FP64 partials, counts and the hard and
     // soft candidate slots.  Calls whose rows would exceed the slot budget run in
     // batches of i (whole i-blocks); chunks and results do not change.
-    // FP64 pass: per (i, chunk) a bucket entry
+    // FP32 partial sums per segment of a chunk: with long chunks, P segments of at
+    // least SEG_MIN j (whole tiles), so that the FP64 pass recomputes a segment, not the
+    // chunk.  FP64 pass: per (i, segment) a bucket entry
     const bool refine = cfg.refine != 0 && cfg.prec == 0;
-    const size_t item_bytes = refine ? sizeof(int) : 0;
-    const size_t row_bytes = NPART * sizeof(double) + 2 * sizeof(int) +
-                             ((size_t)caph + (size_t)caps) * sizeof(int) + item_bytes;
+    const int T = cfg.threads;
+    auto row_size = [&](int nsegc) {
+        return (size_t)nsegc * NPART * sizeof(double) + 2 * sizeof(int) +
+               ((size_t)caph + (size_t)caps) * sizeof(int) + (refine ? nsegc * sizeof(int) : 0);
+    };
     const size_t budget = na ? na->rows_bytes - ARENA_SLACK
                              : (size_t)std::max(cfg.slot_budget_mb, 1) << 20;
+    int P = 1;
+    // (at most 64 per i, the width of the finalize's greedy selection; with caller
+    // work memory, one i-block of rows must still fit)
+    while (refine && P < 8 && nchunk * 2 * P <= 64 && clen / (2 * P) >= SEG_MIN &&
+           (clen / (2 * P)) % T == 0 && clen % (2 * P) == 0 &&
+           (!na || (size_t)nchunk * nbi * row_size(2 * P) <= budget))
+        P *= 2;
+    const int seglen = clen / P;
+    const size_t row_bytes = row_size(P);
     int nbatch_i = ni;
     if ((size_t)nchunk * (size_t)ni * row_bytes > budget) {
         const size_t rows_max = budget / row_bytes;
@@ -1696,9 +1797,10 @@ int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, i
     const size_t fix_cap = std::max((size_t)fix_items * 256, (size_t)ni);
     // items of the chunk-selective pass: (slot, chunk), candidate terms, FP64 sums per
     // sub-range; at most 4 per row of a batch (more go to the whole-row pass)
-    const int nsub = (clen + FIX_SUB - 1) / FIX_SUB;
+    const int nsub = (seglen + FIX_SUB - 1) / FIX_SUB;
     const size_t per_item = sizeof(int2) + 7 * sizeof(double) * (1 + (size_t)nsub);
-    size_t item_cap = std::min(rows, std::max<size_t>(4 * (size_t)nbatch_i, 65536));
+    const int nseg = nchunk * P;
+    size_t item_cap = std::min(rows * P, std::max<size_t>(4 * (size_t)nbatch_i, 65536));
     double *part, *fixpart = nullptr, *cand = nullptr, *part64 = nullptr;
     int *cnt, *hslot, *sslot, *fix_rows = nullptr, *nfix = nullptr, *ctr = nullptr;
     int *seg_items = nullptr;
@@ -1706,15 +1808,15 @@ int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, i
     int4 *rowsel = nullptr;
     if (na) {
         Arena rows_ar{na->arena->p, std::min(na->rows_bytes, na->arena->left)};
-        part = static_cast<double *>(rows_ar.take(rows * NPART * sizeof(double)));
+        part = static_cast<double *>(rows_ar.take(rows * P * NPART * sizeof(double)));
         cnt = static_cast<int *>(rows_ar.take(rows * 2 * sizeof(int)));
         hslot = static_cast<int *>(rows_ar.take(rows * caph * sizeof(int)));
         sslot = static_cast<int *>(rows_ar.take(rows * caps * sizeof(int)));
-        if (refine) seg_items = static_cast<int *>(rows_ar.take(rows * sizeof(int)));
+        if (refine) seg_items = static_cast<int *>(rows_ar.take(rows * P * sizeof(int)));
         na->arena->take(na->rows_bytes);
         if (refine) {
             fix_rows = static_cast<int *>(na->arena->take((size_t)ni * sizeof(int)));
-            ctr = static_cast<int *>(na->arena->take((6 + (size_t)nchunk) * sizeof(int)));
+            ctr = static_cast<int *>(na->arena->take((6 + (size_t)nseg) * sizeof(int)));
             fixpart = static_cast<double *>(na->arena->take(fix_cap * 7 * sizeof(double)));
             rowsel = static_cast<int4 *>(na->arena->take((size_t)nbatch_i * sizeof(int4)));
             // the items take what is left
@@ -1731,15 +1833,15 @@ int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, i
             return E_ARG;
         }
     } else {
-        part = static_cast<double *>(g_work.part.get(rows * NPART * sizeof(double)));
+        part = static_cast<double *>(g_work.part.get(rows * P * NPART * sizeof(double)));
         cnt = static_cast<int *>(g_work.cnt.get(rows * 2 * sizeof(int)));
         hslot = static_cast<int *>(g_work.hslot.get(rows * caph * sizeof(int)));
         sslot = static_cast<int *>(g_work.sslot.get(rows * caps * sizeof(int)));
         if (refine) {
             fix_rows = static_cast<int *>(g_work.fixrows.get((size_t)ni * sizeof(int)));
-            ctr = static_cast<int *>(g_work.nfix.get((6 + (size_t)nchunk) * sizeof(int)));
+            ctr = static_cast<int *>(g_work.nfix.get((6 + (size_t)nseg) * sizeof(int)));
             fixpart = static_cast<double *>(g_work.fixpart.get(fix_cap * 7 * sizeof(double)));
-            seg_items = static_cast<int *>(g_work.segitems.get(rows * sizeof(int)));
+            seg_items = static_cast<int *>(g_work.segitems.get(rows * P * sizeof(int)));
             items = static_cast<int2 *>(g_work.items.get(item_cap * sizeof(int2)));
             cand = static_cast<double *>(g_work.delta.get(item_cap * 7 * sizeof(double)));
             part64 = static_cast<double *>(g_work.part64.get(item_cap * nsub * 7 * sizeof(double)));
@@ -1755,8 +1857,8 @@ int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, i
     if (refine) {
         nfix = ctr;
         g_nfix = ctr;
-        hipLaunchKernelGGL(zero_ints_kernel, dim3((6 + nchunk + 255) / 256), dim3(256), 0, s, ctr,
-                           6 + nchunk);
+        hipLaunchKernelGGL(zero_ints_kernel, dim3((6 + nseg + 255) / 256), dim3(256), 0, s, ctr,
+                           6 + nseg);
     }
     g_last_refine = refine;
 
@@ -1771,6 +1873,10 @@ int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, i
     KTimer &t2 = na ? dummy : ktimer("regf_finalize");
     KTimer &t4 = na ? dummy : ktimer("regf_chunkfix");
     const bool timed = !na;
+    if (timed) {   // the FP64-pass timers report this call only (or -1: no such pass)
+        t4.armed = false;
+        ktimer("regf_fix").armed = false;
+    }
     for (int b0 = 0; b0 < ni; b0 += nbatch_i) {
         const int nb = std::min(nbatch_i, ni - b0);
         // slots [b0, b0 + nb): through imap the rows are absolute; without it the
@@ -1793,6 +1899,8 @@ int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, i
         tp.part = part; tp.cnt = cnt;
         tp.hslot = hslot; tp.caph = caph; tp.sslot = sslot; tp.caps = caps;
         tp.fnear = cfg.near;
+        tp.nseg_c = P;
+        tp.seg_tiles = P > 1 ? seglen / T : 0;
         dim3 grid(bnib * nchunk);
         if (timed && b0 == 0) t1.start(s);
         launch_tile(cfg, m_flag != 0, grid, tp, s);
@@ -1804,6 +1912,7 @@ int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, i
         fp.rec64 = js.rec64.as<double>(); fp.perm = js.perm.as<int>(); fp.jbase = nullptr;
         fp.nj = nj;
         fp.nchunk = nchunk;
+        fp.nprow = nseg;
         fp.part = part; fp.cnt = cnt;
         fp.hslot = hslot; fp.caph = caph; fp.sslot = sslot; fp.caps = caps;
         fp.lmax = lmax; fp.nnbmax = nnbeff;
@@ -1811,7 +1920,7 @@ int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, i
         fp.refine = refine ? 1 : 0;
         fp.row_base = imap ? 0 : b0;
         fp.fix_slot = 0;
-        for (int q = 0; q < NERR; q++) fp.kf[q] = cfg.kref[q];
+        for (int q = 0; q < NERR; q++) fp.kf[q] = nj <= cfg.small_nj ? cfg.kref_small[q] : cfg.kref[q];
         fp.fix_rows = fix_rows;
         fp.nfix = nfix;
         fp.items = items; fp.nitems = nitems;
@@ -1821,7 +1930,7 @@ int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, i
         fp.dbg = g_dbg ? (imap ? g_dbg : g_dbg + b0) : nullptr;
         fp.rowsel = rowsel; fp.nrows = nrows;
         if (refine && b0 > 0)   // item counters per batch (the rows' total is kept in [3])
-            hipLaunchKernelGGL(next_batch_kernel, dim3(1), dim3(256), 0, s, ctr, nchunk);
+            hipLaunchKernelGGL(next_batch_kernel, dim3(1), dim3(256), 0, s, ctr, nseg);
         if (timed && b0 == 0) t2.start(s);
         if (fin_form == 8) {
             // few i: eight wavefronts per i
@@ -1849,8 +1958,8 @@ int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, i
             // arrays (partials, slots) are still there
             ChunkFixParams cp;
             cp.ib = bi; cp.ib_stride = ni_stride; cp.imap = bmap;
-            cp.rec64 = js.rec64.as<double>(); cp.nj = nj; cp.clen = clen; cp.nchunk = nchunk;
-            cp.ni = nb; cp.nsub = nsub;
+            cp.rec64 = js.rec64.as<double>(); cp.nj = nj; cp.clen = seglen; cp.nchunk = nseg;
+            cp.ni = nb; cp.nsub = nsub; cp.nseg_c = P;
             cp.part = part; cp.cnt = cnt; cp.hslot = hslot; cp.caph = caph;
             cp.sslot = sslot; cp.caps = caps;
             cp.items = items; cp.nitems = nitems; cp.item_cap = fp.item_cap;
@@ -1960,6 +2069,8 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
     tp.part = part; tp.cnt = cnt;
     tp.hslot = hslot; tp.caph = caph; tp.sslot = sslot; tp.caps = caps;
     tp.fnear = cfg.near;
+    tp.nseg_c = 1;
+    tp.seg_tiles = 0;
     KTimer &t1 = ktimer("regf_items");
     if (!capturing) t1.start(s);
     launch_items(cfg, m_flag != 0, grid, tp, items, nitems, queue, s);
@@ -1969,6 +2080,7 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
     fp.ib = d_i; fp.ni = nslots; fp.ib_stride = stride; fp.imap = imap;
     fp.rec64 = js.rec64.as<double>(); fp.perm = js.perm.as<int>(); fp.jbase = jbase; fp.nj = js.n;
     fp.nchunk = 1;
+    fp.nprow = 1;
     fp.part = part; fp.cnt = cnt;
     fp.hslot = hslot; fp.caph = caph; fp.sslot = sslot; fp.caps = caps;
     fp.lmax = lmax; fp.nnbmax = nnbeff;
@@ -1980,7 +2092,7 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
     fp.dbg = g_dbg;
     fp.rowsel = rowsel; fp.nrows = ctr ? ctr + 2 : nullptr;
     fp.padoff = padoff; fp.nseg = nc;
-    for (int q = 0; q < NERR; q++) fp.kf[q] = cfg.kref[q];
+    for (int q = 0; q < NERR; q++) fp.kf[q] = cfg.kref_ens[q];
     fp.fix_rows = fix_slots; fp.nfix = nfix;
     const dim3 fgrid((nslots + 3) / 4);
     if (m_flag)
@@ -1994,7 +2106,7 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
         ChunkFixParams cp;
         cp.ib = d_i; cp.ib_stride = stride; cp.imap = imap;
         cp.rec64 = js.rec64.as<double>(); cp.nj = js.n; cp.clen = 0; cp.nchunk = nc;
-        cp.ni = nslots; cp.nsub = nsub;
+        cp.ni = nslots; cp.nsub = nsub; cp.nseg_c = 1;
         cp.part = part; cp.cnt = cnt; cp.hslot = hslot; cp.caph = caph;
         cp.sslot = sslot; cp.caps = caps;
         cp.items = fitems; cp.nitems = ctr + 1; cp.item_cap = fp.item_cap;

@@ -113,8 +113,39 @@ struct GradLds {
     float dy[GB];
 };
 
+// 64 x 64 x 64 products of the hidden layer on the matrix cores, one wavefront:
+// C[m][n] = sum_k A(m, k) B(k, n), A and B read through the callables (LDS), each
+// element of C handed to out(m, n, c).  v_mfma_f32_16x16x4f32 (FP32 in, FP32
+// accumulate): lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] of a 16 x 4 and
+// a 4 x 16 block, and holds C[4 (l / 16) + i][l % 16], i = 0..3, of the 16 x 16 block.
+typedef float v4f __attribute__((ext_vector_type(4)));
+template <class FA, class FB, class FO>
+__device__ __forceinline__ void gemm64(FA a, FB b, FO out, int lane) {
+    const int r = lane & 15, q = lane >> 4;
+    for (int mt = 0; mt < 4; mt++) {
+        v4f acc[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) acc[nt] = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int kb = 0; kb < NH; kb += 4) {
+            const float av = a(mt * 16 + r, kb + q);
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++)
+                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b(kb + q, nt * 16 + r), acc[nt],
+                                                               0, 0, 0);
+        }
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) out(mt * 16 + q * 4 + i, nt * 16 + r, acc[nt][i]);
+        }
+    }
+}
+
 // Minibatch of n samples: sample s is row idx[base + s] (idx null: base + s) of the
-// SoA features float[8][fstride] and of the targets.
+// SoA features float[8][fstride] and of the targets.  One wavefront per 64 samples:
+// the 8 -> 64 layer and the gradients of W1, b1, W3, b3 on the VALU, the three 64 x 64
+// products of the hidden layer (forward, dW2, back-propagated delta) on the matrix
+// cores.
 __global__ __launch_bounds__(GB) void rsnet_grad_kernel(int n, const float *__restrict__ feat,
                                                         const float *__restrict__ tgt,
                                                         const int *__restrict__ idx, int base,
@@ -129,31 +160,29 @@ __global__ __launch_bounds__(GB) void rsnet_grad_kernel(int n, const float *__re
     const int s = blockIdx.x * GB + t;
     const bool ok = s < n;
     const int g = ok ? (idx ? idx[base + s] : base + s) : 0;
-    float x[NF], h1[NH];
+    float x[NF];
     if (ok) load_features(feat, fstride, g, norm, x);
     else
         for (int f = 0; f < NF; f++) x[f] = 0.f;
     __syncthreads();
-    // forward, keeping h1 and h2 (in the d2 rows) for the backward pass
+    // layer 1 (VALU): h1 of this lane's sample into LDS
 #pragma unroll
     for (int k = 0; k < NH; k++) {
         float z = L.w.b1[k];
 #pragma unroll
         for (int f = 0; f < NF; f++) z = fmaf(L.w.w1[k * NF + f], x[f], z);
-        h1[k] = fmaxf(z, 0.f);
-        L.h1[t][k] = h1[k];
-    }
-    float y = L.w.b3;
-    for (int o = 0; o < NH; o++) {
-        float z = L.w.b2[o];
-#pragma unroll
-        for (int k = 0; k < NH; k++) z = fmaf(L.w.w2[o * (NH + 1) + k], h1[k], z);
-        const float h2 = fmaxf(z, 0.f);
-        L.d2[t][o] = h2;
-        y = fmaf(L.w.w3[o], h2, y);
+        L.h1[t][k] = fmaxf(z, 0.f);
     }
 #pragma unroll
     for (int f = 0; f < NF; f++) L.x[t][f] = x[f];
+    __syncthreads();
+    // layer 2 (matrix cores): h2[s][o] = relu(b2[o] + sum_k h1[s][k] W2[o][k]) into d2
+    gemm64([&](int m, int k) { return L.h1[m][k]; },
+           [&](int k, int o) { return L.w.w2[o * (NH + 1) + k]; },
+           [&](int m, int o, float z) { L.d2[m][o] = fmaxf(z + L.w.b2[o], 0.f); }, t);
+    __syncthreads();
+    float y = L.w.b3;
+    for (int o = 0; o < NH; o++) y = fmaf(L.w.w3[o], L.d2[t][o], y);
     const float r = ok ? y - tgt[g] : 0.f;
     const float dy = 2.0f * r / (float)n;
     L.dy[t] = dy;
@@ -162,9 +191,9 @@ __global__ __launch_bounds__(GB) void rsnet_grad_kernel(int n, const float *__re
     float *P = part + (size_t)blockIdx.x * NPARAM;
     // w3, b3 from h2 (still in d2)
     {
-        float g = 0.f;
-        for (int q = 0; q < GB; q++) g = fmaf(L.dy[q], L.d2[q][t], g);
-        P[OFF_W3 + t] = g;
+        float gw = 0.f;
+        for (int q = 0; q < GB; q++) gw = fmaf(L.dy[q], L.d2[q][t], gw);
+        P[OFF_W3 + t] = gw;
         if (t == 0) {
             float gb = 0.f;
             for (int q = 0; q < GB; q++) gb += L.dy[q];
@@ -178,38 +207,21 @@ __global__ __launch_bounds__(GB) void rsnet_grad_kernel(int n, const float *__re
         L.d2[t][o] = h2 > 0.f ? dy * L.w.w3[o] : 0.f;
     }
     __syncthreads();
-    // W2 (row o = t), b2
+    // dW2[o][k] = sum_s delta2[s][o] h1[s][k] (matrix cores); b2 = sum_s delta2[s][o]
+    gemm64([&](int o, int q) { return L.d2[q][o]; },
+           [&](int q, int k) { return L.h1[q][k]; },
+           [&](int o, int k, float v) { P[OFF_W2 + o * NH + k] = v; }, t);
     {
-        const int o = t;
-        float acc[NH];
-#pragma unroll
-        for (int k = 0; k < NH; k++) acc[k] = 0.f;
         float gb = 0.f;
-        for (int q = 0; q < GB; q++) {
-            const float d = L.d2[q][o];
-            gb += d;
-#pragma unroll
-            for (int k = 0; k < NH; k++) acc[k] = fmaf(d, L.h1[q][k], acc[k]);
-        }
-#pragma unroll
-        for (int k = 0; k < NH; k++) P[OFF_W2 + o * NH + k] = acc[k];
-        P[OFF_B2 + o] = gb;
+        for (int q = 0; q < GB; q++) gb += L.d2[q][t];
+        P[OFF_B2 + t] = gb;
     }
     __syncthreads();
-    // delta1 of this sample = (delta2 W2) [h1 > 0], in place of h1
-    {
-        float d1[NH];
-#pragma unroll
-        for (int k = 0; k < NH; k++) d1[k] = 0.f;
-        for (int o = 0; o < NH; o++) {
-            const float d = L.d2[t][o];
-#pragma unroll
-            for (int k = 0; k < NH; k++) d1[k] = fmaf(d, L.w.w2[o * (NH + 1) + k], d1[k]);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < NH; k++) L.h1[t][k] = h1[k] > 0.f ? d1[k] : 0.f;
-    }
+    // delta1[s][k] = (sum_o delta2[s][o] W2[o][k]) [h1 > 0] (matrix cores), in place of h1:
+    // each element is read (for the mask) and written by the same lane only
+    gemm64([&](int m, int o) { return L.d2[m][o]; },
+           [&](int o, int k) { return L.w.w2[o * (NH + 1) + k]; },
+           [&](int m, int k, float v) { L.h1[m][k] = L.h1[m][k] > 0.f ? v : 0.f; }, t);
     __syncthreads();
     // W1 (row k = t), b1
     {

@@ -63,6 +63,7 @@ def load():
     L.nb6_set_regf_refine.argtypes = [ctypes.c_int]
     L.nb6_set_regf_near.argtypes = [ctypes.c_double]
     L.nb6_set_regf_refine_weights.argtypes = [ctypes.c_double] * 4
+    L.nb6_set_ens_refine_weights.argtypes = [ctypes.c_double] * 4
     L.nb6_set_regcor_pack.argtypes = [ctypes.c_int]
     L.nb6_set_regf_debug_bounds.argtypes = [_vp]
     L.nb6_last_regf_refined.restype = ctypes.c_int
