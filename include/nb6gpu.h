@@ -121,6 +121,18 @@ int nb6_device_count(void);
 int nb6_regf_device(int ni, int ni_stride, const double *d_i, int lmax, int nnbmax,
                     int m_flag, double *d_out, int *d_list, void *stream);
 
+/* nb6_regf_device that allocates nothing (no counterpart in the reference): for
+ * callers that cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  Same arguments, lists and FP64
+ * pass; the work memory is one fixed 256 MiB device array per GPU, held until the
+ * process ends and shared by every call on that GPU, so calls that overlap on one
+ * GPU race.  ni at most 65536 (larger: E_ARG).  Against nb6_regf_device the i-set
+ * is not Morton-ordered and fewer j-chunks may be used, so sums can differ in the
+ * last FP32 roundings; lists are the same.  No kernel timers are recorded.  Needs
+ * a j-set already sent (gpunb_send_ / nb6_send_device allocate). */
+int nb6_regf_device_noalloc(int ni, int ni_stride, const double *d_i, int lmax, int nnbmax,
+                            int m_flag, double *d_out, int *d_list, void *stream);
+
 /* Device-resident j upload: d_j is SoA double[7][nj_stride] (x,y,z,vx,vy,vz,m). */
 int nb6_send_device(int nj, int nj_stride, const double *d_j, void *stream);
 

@@ -236,6 +236,8 @@ void gpunb_open_(int *nbmax, int *irank) {
     g.nbmax = *nbmax;
     jset_reserve(g.js, g.nbmax);
     g.js.n = 0;
+    // device address of the no-alloc form's work array, with this device current
+    if (regf_noalloc_setup() != OK) return;
     g.open = true;
     fprintf(stderr, "# Open GPU regular force - rank: %d; nbmax: %d\n", *irank, *nbmax);
 }
@@ -530,6 +532,20 @@ int nb6_regf_device(int ni, int ni_stride, const double *d_i, int lmax, int nnbm
     g.ini += ni;
     g.icall++;
     return regf_launch(g.js, ni, ni_stride, d_i, lmax, nnbmax, m_flag, d_out, d_list, s);
+}
+
+int nb6_regf_device_noalloc(int ni, int ni_stride, const double *d_i, int lmax, int nnbmax,
+                            int m_flag, double *d_out, int *d_list, void *stream) {
+    Gpunb &g = g_nb;
+    if (!g.open) { set_error(E_STATE, "nb6_regf_device_noalloc: library not open"); return E_STATE; }
+    if (ni < 0 || ni_stride < ni) { set_error(E_ARG, "nb6_regf_device_noalloc: bad ni / stride"); return E_ARG; }
+    if (ni == 0) return OK;
+    Device &d = device();
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : d.stream;
+    g.num_inter += (long long)ni * g.js.n;
+    g.ini += ni;
+    g.icall++;
+    return regf_launch_noalloc(g.js, ni, ni_stride, d_i, lmax, nnbmax, m_flag, d_out, d_list, s);
 }
 
 int nb6_send_device(int nj, int nj_stride, const double *d_j, void *stream) {

@@ -138,6 +138,12 @@ RegfConfig regf_effective(int ni);
 // d_out SoA[7][ni_stride], d_list [ni][lmax].
 int regf_launch(const JSet &js, int ni, int ni_stride, const double *d_i, int lmax,
                 int nnbmax, int m_flag, double *d_out, int *d_list, hipStream_t s);
+// The same pipeline with no allocation: work memory from a fixed device array (see
+// nb6_regf_device_noalloc); ni at most REGF_NOALLOC_NI_MAX.
+constexpr int REGF_NOALLOC_NI_MAX = 65536;
+int regf_noalloc_setup();
+int regf_launch_noalloc(const JSet &js, int ni, int ni_stride, const double *d_i, int lmax,
+                        int nnbmax, int m_flag, double *d_out, int *d_list, hipStream_t s);
 int regf_last_chunks();
 int regf_last_batches();
 // rows of the last call's FP64 pass, and its (row, chunk) items (synchronises)

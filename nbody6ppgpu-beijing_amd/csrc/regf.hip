@@ -2005,6 +2005,56 @@ int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, i
     return OK;
 }
 
+// ------------------------------------------------------------------ no-alloc form
+// Work memory of nb6_regf_device_noalloc: one array per GPU, held until the process
+// ends and shared by every call on that GPU (calls that overlap on one GPU race).
+// Host code never passes `scratch` itself to a kernel: its device address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+constexpr size_t SCRATCH_BYTES = 256ull << 20;
+constexpr size_t SCRATCH_ROWS = 232ull << 20;   // of which the per-(i, chunk) rows
+constexpr int FIX_ITEMS_NA = 512;
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+static unsigned char *g_scratch_ptr[256] = {};
+
+int regf_noalloc_setup() {
+    int dev = -1;
+    NB6_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 256) {
+        set_error(E_NODEV, "nb6_regf_device_noalloc: device index out of range");
+        return E_NODEV;
+    }
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_error(E_STATE, std::string("nb6_regf_device_noalloc: hipGetSymbolAddress: ") +
+                                   hipGetErrorString(e));
+            return E_STATE;
+        }
+        g_scratch_ptr[dev] = static_cast<unsigned char *>(ptr);
+    }
+    return OK;
+}
+
+int regf_launch_noalloc(const JSet &js, int ni, int ni_stride, const double *d_i, int lmax,
+                        int nnbmax, int m_flag, double *d_out, int *d_list, hipStream_t s) {
+    if (ni > REGF_NOALLOC_NI_MAX) {
+        set_error(E_ARG, "nb6_regf_device_noalloc: ni above 65536");
+        return E_ARG;
+    }
+    if (regf_config().prec != 0) {
+        set_error(E_STATE, "nb6_regf_device_noalloc: only the FP32 production kernel");
+        return E_STATE;
+    }
+    const int rc = regf_noalloc_setup();
+    if (rc != OK) return rc;
+    int dev = 0;
+    NB6_HIP(hipGetDevice(&dev));
+    Arena ar{g_scratch_ptr[dev], SCRATCH_BYTES};
+    const NoAlloc na{&ar, SCRATCH_ROWS, FIX_ITEMS_NA};
+    return regf_launch_impl(js, ni, ni_stride, d_i, lmax, nnbmax, m_flag, d_out, d_list, s, &na);
+}
+
 // Batched form (ensemble.hip): one persistent launch over work items, nchunk = 1,
 // work rows indexed by slot, caller rows by imap[slot].
 int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, const double *d_i,

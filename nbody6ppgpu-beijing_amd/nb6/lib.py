@@ -76,6 +76,7 @@ def load():
     L.nb6_memcpy_dtoh.argtypes = [_vp, _vp, ctypes.c_ulonglong]
     L.nb6_regf_device.argtypes = [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, _vp, _vp, _vp]
+    L.nb6_regf_device_noalloc.argtypes = L.nb6_regf_device.argtypes
     L.nb6_send_device.argtypes = [ctypes.c_int, ctypes.c_int, _vp, _vp]
     L.nb6_pot_device.argtypes = [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int,
                                  _vp, _vp]
