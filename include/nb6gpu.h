@@ -161,9 +161,13 @@ int nb6_set_regf_tuning(int isort_min, int far_test);
 int nb6_set_regf_precision(int mode);
 
 /* Average device time (ms) of the last launch of the named kernel family
- * ("regf_tile", "regf_finalize", "pot", "irr") measured with HIP events, and
- * the number of j-chunks the last regf call used. */
+ * ("regf_tile", "regf_finalize", "regf_chunkfix", "regf_fix", "pot", "irr",
+ * "regcor", "energy") measured with HIP events (-1: not recorded), and the number
+ * of j-chunks the last regf call used.  Events are recorded only after
+ * nb6_set_kernel_timing(1) (off by default: each pair leaves the stream idle for
+ * several microseconds, which small calls feel). */
 double nb6_kernel_ms(const char *name);
+int    nb6_set_kernel_timing(int on);
 int    nb6_last_regf_chunks(void);
 
 /* Work-array budget of one regular-force call, in MiB (default 32768).  The work
@@ -195,8 +199,10 @@ int nb6_last_regf_refined_chunks(void);
  * of at most 4096 (the velocity term's linear bound over-flags large j-sets, and a
  * small j-set's rows are cheap to recompute); a call here sets both to its values. */
 int nb6_set_regf_refine_weights(double k0, double k1, double k2, double k3);
-/* The same weights for the batched ensemble step.  Default 0.5, 0, 1, 0: rows are
- * flagged by the acc bound (and the position part of the jerk bound) only. */
+/* The same weights for the batched ensemble step.  Default 0.5, 0.5, 0.5, 0: the
+ * position-rounding term weighs a quarter of the single-call default, which keeps
+ * acc and pot within 1e-6 of FP64 for every particle and jerk at the 99.9th
+ * percentile while recomputing few rows. */
 int nb6_set_ens_refine_weights(double k0, double k1, double k2, double k3);
 
 /* Near field of the FP32 regular-force kernel (no counterpart in the reference):

@@ -63,6 +63,9 @@ struct KTimer {
     double ms();          // synchronises on the stop event
 };
 KTimer &ktimer(const char *name);
+// kernel timers record events only while this is on (nb6_set_kernel_timing)
+bool kernel_timing();
+void set_kernel_timing(bool on);
 
 // ---------------------------------------------------------------- regf
 constexpr int JTILE = 256;   // j-particles per tile (one tile origin each)
@@ -128,7 +131,7 @@ struct RegfConfig {
     // where recomputing a row costs at most small_nj FP64 pairs
     float kref_small[4] = {0.5f, 0.5f, 2.0f, 1.0f};
     int small_nj = 4096;
-    float kref_ens[4] = {0.5f, 0.0f, 1.0f, 0.0f};   // the same for the batched form
+    float kref_ens[4] = {0.5f, 0.5f, 0.5f, 0.0f};   // the same for the batched form
 };
 RegfConfig &regf_config();
 // the configuration a call with ni i-particles runs with (threads resolved)

@@ -739,6 +739,10 @@ struct FinParams {
     int2 *__restrict__ items;
     int *__restrict__ nitems;
     int item_cap;
+    int row_items;                   // > 0: slot i's items at i * row_items (at most that
+                                     // many; a row needing more goes to the whole-row pass),
+                                     // so the choice depends on the row alone; -1: no room,
+                                     // every flagged row whole; 0: appended (batched form)
     int *__restrict__ seg_items;
     int *__restrict__ seg_cnt;
     int4 *__restrict__ rowsel;
@@ -987,7 +991,10 @@ __device__ __forceinline__ void fin_candidates(const FinParams &f, const IF64 &I
 // (s: the row's sums and error scales, the same in every lane).  The row's items are
 // contiguous and in chunk order; a row whose items do not fit the item buffer takes the
 // whole-row pass instead.
-__device__ void emit_chunks(const FinParams &f, int i, int r, const double *s, int lane) {
+constexpr int SHARE_LDS = 2048;   // segments whose shares the wide finalize keeps in LDS
+
+__device__ void emit_chunks(const FinParams &f, int i, int r, const double *s, int lane,
+                            float2 *lds_share = nullptr) {
     if (f.padoff) {
         // batched form: one item, the row's whole cluster (slot i in [padoff[c], padoff[c+1]))
         if (lane == 0) {
@@ -1035,22 +1042,89 @@ __device__ void emit_chunks(const FinParams &f, int i, int r, const double *s, i
             mine = mine || lane == __builtin_ctzll(m);
         }
     }
+    // more than 64 (many short chunks, small i-sets): the segments whose share exceeds
+    // a threshold t, t found by bisection as the largest for which the rest fits the
+    // allowed error (selecting every segment with a share above 0 always fits)
+    // shares relative to the allowed errors (from LDS where the caller gave room: the
+    // bisection below reads them twenty times)
+    const bool cached = lds_share && f.nprow > 64 && f.nprow <= SHARE_LDS;
+    if (cached) {
+        for (int c = lane; c < f.nprow; c += 64) {
+            double ea, ej;
+            bound(c, ea, ej);
+            lds_share[c] = make_float2((float)(ea / (q.ta + 1e-300)), (float)(ej / (q.tj + 1e-300)));
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    // share of segment c; ea, ej: its bounds relative to the allowed errors
+    auto share = [&](int c, double &ea, double &ej) {
+        if (cached) {
+            const float2 w = c < f.nprow ? lds_share[c] : make_float2(0.f, 0.f);
+            ea = w.x;
+            ej = w.y;
+        } else {
+            bound(c, ea, ej);
+            ea = ea / (q.ta + 1e-300);
+            ej = ej / (q.tj + 1e-300);
+        }
+        return fmax(ea, ej);
+    };
+    double tsel = 0.0;
+    if (f.nprow > 64) {
+        double wmax = 0.0;
+        for (int cb = 0; cb < f.nprow; cb += 64) {
+            double ea, ej;
+            wmax = fmax(wmax, share(cb + lane, ea, ej));
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) wmax = fmax(wmax, __shfl_xor(wmax, o, 64));
+        if (!(wmax < 1e300)) {
+            tsel = 0.0;                      // not finite: every segment with a share
+        } else {
+            double lo = 0.0, hi = wmax;      // fits at lo, not at hi (the row is flagged)
+            for (int it = 0; it < 20; it++) {
+                const double t = 0.5 * (lo + hi);
+                double ra = 0.0, rj = 0.0;
+                for (int cb = 0; cb < f.nprow; cb += 64) {
+                    double ea, ej;
+                    if (!(share(cb + lane, ea, ej) > t)) {
+                        ra += ea;
+                        rj += ej;
+                    }
+                }
+                ra = wave_sum(ra);
+                rj = wave_sum(rj);
+                if (ra <= 1.0 && rj <= 1.0) lo = t; else hi = t;
+            }
+            tsel = lo;
+        }
+    }
     auto sel = [&](int c) {
         if (c >= f.nprow) return false;
         if (f.nprow <= 64) return mine;
         double ea, ej;
-        bound(c, ea, ej);
-        return !(ea <= q.ta / f.nprow) || !(ej <= q.tj / f.nprow);
+        return share(c, ea, ej) > tsel;
     };
     int cnt = 0;
     for (int cb = 0; cb < f.nprow; cb += 64) cnt += __popcll(__ballot(sel(cb + lane)));
     if (cnt == 0) return;    // within the bound with every chunk in FP32
     int start = 0;
-    if (lane == 0) start = atomicAdd(f.nitems, cnt);
-    start = __shfl(start, 0, 64);
-    if (start + cnt > f.item_cap) {
-        if (lane == 0) f.fix_rows[atomicAdd(f.nfix, 1)] = r + f.row_base;
-        return;
+    if (f.row_items != 0) {
+        if (cnt > f.row_items) {
+            if (lane == 0) f.fix_rows[atomicAdd(f.nfix, 1)] = r + f.row_base;
+            return;
+        }
+        if (lane == 0) atomicAdd(f.nitems, cnt);   // (a count only)
+        start = i * f.row_items;
+    } else {
+        if (lane == 0) start = atomicAdd(f.nitems, cnt);
+        start = __shfl(start, 0, 64);
+        if (start + cnt > f.item_cap) {
+            if (lane == 0) f.fix_rows[atomicAdd(f.nfix, 1)] = r + f.row_base;
+            return;
+        }
     }
     int k = start;
     for (int cb = 0; cb < f.nprow; cb += 64) {
@@ -1177,6 +1251,7 @@ __global__ __launch_bounds__(64 * W) void regf_finalize_wide_kernel(FinParams f)
     __shared__ int nshared;
     __shared__ int flag;             // 1: chunk items, 2: whole-row pass
     __shared__ double vtot[NPART];
+    __shared__ float2 shr[SHARE_LDS];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int i = blockIdx.x;   // slot
@@ -1260,7 +1335,7 @@ __global__ __launch_bounds__(64 * W) void regf_finalize_wide_kernel(FinParams f)
         double se[NPART];
 #pragma unroll
         for (int q = 0; q < NPART; q++) se[q] = vtot[q];
-        emit_chunks(f, i, r, se, lane);
+        emit_chunks(f, i, r, se, lane, shr);
     }
     if (!ovf && nall > 0) {
         for (int k = threadIdx.x; k < nall; k += 64 * W) {
@@ -1303,8 +1378,8 @@ __device__ __forceinline__ void fix_chunks(int nfix, int nj, int items, int &nch
 }
 
 template <bool MFLAG>
-__global__ __launch_bounds__(256) void regf_fix_kernel(FixParams f) {
-    __shared__ double4 sA[JTILE], sB[JTILE];
+__device__ __forceinline__ void fix_body(const FixParams &f, int bid, int nblk, double4 *sA,
+                                         double4 *sB) {
     const int nfix = *f.nfix;
     if (nfix == 0) return;
     const int nj = f.j1 - f.j0;
@@ -1312,7 +1387,7 @@ __global__ __launch_bounds__(256) void regf_fix_kernel(FixParams f) {
     fix_chunks(nfix, nj, f.items, nch, tpc);
     const int nib = (nfix + 255) / 256;
     const int items = nib * nch;
-    for (int it = blockIdx.x; it < items; it += gridDim.x) {   // uniform per work-group
+    for (int it = bid; it < items; it += nblk) {   // uniform per work-group
         const int b = it % nib, ch = it / nib;
         const int k = b * 256 + threadIdx.x;
         const bool ok = k < nfix;
@@ -1343,9 +1418,15 @@ __global__ __launch_bounds__(256) void regf_fix_kernel(FixParams f) {
     }
 }
 
-__global__ __launch_bounds__(256) void regf_fix_reduce_kernel(FixParams f) {
+template <bool MFLAG>
+__global__ __launch_bounds__(256) void regf_fix_kernel(FixParams f) {
+    __shared__ double4 sA[JTILE], sB[JTILE];
+    fix_body<MFLAG>(f, blockIdx.x, gridDim.x, sA, sB);
+}
+
+__device__ __forceinline__ void fix_reduce_body(const FixParams &f, int bid) {
     const int nfix = *f.nfix;
-    const int k = blockIdx.x * 256 + threadIdx.x;
+    const int k = bid * 256 + threadIdx.x;
     if (k >= nfix) return;
     int nch, tpc;
     fix_chunks(nfix, f.j1 - f.j0, f.items, nch, tpc);
@@ -1357,6 +1438,10 @@ __global__ __launch_bounds__(256) void regf_fix_reduce_kernel(FixParams f) {
     const int r = f.rows[k];
 #pragma unroll
     for (int q = 0; q < 7; q++) f.out[(size_t)q * f.out_stride + r] = sum[q];
+}
+
+__global__ __launch_bounds__(256) void regf_fix_reduce_kernel(FixParams f) {
+    fix_reduce_body(f, blockIdx.x);
 }
 
 // Chunk-selective FP64 pass.  Item k = (slot i, chunk c) replaces the FP32 sums of
@@ -1373,6 +1458,9 @@ __global__ __launch_bounds__(256) void regf_fix_reduce_kernel(FixParams f) {
 // items were emitted.
 constexpr int FIX_SUB = 1024;          // j per sub-range
 constexpr int SEG_MIN = 8192;          // shortest partial-sum segment
+constexpr size_t ITEM_BUDGET = 2ull << 30;   // FP64-pass item memory aimed at per call
+constexpr int MIN_ROW_ITEMS = 8;
+constexpr int PASS_MERGE_NI = 16384;   // batches up to this many i: the FP64 pass in two launches
 
 struct ChunkFixParams {
     const double *__restrict__ ib;   // SoA [8][ib_stride] (as the finalize of the batch)
@@ -1390,6 +1478,7 @@ struct ChunkFixParams {
     const int2 *__restrict__ items;
     const int *__restrict__ nitems;
     int item_cap;
+    int row_items;                   // most items of one row entry (1 in the batched form)
     const int *__restrict__ seg_items;
     const int *__restrict__ seg_cnt;
     double *__restrict__ cand;       // [item_cap][7]
@@ -1417,13 +1506,32 @@ __device__ __forceinline__ IF64 load_if64(const double *__restrict__ ib, int is,
     return I;
 }
 
-// one wavefront per row entry (only rows whose items were all stored), its items in turn
+// one wavefront per item of the row entries (rows whose items were all stored)
 template <bool MFLAG>
-__global__ __launch_bounds__(256) void regf_cand_kernel(ChunkFixParams f) {
+__device__ __forceinline__ void cand_body(const ChunkFixParams &f, int bid, int nblk) {
     const int lane = threadIdx.x & 63;
     const int nr = *f.nrows;
-    for (int e = blockIdx.x * 4 + (threadIdx.x >> 6); e < nr; e += gridDim.x * 4)
-    for (int k = f.rowsel[e].y; k < f.rowsel[e].y + f.rowsel[e].z; k++) {
+    const int ri = max(f.row_items, 1);
+    // (entry, item) pairs in contiguous runs per wavefront, a row's items one after
+    // another; past a row's last item the run goes on at the next row
+    const long long nw = (long long)nr * ri;
+    const long long nwave = (long long)nblk * 4;
+    const long long per = (nw + nwave - 1) / nwave;
+    const long long w0 = (long long)(bid * 4 + (threadIdx.x >> 6)) * per;
+    const long long w1 = min(nw, w0 + per);
+    int ecur = -1;
+    int4 rs = make_int4(0, 0, 0, 0);
+    for (long long w = w0; w < w1; w++) {
+        const int e = (int)(w / ri), m = (int)(w - (long long)e * ri);
+        if (e != ecur) {
+            rs = f.rowsel[e];
+            ecur = e;
+        }
+        if (m >= rs.z) {
+            w = (long long)(e + 1) * ri - 1;   // the next row
+            continue;
+        }
+        const int k = rs.y + m;
         const int2 it = f.items[k];
         const int i = it.x, c = f.padoff ? 0 : it.y / f.nseg_c;   // work-array chunk
         // the segment's j range (the batched form: the whole cluster, any j)
@@ -1451,15 +1559,20 @@ __global__ __launch_bounds__(256) void regf_cand_kernel(ChunkFixParams f) {
 }
 
 template <bool MFLAG>
-__global__ __launch_bounds__(256) void regf_segfix_kernel(ChunkFixParams f) {
-    __shared__ double4 sA[JTILE], sB[JTILE];
+__global__ __launch_bounds__(256) void regf_cand_kernel(ChunkFixParams f) {
+    cand_body<MFLAG>(f, blockIdx.x, gridDim.x);
+}
+
+template <bool MFLAG>
+__device__ __forceinline__ void segfix_body(const ChunkFixParams &f, int bid, int nblk,
+                                            double4 *sA, double4 *sB) {
     const int t = threadIdx.x;
     const int gmax = ((f.padoff ? f.maxseg : f.ni) + 255) / 256;
     const long long units = (long long)f.nchunk * gmax * f.nsub;
     // units ordered (item group, chunk, sub-range): the few busy groups of every chunk
     // come first and spread over the whole grid
     const int per_g = f.nchunk * f.nsub;
-    for (long long u = blockIdx.x; u < units; u += gridDim.x) {   // uniform per work-group
+    for (long long u = bid; u < units; u += nblk) {   // uniform per work-group
         const int g = (int)(u / per_g);
         const int rem = (int)(u - (long long)g * per_g);
         const int c = rem / f.nsub, sub = rem - c * f.nsub;
@@ -1498,13 +1611,21 @@ __global__ __launch_bounds__(256) void regf_segfix_kernel(ChunkFixParams f) {
     }
 }
 
-// each flagged row: its items' corrections in item and sub-range order
-__global__ __launch_bounds__(256) void regf_segapply_kernel(ChunkFixParams f) {
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= *f.nrows) return;
+template <bool MFLAG>
+__global__ __launch_bounds__(256) void regf_segfix_kernel(ChunkFixParams f) {
+    __shared__ double4 sA[JTILE], sB[JTILE];
+    segfix_body<MFLAG>(f, blockIdx.x, gridDim.x, sA, sB);
+}
+
+// each flagged row, one wavefront: lane l sums the corrections of items l, l + 64, ...
+// (sub-ranges in order), then a fixed tree over the lanes
+__device__ __forceinline__ void segapply_body(const ChunkFixParams &f, int bid) {
+    const int lane = threadIdx.x & 63;
+    const int k = bid * 4 + (threadIdx.x >> 6);
+    if (k >= *f.nrows) return;   // whole wavefront
     const int4 e = f.rowsel[k];
     double d[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (int m = 0; m < e.z; m++) {
+    for (int m = lane; m < e.z; m += 64) {
         const int kk = e.y + m;
         const int2 it = f.items[kk];
         const double *pp = f.part + (size_t)(f.padoff ? 0 : it.y) * NPART * f.ni + it.x;
@@ -1517,7 +1638,37 @@ __global__ __launch_bounds__(256) void regf_segapply_kernel(ChunkFixParams f) {
         for (int q = 0; q < 7; q++) d[q] += (v[q] - f.cand[(size_t)kk * 7 + q]) - pp[(size_t)q * f.ni];
     }
 #pragma unroll
-    for (int q = 0; q < 7; q++) f.out[(size_t)q * f.out_stride + e.x] += d[q];
+    for (int q = 0; q < 7; q++) d[q] = wave_sum(d[q]);
+    if (lane < 7) f.out[(size_t)lane * f.out_stride + e.x] += pick7(d, lane);
+}
+
+__global__ __launch_bounds__(256) void regf_segapply_kernel(ChunkFixParams f) {
+    segapply_body(f, blockIdx.x);
+}
+
+// A batch's FP64 pass in two launches (small calls feel every launch): the segment
+// sums, the candidates' FP64 terms and, with the last batch, the whole-row partial
+// sums, in disjoint ranges of work-groups (none reads what another writes; the heavy
+// segment sums first); then the segment corrections and the whole-row results
+// (disjoint rows).
+template <bool MFLAG>
+__global__ __launch_bounds__(256) void regf_pass1_kernel(ChunkFixParams f, FixParams x, int gs,
+                                                         int gc) {
+    __shared__ double4 sA[JTILE], sB[JTILE];
+    const int b = blockIdx.x;   // uniform per work-group
+    if (b < gs)
+        segfix_body<MFLAG>(f, b, gs, sA, sB);
+    else if (b < gs + gc)
+        cand_body<MFLAG>(f, b - gs, gc);
+    else
+        fix_body<MFLAG>(x, b - gs - gc, (int)gridDim.x - gs - gc, sA, sB);
+}
+__global__ __launch_bounds__(256) void regf_pass2_kernel(ChunkFixParams f, FixParams x, int ga) {
+    const int b = blockIdx.x;
+    if (b < ga)
+        segapply_body(f, b);
+    else
+        fix_reduce_body(x, b - ga);
 }
 
 // Ensemble rows: one work-group per flagged slot, over the padded j range of the
@@ -1795,12 +1946,17 @@ int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, i
     // rows the finalize flags for the FP64 pass (FP32 production kernel only)
     const int fix_items = na ? na->fix_items : FIX_ITEMS;
     const size_t fix_cap = std::max((size_t)fix_items * 256, (size_t)ni);
-    // items of the chunk-selective pass: (slot, chunk), candidate terms, FP64 sums per
-    // sub-range; at most 4 per row of a batch (more go to the whole-row pass)
+    // items of the segment-selective pass: (slot, segment), candidate terms, FP64 sums
+    // per sub-range
     const int nsub = (seglen + FIX_SUB - 1) / FIX_SUB;
     const size_t per_item = sizeof(int2) + 7 * sizeof(double) * (1 + (size_t)nsub);
     const int nseg = nchunk * P;
-    size_t item_cap = std::min(rows * P, std::max<size_t>(4 * (size_t)nbatch_i, 65536));
+    // items per row: every segment where the whole call's rows allow it within
+    // ITEM_BUDGET, at least MIN_ROW_ITEMS (from ni, not the batch: a batched call makes
+    // the same choices)
+    int row_items = (int)std::min<size_t>(
+        nseg, std::max<size_t>(MIN_ROW_ITEMS, ITEM_BUDGET / ((size_t)ni * per_item)));
+    size_t item_cap = (size_t)nbatch_i * row_items;
     double *part, *fixpart = nullptr, *cand = nullptr, *part64 = nullptr;
     int *cnt, *hslot, *sslot, *fix_rows = nullptr, *nfix = nullptr, *ctr = nullptr;
     int *seg_items = nullptr;
@@ -1821,7 +1977,8 @@ int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, i
             rowsel = static_cast<int4 *>(na->arena->take((size_t)nbatch_i * sizeof(int4)));
             // the items take what is left
             const size_t left = na->arena->left > ARENA_SLACK ? na->arena->left - ARENA_SLACK : 0;
-            item_cap = std::min(item_cap, left / per_item);
+            row_items = (int)std::min<size_t>(row_items, left / ((size_t)nbatch_i * per_item));
+            item_cap = std::max<size_t>((size_t)nbatch_i * row_items, 1);
             items = static_cast<int2 *>(na->arena->take(item_cap * sizeof(int2)));
             cand = static_cast<double *>(na->arena->take(item_cap * 7 * sizeof(double)));
             part64 = static_cast<double *>(na->arena->take(item_cap * nsub * 7 * sizeof(double)));
@@ -1925,6 +2082,7 @@ int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, i
         fp.nfix = nfix;
         fp.items = items; fp.nitems = nitems;
         fp.item_cap = (int)std::min<size_t>(item_cap, INT32_MAX);
+        fp.row_items = row_items > 0 ? row_items : -1;
         fp.seg_items = seg_items; fp.seg_cnt = seg_cnt;
         fp.padoff = nullptr; fp.nseg = 0;
         fp.dbg = g_dbg ? (imap ? g_dbg : g_dbg + b0) : nullptr;
@@ -1963,44 +2121,63 @@ int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, i
             cp.part = part; cp.cnt = cnt; cp.hslot = hslot; cp.caph = caph;
             cp.sslot = sslot; cp.caps = caps;
             cp.items = items; cp.nitems = nitems; cp.item_cap = fp.item_cap;
+            cp.row_items = std::max(row_items, 1);
             cp.seg_items = seg_items; cp.seg_cnt = seg_cnt;
             cp.cand = cand; cp.part64 = part64;
             cp.rowsel = rowsel; cp.nrows = nrows;
             cp.out = bout; cp.out_stride = ni_stride;
             cp.padoff = nullptr; cp.maxseg = 0;
             if (timed && b0 == 0) t4.start(s);
-            if (m_flag) {
-                hipLaunchKernelGGL(regf_cand_kernel<true>, dim3(CHUNKFIX_GROUPS), dim3(256), 0, s, cp);
-                hipLaunchKernelGGL(regf_segfix_kernel<true>, dim3(CHUNKFIX_GROUPS), dim3(256), 0, s, cp);
+            // whole-row FP64 pass (rows with NaN sums, rows with more segments than
+            // they may store; their count stays on the device) with the last batch,
+            // when every batch has listed its rows
+            const bool last = b0 + nb >= ni;
+            FixParams xp;
+            xp.ib = d_i; xp.ib_stride = ni_stride;
+            xp.rows = fix_rows; xp.nfix = nfix;
+            xp.rec64 = js.rec64.as<double>(); xp.j0 = 0; xp.j1 = nj;
+            xp.part = fixpart;
+            xp.out = d_out; xp.out_stride = ni_stride;
+            xp.items = fix_items;
+            // grids no larger than the work there can be (small calls launch small
+            // grids); whole rows: at most (ni / 256 row groups) x (j tiles) items
+            const int gc = (int)std::min<size_t>(CHUNKFIX_GROUPS,
+                                                 ((size_t)nb * cp.row_items + 3) / 4);
+            const int gs = (int)std::min<size_t>(
+                CHUNKFIX_GROUPS, (size_t)nseg * (((size_t)nb + 255) / 256) * (size_t)nsub);
+            const int gf = last ? (int)std::min<size_t>(fix_items, (((size_t)ni + 255) / 256) *
+                                                      (((size_t)nj + JTILE - 1) / JTILE))
+                                : 0;
+            const int ga = (nb + 3) / 4;
+            const int gr = last ? (ni + 255) / 256 : 0;
+            if (nb <= PASS_MERGE_NI) {
+                // small batches: two launches
+                if (m_flag)
+                    hipLaunchKernelGGL(regf_pass1_kernel<true>, dim3(gs + gc + gf), dim3(256), 0,
+                                       s, cp, xp, gs, gc);
+                else
+                    hipLaunchKernelGGL(regf_pass1_kernel<false>, dim3(gs + gc + gf), dim3(256), 0,
+                                       s, cp, xp, gs, gc);
+                NB6_HIP(hipGetLastError());
+                hipLaunchKernelGGL(regf_pass2_kernel, dim3(ga + gr), dim3(256), 0, s, cp, xp, ga);
             } else {
-                hipLaunchKernelGGL(regf_cand_kernel<false>, dim3(CHUNKFIX_GROUPS), dim3(256), 0, s, cp);
-                hipLaunchKernelGGL(regf_segfix_kernel<false>, dim3(CHUNKFIX_GROUPS), dim3(256), 0, s, cp);
+                // large ones: each part at its own occupancy (the same work, the same bits)
+                if (m_flag) {
+                    hipLaunchKernelGGL(regf_segfix_kernel<true>, dim3(gs), dim3(256), 0, s, cp);
+                    hipLaunchKernelGGL(regf_cand_kernel<true>, dim3(gc), dim3(256), 0, s, cp);
+                    if (gf) hipLaunchKernelGGL(regf_fix_kernel<true>, dim3(gf), dim3(256), 0, s, xp);
+                } else {
+                    hipLaunchKernelGGL(regf_segfix_kernel<false>, dim3(gs), dim3(256), 0, s, cp);
+                    hipLaunchKernelGGL(regf_cand_kernel<false>, dim3(gc), dim3(256), 0, s, cp);
+                    if (gf) hipLaunchKernelGGL(regf_fix_kernel<false>, dim3(gf), dim3(256), 0, s, xp);
+                }
+                NB6_HIP(hipGetLastError());
+                hipLaunchKernelGGL(regf_segapply_kernel, dim3(ga), dim3(256), 0, s, cp);
+                if (gr) hipLaunchKernelGGL(regf_fix_reduce_kernel, dim3(gr), dim3(256), 0, s, xp);
             }
             NB6_HIP(hipGetLastError());
-            hipLaunchKernelGGL(regf_segapply_kernel, dim3((nb + 255) / 256), dim3(256), 0, s, cp);
-            NB6_HIP(hipGetLastError());
-            if (timed && b0 + nb >= ni) t4.stop(s);
+            if (timed && last) t4.stop(s);
         }
-    }
-    if (refine) {
-        // whole-row FP64 pass over the rows with NaN sums (their count stays on the device)
-        FixParams xp;
-        xp.ib = d_i; xp.ib_stride = ni_stride;
-        xp.rows = fix_rows; xp.nfix = nfix;
-        xp.rec64 = js.rec64.as<double>(); xp.j0 = 0; xp.j1 = nj;
-        xp.part = fixpart;
-        xp.out = d_out; xp.out_stride = ni_stride;
-        xp.items = fix_items;
-        KTimer &t3 = na ? dummy : ktimer("regf_fix");
-        if (timed) t3.start(s);
-        if (m_flag)
-            hipLaunchKernelGGL(regf_fix_kernel<true>, dim3(fix_items), dim3(256), 0, s, xp);
-        else
-            hipLaunchKernelGGL(regf_fix_kernel<false>, dim3(fix_items), dim3(256), 0, s, xp);
-        NB6_HIP(hipGetLastError());
-        hipLaunchKernelGGL(regf_fix_reduce_kernel, dim3((ni + 255) / 256), dim3(256), 0, s, xp);
-        NB6_HIP(hipGetLastError());
-        if (timed) t3.stop(s);
     }
     return OK;
 }
@@ -2160,6 +2337,7 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
         cp.part = part; cp.cnt = cnt; cp.hslot = hslot; cp.caph = caph;
         cp.sslot = sslot; cp.caps = caps;
         cp.items = fitems; cp.nitems = ctr + 1; cp.item_cap = fp.item_cap;
+        cp.row_items = 1;
         cp.seg_items = seg_items; cp.seg_cnt = ctr + 6;
         cp.cand = cand; cp.part64 = part64;
         cp.rowsel = rowsel; cp.nrows = ctr + 2;
@@ -2173,7 +2351,7 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
             hipLaunchKernelGGL(regf_segfix_kernel<false>, dim3(CHUNKFIX_GROUPS), dim3(256), 0, s, cp);
         }
         NB6_HIP(hipGetLastError());
-        hipLaunchKernelGGL(regf_segapply_kernel, dim3((nslots + 255) / 256), dim3(256), 0, s, cp);
+        hipLaunchKernelGGL(regf_segapply_kernel, dim3((nslots + 3) / 4), dim3(256), 0, s, cp);
         NB6_HIP(hipGetLastError());
         // rows whose items did not fit and rows with NaN sums: whole rows
         FixRowParams xp;
@@ -2227,6 +2405,8 @@ int regf_autotune(const JSet &js, int ni, int ni_stride, const double *d_i, int 
     float best_ms = 1e30f;
     int bt = 256, bipl = 1, bmg = 4096;
     const int threads[3] = {128, 256, 512};
+    const bool timing = kernel_timing();
+    set_kernel_timing(true);   // the tuner times with the kernel timers
     for (int ti = 0; ti < 3; ti++)
         for (int ipl = 1; ipl <= 2; ipl++)
             for (int mg : {2048, 4096, 8192}) {
@@ -2237,11 +2417,12 @@ int regf_autotune(const JSet &js, int ni, int ni_stride, const double *d_i, int 
                 for (int rep = 0; rep < 2; rep++) {   // the first run warms up
                     const int rc = regf_launch(js, ni, ni_stride, d_i, lmax, nnbmax, m_flag, d_out,
                                                d_list, s);
-                    if (rc != OK) { cfg = keep; return rc; }
+                    if (rc != OK) { cfg = keep; set_kernel_timing(timing); return rc; }
                     ms = (float)(ktimer("regf_tile").ms() + ktimer("regf_finalize").ms());
                 }
                 if (ms < best_ms) { best_ms = ms; bt = threads[ti]; bipl = ipl; bmg = mg; }
             }
+    set_kernel_timing(timing);
     cfg.threads = bt;
     cfg.ipl = bipl;
     cfg.min_groups = bmg;

@@ -57,6 +57,7 @@ def load():
     L.nb6_device_count.restype = ctypes.c_int
     L.nb6_kernel_ms.restype = ctypes.c_double
     L.nb6_kernel_ms.argtypes = [ctypes.c_char_p]
+    L.nb6_set_kernel_timing.argtypes = [ctypes.c_int]
     L.nb6_last_regf_chunks.restype = ctypes.c_int
     L.nb6_last_regf_batches.restype = ctypes.c_int
     L.nb6_set_regf_slot_budget.argtypes = [ctypes.c_int]

@@ -9,6 +9,8 @@
   p(n) ~ 1/n on [1, 2^17]; gpupot over all N.
 * 256 Plummer clusters (N log-uniform in 1024-32768): the batched, graph-replayed
   ensemble step against the FP64 kernel on every particle of every cluster.
+* King W0 = 7, N = 262,144, NNBOPT = 100: the learned neighbour radius trained on
+  block steps of one realisation, then the rule and the network on a second.
 
 Bar (the library's accuracy target): lists identical to the FP64 lists; acc,
 jerk and pot within 1e-6 (per-particle vector relative error) of FP64 for every
@@ -239,8 +241,65 @@ def test_c4_256_clusters_every_particle():
         L.nb6_ens_close()
         for d in (dj, di, do, dl):
             d.free()
-    # the batched step's FP64 pass is driven by the acc bound (nb6_set_ens_refine_weights):
-    # acc and pot within 1e-6 of FP64 for every particle, jerk at the 99.9th percentile
-    # (its tail: the FP32 sums of clusters whose regular jerk nearly cancels)
+    # the batched step's FP64 pass weighs the position-rounding bound less than a single
+    # call does (nb6_set_ens_refine_weights): acc and pot within 1e-6 of FP64 for every
+    # particle, jerk at the 99.9th percentile (its tail: the FP32 sums of clusters whose
+    # regular jerk nearly cancels)
     check_every_particle(f"ensemble {nc} clusters, {ntot} particles", out, lst, ref, rlst,
                          jerk_max=3e-5)
+
+
+# ----------------------------------------------------------------- learned radius
+
+
+def test_c5_learned_radius_king262k():
+    """Learned RS at its stated size: 2 M (features, radius) pairs from block steps
+    of one King W0 = 7 realisation (N = 262,144, NNBOPT = 100, LMAX = 600) under the
+    reference rule train the network on the GPU; a second realisation then runs 60
+    block steps under the rule and under the network (rule as fallback).  After
+    each particle's first regular step (whose radius comes from start-up, the same
+    for both), the network's counts must miss NNBOPT by more than 30 % at least 20
+    times less often than the rule's and sit 5 times closer to it in |ln(c/NNBOPT)|;
+    it must not re-run gpunb_regf for overflow more often than the rule."""
+    from nb6.rsdrive import RSNet, Sim
+    n, nnbopt, lmax = 262144, 100, 600
+    m, x, v = ics.king(n, 20261300, 7.0, masses=ics.kroupa_masses(n, 20261301))
+    rec = []
+    s = Sim(m, x, v, nnbopt=nnbopt, lmax=lmax)
+    try:
+        s.start()
+        while sum(r[1].size for r in rec) < 2_000_000:
+            s.block_step("rule", record=rec)
+    finally:
+        s.close()
+    feat = np.concatenate([r[0] for r in rec])
+    tgt = np.concatenate([r[1] for r in rec])
+    m2, x2, v2 = ics.king(n, 20261310, 7.0, masses=ics.kroupa_masses(n, 20261311))
+    net = RSNet(seed=20261302)
+    st = {}
+    try:
+        hist = net.train(feat, tgt, epochs=4, batch=8192, lr=2e-3)
+        print(f"C5: {tgt.size} pairs, loss {[round(h, 5) for h in hist]}, target var "
+              f"{float(tgt.var()):.4f}", flush=True)
+        for pol in ("rule", "net"):
+            s = Sim(m2, x2, v2, nnbopt=nnbopt, lmax=lmax)
+            try:
+                s.start(pol, net if pol == "net" else None)
+                s.run(60, pol, net=net if pol == "net" else None)
+                st[pol] = dict(s.stats)
+            finally:
+                s.close()
+    finally:
+        net.close()
+    later = {}
+    for pol, q in st.items():
+        ns = max(q["samples_later"], 1)
+        later[pol] = (q["miss30_later"] / ns, q["absdev_later"] / ns)
+        print(f"C5 {pol}: later steps {q['samples_later']} samples, miss30 {later[pol][0]:.4f}, "
+              f"mean |ln(c/NNBOPT)| {later[pol][1]:.4f}; overflow re-runs first step "
+              f"{q['overflow_first']}, later {q['overflow_later']}", flush=True)
+    r, q = st["rule"], st["net"]
+    assert r["samples_later"] > 100_000 and q["samples_later"] > 100_000
+    assert later["net"][0] * 20 < later["rule"][0]
+    assert later["net"][1] * 5 < later["rule"][1]
+    assert q["overflow_events"] <= r["overflow_events"]

@@ -176,11 +176,12 @@ def test_regf_autotune_keeps_results(gpunb):
     _compare(got, ref, "autotuned")
 
 
-@pytest.mark.parametrize("ni", [1500, 4096])
+@pytest.mark.parametrize("ni", [1500, 4096, 20000])
 def test_regf_slot_budget_batches_give_the_same_bits(gpunb, ni):
     """A call whose work arrays exceed the slot budget runs in batches of i-blocks
     (unsorted i-set at ni = 1500, Morton-sorted at 4096); counts, lists and forces
-    are bit-identical to the unbatched call."""
+    are bit-identical to the unbatched call.  At ni = 20000 the unbatched call runs
+    the FP64 pass as separate launches and the small batches as two merged ones."""
     m, x, v, idx, h2, dtr = c2_inputs(n=65536, ni=ni)
     n = m.size
     lmax, nnbmax = 500, 450
