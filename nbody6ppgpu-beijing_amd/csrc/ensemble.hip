@@ -152,6 +152,8 @@ struct Ens {
     RegfWork work;                   // own scratch: a captured graph holds its addresses
     int nitems = 0;
     int items_bi = 0;                // i-block the items were cut for
+    bool layout = false;             // off/padoff, cluster ids and bases on the device match
+                                     // the last send's offsets
     int per_cu = 0;                  // persistent work-groups per CU (0: 8 waves per SIMD)
     // graph of the last regular step and everything it captured by address or value
     hipGraph_t graph = nullptr;
@@ -242,6 +244,7 @@ int ens_close() {
     e.nc = 0;
     e.per_cu = 0;
     e.items_bi = 0;
+    e.layout = false;
     e.open = false;
     return OK;
 }
@@ -260,27 +263,37 @@ int ens_send_device(int nc, const int *offsets, const double *d_soa, int stride,
             set_error(E_ARG, "nb6_ens_send_device: every cluster needs at least one particle");
             return E_ARG;
         }
-    e.nc = nc;
-    e.ntot = offsets[nc];
-    e.off.assign(offsets, offsets + nc + 1);
-    e.padoff.assign(nc + 1, 0);
-    for (int c = 0; c < nc; c++) {
-        const int n = offsets[c + 1] - offsets[c];
-        e.padoff[c + 1] = e.padoff[c] + (n + JTILE - 1) / JTILE * JTILE;
-    }
-    e.npad = e.padoff[nc];
-    e.maxseg = 0;
-    for (int c = 0; c < nc; c++) e.maxseg = std::max(e.maxseg, e.padoff[c + 1] - e.padoff[c]);
-    std::vector<int> cid(e.ntot), jb(e.ntot);
-    for (int c = 0; c < nc; c++)
-        for (int j = offsets[c]; j < offsets[c + 1]; j++) {
-            cid[j] = c;
-            jb[j] = offsets[c];
+    // the cluster layout (padded offsets, per-particle cluster ids and bases, work items)
+    // depends on the offsets alone: an ensemble sent again with the same offsets, as
+    // every step of an integration is, keeps it
+    const bool same_layout = e.layout && e.nc == nc && (int)e.off.size() == nc + 1 &&
+                             std::equal(offsets, offsets + nc + 1, e.off.begin());
+    if (!same_layout) {
+        e.layout = false;
+        e.nc = nc;
+        e.ntot = offsets[nc];
+        e.off.assign(offsets, offsets + nc + 1);
+        e.padoff.assign(nc + 1, 0);
+        for (int c = 0; c < nc; c++) {
+            const int n = offsets[c + 1] - offsets[c];
+            e.padoff[c + 1] = e.padoff[c] + (n + JTILE - 1) / JTILE * JTILE;
         }
-    NB6_HIP(hipMemcpyAsync(e.d_off.p, e.off.data(), (nc + 1) * sizeof(int), hipMemcpyHostToDevice, s));
-    NB6_HIP(hipMemcpyAsync(e.d_padoff.p, e.padoff.data(), (nc + 1) * sizeof(int), hipMemcpyHostToDevice, s));
-    NB6_HIP(hipMemcpyAsync(e.d_cid.p, cid.data(), e.ntot * sizeof(int), hipMemcpyHostToDevice, s));
-    NB6_HIP(hipMemcpyAsync(e.d_jbase.p, jb.data(), e.ntot * sizeof(int), hipMemcpyHostToDevice, s));
+        e.npad = e.padoff[nc];
+        e.maxseg = 0;
+        for (int c = 0; c < nc; c++) e.maxseg = std::max(e.maxseg, e.padoff[c + 1] - e.padoff[c]);
+        std::vector<int> cid(e.ntot), jb(e.ntot);
+        for (int c = 0; c < nc; c++)
+            for (int j = offsets[c]; j < offsets[c + 1]; j++) {
+                cid[j] = c;
+                jb[j] = offsets[c];
+            }
+        NB6_HIP(hipMemcpyAsync(e.d_off.p, e.off.data(), (nc + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+        NB6_HIP(hipMemcpyAsync(e.d_padoff.p, e.padoff.data(), (nc + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+        NB6_HIP(hipMemcpyAsync(e.d_cid.p, cid.data(), e.ntot * sizeof(int), hipMemcpyHostToDevice, s));
+        NB6_HIP(hipMemcpyAsync(e.d_jbase.p, jb.data(), e.ntot * sizeof(int), hipMemcpyHostToDevice, s));
+        // (the copies read host vectors that end with this block)
+        NB6_HIP(hipStreamSynchronize(s));
+    }
     const int n = e.ntot;
     double *in64 = e.js.in64.as<double>();
     hipLaunchKernelGGL(ens_soa_to_aos, dim3((n + 255) / 256), dim3(256), 0, s, d_soa, n, stride, in64);
@@ -319,7 +332,11 @@ int ens_send_device(int nc, const int *offsets, const double *d_soa, int stride,
     double mmax = 0.0;
     for (int c = 0; c < nc; c++) mmax = std::max(mmax, box[(size_t)c * 7 + 6]);
     e.js.mmax = mmax;
-    return build_items(e, s);
+    const RegfConfig cfg = regf_effective(0);
+    if (same_layout && e.items_bi == cfg.threads * cfg.ipl) return OK;
+    const int rc = build_items(e, s);
+    if (rc == OK) e.layout = true;
+    return rc;
 }
 
 int ens_regf_device(const double *d_i, int stride, int lmax, int nnbmax, int m_flag, double *d_out,

@@ -995,26 +995,17 @@ constexpr int SHARE_LDS = 2048;   // segments whose shares the wide finalize kee
 
 __device__ void emit_chunks(const FinParams &f, int i, int r, const double *s, int lane,
                             float2 *lds_share = nullptr) {
+    // batched form: the row's cluster cl (slot i in [padoff[cl], padoff[cl + 1])); its
+    // segments are the cluster's, bucketed per (cluster, segment)
+    int cl = 0;
     if (f.padoff) {
-        // batched form: one item, the row's whole cluster (slot i in [padoff[c], padoff[c+1]))
-        if (lane == 0) {
-            int lo = 0, hi = f.nseg;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (f.padoff[mid] <= i) lo = mid; else hi = mid;
-            }
-            const int k = atomicAdd(f.nitems, 1);
-            if (k >= f.item_cap) {
-                f.fix_rows[atomicAdd(f.nfix, 1)] = f.fix_slot ? i : r + f.row_base;
-            } else {
-                f.items[k] = make_int2(i, lo);
-                const int pos = atomicAdd(&f.seg_cnt[lo], 1);
-                f.seg_items[(size_t)f.padoff[lo] + pos] = k;
-                f.rowsel[atomicAdd(f.nrows, 1)] = make_int4(r, k, 1, 0);
-            }
+        int hi = f.nseg;
+        while (hi - cl > 1) {
+            const int mid = (cl + hi) >> 1;
+            if (f.padoff[mid] <= i) cl = mid; else hi = mid;
         }
-        return;
     }
+    const int fix_id = f.fix_slot ? i : r + f.row_base;
     const RowRef q = row_ref(s);
     auto bound = [&](int c, double &ea, double &ej) {
         ea = ej = 0.0;
@@ -1113,7 +1104,7 @@ __device__ void emit_chunks(const FinParams &f, int i, int r, const double *s, i
     int start = 0;
     if (f.row_items != 0) {
         if (cnt > f.row_items) {
-            if (lane == 0) f.fix_rows[atomicAdd(f.nfix, 1)] = r + f.row_base;
+            if (lane == 0) f.fix_rows[atomicAdd(f.nfix, 1)] = fix_id;
             return;
         }
         if (lane == 0) atomicAdd(f.nitems, cnt);   // (a count only)
@@ -1122,7 +1113,7 @@ __device__ void emit_chunks(const FinParams &f, int i, int r, const double *s, i
         if (lane == 0) start = atomicAdd(f.nitems, cnt);
         start = __shfl(start, 0, 64);
         if (start + cnt > f.item_cap) {
-            if (lane == 0) f.fix_rows[atomicAdd(f.nfix, 1)] = r + f.row_base;
+            if (lane == 0) f.fix_rows[atomicAdd(f.nfix, 1)] = fix_id;
             return;
         }
     }
@@ -1133,9 +1124,18 @@ __device__ void emit_chunks(const FinParams &f, int i, int r, const double *s, i
         const unsigned long long m = __ballot(b);
         if (b) {
             const int kk = k + lane_prefix(m);
-            f.items[kk] = make_int2(i, c);
-            const int pos = atomicAdd(&f.seg_cnt[c], 1);
-            f.seg_items[(size_t)c * f.ni + pos] = kk;
+            if (f.padoff) {
+                // (cluster, segment) g; bucket g holds at most the cluster's slots
+                const int g = cl * f.nprow + c;
+                f.items[kk] = make_int2(i, g);
+                const int pos = atomicAdd(&f.seg_cnt[g], 1);
+                f.seg_items[(size_t)f.padoff[cl] * f.nprow +
+                            (size_t)c * (f.padoff[cl + 1] - f.padoff[cl]) + pos] = kk;
+            } else {
+                f.items[kk] = make_int2(i, c);
+                const int pos = atomicAdd(&f.seg_cnt[c], 1);
+                f.seg_items[(size_t)c * f.ni + pos] = kk;
+            }
         }
         k += __popcll(m);
     }
@@ -1424,20 +1424,32 @@ __global__ __launch_bounds__(256) void regf_fix_kernel(FixParams f) {
     fix_body<MFLAG>(f, blockIdx.x, gridDim.x, sA, sB);
 }
 
+// element q of v[7] without indexing by a lane-varying value
+__device__ __forceinline__ double pick7(const double *v, int q) {
+    double r = v[0];
+#pragma unroll
+    for (int k = 1; k < 7; k++) r = q == k ? v[k] : r;
+    return r;
+}
+
+// one wavefront per whole row: lane l sums chunks l, l + 64, ... in order, then a
+// fixed tree over the lanes
 __device__ __forceinline__ void fix_reduce_body(const FixParams &f, int bid) {
     const int nfix = *f.nfix;
-    const int k = bid * 256 + threadIdx.x;
-    if (k >= nfix) return;
+    const int lane = threadIdx.x & 63;
+    const int k = bid * 4 + (threadIdx.x >> 6);
+    if (k >= nfix) return;   // whole wavefront
     int nch, tpc;
     fix_chunks(nfix, f.j1 - f.j0, f.items, nch, tpc);
     double sum[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (int ch = 0; ch < nch; ch++) {
+    for (int ch = lane; ch < nch; ch += 64) {
 #pragma unroll
         for (int q = 0; q < 7; q++) sum[q] += f.part[((size_t)ch * 7 + q) * nfix + k];
     }
-    const int r = f.rows[k];
 #pragma unroll
-    for (int q = 0; q < 7; q++) f.out[(size_t)q * f.out_stride + r] = sum[q];
+    for (int q = 0; q < 7; q++) sum[q] = wave_sum(sum[q]);
+    const int r = f.rows[k];
+    if (lane < 7) f.out[(size_t)lane * f.out_stride + r] = pick7(sum, lane);
 }
 
 __global__ __launch_bounds__(256) void regf_fix_reduce_kernel(FixParams f) {
@@ -1458,7 +1470,9 @@ __global__ __launch_bounds__(256) void regf_fix_reduce_kernel(FixParams f) {
 // items were emitted.
 constexpr int FIX_SUB = 1024;          // j per sub-range
 constexpr int SEG_MIN = 8192;          // shortest partial-sum segment
-constexpr size_t ITEM_BUDGET = 2ull << 30;   // FP64-pass item memory aimed at per call
+constexpr int SEG_ENS = 4096;          // partial-sum segment of a cluster (batched form)
+constexpr int SEGCNT_LDS = 4096;       // bucket sizes the segment-sum kernel keeps in LDS
+constexpr size_t ITEM_BUDGET = 4ull << 30;   // FP64-pass item memory aimed at per call
 constexpr int MIN_ROW_ITEMS = 8;
 constexpr int PASS_MERGE_NI = 16384;   // batches up to this many i: the FP64 pass in two launches
 
@@ -1490,13 +1504,6 @@ struct ChunkFixParams {
     const int *__restrict__ padoff;  // batched form: segment c = cluster [padoff[c],
     int maxseg;                      // padoff[c + 1]), work-array chunk 0; maxseg its
 };                                   // longest segment
-
-__device__ __forceinline__ double pick7(const double *v, int q) {
-    double r = v[0];
-#pragma unroll
-    for (int k = 1; k < 7; k++) r = q == k ? v[k] : r;
-    return r;
-}
 
 __device__ __forceinline__ IF64 load_if64(const double *__restrict__ ib, int is, int r) {
     IF64 I;
@@ -1534,9 +1541,17 @@ __device__ __forceinline__ void cand_body(const ChunkFixParams &f, int bid, int 
         const int k = rs.y + m;
         const int2 it = f.items[k];
         const int i = it.x, c = f.padoff ? 0 : it.y / f.nseg_c;   // work-array chunk
-        // the segment's j range (the batched form: the whole cluster, any j)
-        const unsigned jlo = f.padoff ? 0u : (unsigned)(it.y * f.clen);
-        const unsigned jhi = f.padoff ? (unsigned)f.nj : (unsigned)min(f.nj, (it.y + 1) * f.clen);
+        // the segment's j range (the batched form: segment it.y % nseg_c of cluster
+        // it.y / nseg_c)
+        unsigned jlo, jhi;
+        if (f.padoff) {
+            const int cl = it.y / f.nseg_c, sg = it.y - cl * f.nseg_c;
+            jlo = (unsigned)(f.padoff[cl] + sg * f.clen);
+            jhi = (unsigned)min(f.padoff[cl + 1], f.padoff[cl] + (sg + 1) * f.clen);
+        } else {
+            jlo = (unsigned)(it.y * f.clen);
+            jhi = (unsigned)min(f.nj, (it.y + 1) * f.clen);
+        }
         const IF64 I = load_if64(f.ib, f.ib_stride, f.imap ? f.imap[i] : i);
         double cd[7] = {0, 0, 0, 0, 0, 0, 0};
         const size_t rowc = (size_t)c * f.ni + i;
@@ -1565,9 +1580,25 @@ __global__ __launch_bounds__(256) void regf_cand_kernel(ChunkFixParams f) {
 
 template <bool MFLAG>
 __device__ __forceinline__ void segfix_body(const ChunkFixParams &f, int bid, int nblk,
-                                            double4 *sA, double4 *sB) {
+                                            double4 *sA, double4 *sB, int *scnt) {
     const int t = threadIdx.x;
-    const int gmax = ((f.padoff ? f.maxseg : f.ni) + 255) / 256;
+    int gmax = ((f.padoff ? f.maxseg : f.ni) + 255) / 256;
+    // bucket sizes into LDS (scnt[SEGCNT_LDS]: their maximum): the unit loop then reads
+    // them there and ends after the last group that any bucket fills
+    const bool cached = f.nchunk <= SEGCNT_LDS;
+    if (cached) {
+        if (t == 0) scnt[SEGCNT_LDS] = 0;
+        __syncthreads();
+        int mx = 0;
+        for (int c = t; c < f.nchunk; c += 256) {
+            const int v = f.seg_cnt[c];
+            scnt[c] = v;
+            mx = max(mx, v);
+        }
+        atomicMax(&scnt[SEGCNT_LDS], mx);
+        __syncthreads();
+        gmax = min(gmax, (scnt[SEGCNT_LDS] + 255) / 256);
+    }
     const long long units = (long long)f.nchunk * gmax * f.nsub;
     // units ordered (item group, chunk, sub-range): the few busy groups of every chunk
     // come first and spread over the whole grid
@@ -1576,15 +1607,26 @@ __device__ __forceinline__ void segfix_body(const ChunkFixParams &f, int bid, in
         const int g = (int)(u / per_g);
         const int rem = (int)(u - (long long)g * per_g);
         const int c = rem / f.nsub, sub = rem - c * f.nsub;
-        const int nc = f.seg_cnt[c];
+        const int nc = cached ? scnt[c] : f.seg_cnt[c];
         if (g * 256 >= nc) continue;
         // (the last chunk or a short cluster: trailing sub-ranges are empty and write 0)
-        const int sb = f.padoff ? f.padoff[c] : c * f.clen;
-        const int se = f.padoff ? f.padoff[c + 1] : min(f.nj, (c + 1) * f.clen);
+        int sb, se;
+        size_t bucket;
+        if (f.padoff) {
+            // segment sg of cluster cl
+            const int cl = c / f.nseg_c, sg = c - cl * f.nseg_c;
+            const int p0 = f.padoff[cl], p1 = f.padoff[cl + 1];
+            sb = p0 + sg * f.clen;
+            se = min(p1, sb + f.clen);
+            bucket = (size_t)p0 * f.nseg_c + (size_t)sg * (p1 - p0);
+        } else {
+            sb = c * f.clen;
+            se = min(f.nj, (c + 1) * f.clen);
+            bucket = (size_t)c * f.ni;
+        }
         const int j0 = sb + sub * FIX_SUB;
         const int j1 = min(se, j0 + FIX_SUB);
         const bool ok = g * 256 + t < nc;
-        const size_t bucket = f.padoff ? (size_t)sb : (size_t)c * f.ni;
         const int k = ok ? f.seg_items[bucket + g * 256 + t] : 0;
         IF64 I;
         if (ok) {
@@ -1614,36 +1656,40 @@ __device__ __forceinline__ void segfix_body(const ChunkFixParams &f, int bid, in
 template <bool MFLAG>
 __global__ __launch_bounds__(256) void regf_segfix_kernel(ChunkFixParams f) {
     __shared__ double4 sA[JTILE], sB[JTILE];
-    segfix_body<MFLAG>(f, blockIdx.x, gridDim.x, sA, sB);
+    __shared__ int scnt[SEGCNT_LDS + 1];
+    segfix_body<MFLAG>(f, blockIdx.x, gridDim.x, sA, sB, scnt);
 }
 
 // each flagged row, one wavefront: lane l sums the corrections of items l, l + 64, ...
 // (sub-ranges in order), then a fixed tree over the lanes
-__device__ __forceinline__ void segapply_body(const ChunkFixParams &f, int bid) {
+__device__ __forceinline__ void segapply_body(const ChunkFixParams &f, int bid, int nblk) {
     const int lane = threadIdx.x & 63;
-    const int k = bid * 4 + (threadIdx.x >> 6);
-    if (k >= *f.nrows) return;   // whole wavefront
-    const int4 e = f.rowsel[k];
-    double d[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (int m = lane; m < e.z; m += 64) {
-        const int kk = e.y + m;
-        const int2 it = f.items[kk];
-        const double *pp = f.part + (size_t)(f.padoff ? 0 : it.y) * NPART * f.ni + it.x;
-        double v[7] = {0, 0, 0, 0, 0, 0, 0};
-        for (int sb = 0; sb < f.nsub; sb++) {
+    const int nr = *f.nrows;
+    for (int k = bid * 4 + (threadIdx.x >> 6); k < nr; k += nblk * 4) {   // whole wavefronts
+        const int4 e = f.rowsel[k];
+        double d[7] = {0, 0, 0, 0, 0, 0, 0};
+        for (int m = lane; m < e.z; m += 64) {
+            const int kk = e.y + m;
+            const int2 it = f.items[kk];
+            // the partial-sum row: the segment (the batched form: within its cluster)
+            const int prow = f.padoff ? it.y % f.nseg_c : it.y;
+            const double *pp = f.part + (size_t)prow * NPART * f.ni + it.x;
+            double v[7] = {0, 0, 0, 0, 0, 0, 0};
+            for (int sb = 0; sb < f.nsub; sb++) {
 #pragma unroll
-            for (int q = 0; q < 7; q++) v[q] += f.part64[((size_t)kk * f.nsub + sb) * 7 + q];
+                for (int q = 0; q < 7; q++) v[q] += f.part64[((size_t)kk * f.nsub + sb) * 7 + q];
+            }
+#pragma unroll
+            for (int q = 0; q < 7; q++) d[q] += (v[q] - f.cand[(size_t)kk * 7 + q]) - pp[(size_t)q * f.ni];
         }
 #pragma unroll
-        for (int q = 0; q < 7; q++) d[q] += (v[q] - f.cand[(size_t)kk * 7 + q]) - pp[(size_t)q * f.ni];
+        for (int q = 0; q < 7; q++) d[q] = wave_sum(d[q]);
+        if (lane < 7) f.out[(size_t)lane * f.out_stride + e.x] += pick7(d, lane);
     }
-#pragma unroll
-    for (int q = 0; q < 7; q++) d[q] = wave_sum(d[q]);
-    if (lane < 7) f.out[(size_t)lane * f.out_stride + e.x] += pick7(d, lane);
 }
 
 __global__ __launch_bounds__(256) void regf_segapply_kernel(ChunkFixParams f) {
-    segapply_body(f, blockIdx.x);
+    segapply_body(f, blockIdx.x, gridDim.x);
 }
 
 // A batch's FP64 pass in two launches (small calls feel every launch): the segment
@@ -1655,9 +1701,10 @@ template <bool MFLAG>
 __global__ __launch_bounds__(256) void regf_pass1_kernel(ChunkFixParams f, FixParams x, int gs,
                                                          int gc) {
     __shared__ double4 sA[JTILE], sB[JTILE];
+    __shared__ int scnt[SEGCNT_LDS + 1];
     const int b = blockIdx.x;   // uniform per work-group
     if (b < gs)
-        segfix_body<MFLAG>(f, b, gs, sA, sB);
+        segfix_body<MFLAG>(f, b, gs, sA, sB, scnt);
     else if (b < gs + gc)
         cand_body<MFLAG>(f, b - gs, gc);
     else
@@ -1666,7 +1713,7 @@ __global__ __launch_bounds__(256) void regf_pass1_kernel(ChunkFixParams f, FixPa
 __global__ __launch_bounds__(256) void regf_pass2_kernel(ChunkFixParams f, FixParams x, int ga) {
     const int b = blockIdx.x;
     if (b < ga)
-        segapply_body(f, b);
+        segapply_body(f, b, ga);
     else
         fix_reduce_body(x, b - ga);
 }
@@ -2044,7 +2091,7 @@ int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, i
         int *blist = imap ? d_list : d_list + (size_t)b0 * lmax;
         const int bnib = (nb + nbi - 1) / nbi;
 
-        TileParams tp;
+        TileParams tp{};
         tp.ib = bi; tp.ni = nb; tp.ib_stride = ni_stride; tp.imap = bmap;
         tp.rec32 = js.rec32.as<float4>(); tp.torg = js.torg.as<double>(); tp.nj = nj;
         tp.tbox = js.tbox.as<float>();
@@ -2064,7 +2111,7 @@ int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, i
         NB6_HIP(hipGetLastError());
         if (timed && b0 + nb >= ni) t1.stop(s);
 
-        FinParams fp;
+        FinParams fp{};
         fp.ib = bi; fp.ni = nb; fp.ib_stride = ni_stride; fp.imap = bmap;
         fp.rec64 = js.rec64.as<double>(); fp.perm = js.perm.as<int>(); fp.jbase = nullptr;
         fp.nj = nj;
@@ -2114,7 +2161,7 @@ int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, i
         if (refine) {
             // chunk-selective FP64 pass of this batch's flagged rows, while its work
             // arrays (partials, slots) are still there
-            ChunkFixParams cp;
+            ChunkFixParams cp{};
             cp.ib = bi; cp.ib_stride = ni_stride; cp.imap = bmap;
             cp.rec64 = js.rec64.as<double>(); cp.nj = nj; cp.clen = seglen; cp.nchunk = nseg;
             cp.ni = nb; cp.nsub = nsub; cp.nseg_c = P;
@@ -2132,7 +2179,7 @@ int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, i
             // they may store; their count stays on the device) with the last batch,
             // when every batch has listed its rows
             const bool last = b0 + nb >= ni;
-            FixParams xp;
+            FixParams xp{};
             xp.ib = d_i; xp.ib_stride = ni_stride;
             xp.rows = fix_rows; xp.nfix = nfix;
             xp.rec64 = js.rec64.as<double>(); xp.j0 = 0; xp.j1 = nj;
@@ -2149,7 +2196,7 @@ int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, i
                                                       (((size_t)nj + JTILE - 1) / JTILE))
                                 : 0;
             const int ga = (nb + 3) / 4;
-            const int gr = last ? (ni + 255) / 256 : 0;
+            const int gr = last ? (ni + 3) / 4 : 0;
             if (nb <= PASS_MERGE_NI) {
                 // small batches: two launches
                 if (m_flag)
@@ -2251,14 +2298,21 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
     NB6_HIP(hipStreamIsCapturing(s, &cs));
     const bool capturing = cs != hipStreamCaptureStatusNone;
     const bool refine = cfg.refine != 0;
-    // FP64 pass: one item per flagged row (its whole cluster), items bucketed by cluster
-    const int nsub = (maxseg + FIX_SUB - 1) / FIX_SUB;
+    // FP32 partial sums per segment of SEG_ENS j of a cluster (whole tiles of every
+    // work-group width; at most 64 per cluster); FP64 pass: items (row, segment of its
+    // cluster), bucketed per (cluster, segment)
+    int sege = SEG_ENS;
+    while ((maxseg + sege - 1) / sege > 64) sege *= 2;
+    const int nsegc = refine ? std::max(1, (maxseg + sege - 1) / sege) : 1;
+    const int nsub = (sege + FIX_SUB - 1) / FIX_SUB;
     const size_t item_cap = std::max<size_t>(rows / 2, 1);
-    const size_t ctr_bytes = (6 + (size_t)nc) * sizeof(int);
-    if (capturing && (w.part.bytes < rows * NPART * sizeof(double) || w.hslot.bytes < rows * caph * sizeof(int) ||
+    const size_t nbk = (size_t)nc * nsegc;
+    const size_t ctr_bytes = (6 + nbk) * sizeof(int);
+    if (capturing && (w.part.bytes < rows * nsegc * NPART * sizeof(double) ||
+                      w.hslot.bytes < rows * caph * sizeof(int) ||
                       w.sslot.bytes < rows * caps * sizeof(int) || w.cnt.bytes < rows * 2 * sizeof(int) ||
                       (refine && (w.fixrows.bytes < rows * sizeof(int) || w.nfix.bytes < ctr_bytes ||
-                                  w.segitems.bytes < rows * sizeof(int) ||
+                                  w.segitems.bytes < rows * nsegc * sizeof(int) ||
                                   w.items.bytes < item_cap * sizeof(int2) ||
                                   w.delta.bytes < item_cap * 7 * sizeof(double) ||
                                   w.part64.bytes < item_cap * nsub * 7 * sizeof(double) ||
@@ -2266,7 +2320,7 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
         set_error(E_STATE, "regf_items_launch: work buffers must be sized before stream capture");
         return E_STATE;
     }
-    double *part = static_cast<double *>(w.part.get(rows * NPART * sizeof(double)));
+    double *part = static_cast<double *>(w.part.get(rows * nsegc * NPART * sizeof(double)));
     int *cnt = static_cast<int *>(w.cnt.get(rows * 2 * sizeof(int)));
     int *hslot = static_cast<int *>(w.hslot.get(rows * caph * sizeof(int)));
     int *sslot = static_cast<int *>(w.sslot.get(rows * caps * sizeof(int)));
@@ -2277,15 +2331,16 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
     if (refine) {
         fix_slots = static_cast<int *>(w.fixrows.get(rows * sizeof(int)));
         ctr = static_cast<int *>(w.nfix.get(ctr_bytes));
-        seg_items = static_cast<int *>(w.segitems.get(rows * sizeof(int)));
+        seg_items = static_cast<int *>(w.segitems.get(rows * nsegc * sizeof(int)));
         fitems = static_cast<int2 *>(w.items.get(item_cap * sizeof(int2)));
         cand = static_cast<double *>(w.delta.get(item_cap * 7 * sizeof(double)));
         part64 = static_cast<double *>(w.part64.get(item_cap * nsub * 7 * sizeof(double)));
         rowsel = static_cast<int4 *>(w.rowsel.get(rows * sizeof(int4)));
-        nfix = ctr;   // [0] whole rows, [1] items, [2] rows with items, [6 ..] per cluster
-        hipLaunchKernelGGL(zero_ints_kernel, dim3((6 + nc + 255) / 256), dim3(256), 0, s, ctr, 6 + nc);
+        nfix = ctr;   // [0] whole rows, [1] items, [2] rows with items, [6 ..] per bucket
+        hipLaunchKernelGGL(zero_ints_kernel, dim3((unsigned)((6 + nbk + 255) / 256)), dim3(256), 0,
+                           s, ctr, (int)(6 + nbk));
     }
-    TileParams tp;
+    TileParams tp{};
     tp.ib = d_i; tp.ni = nslots; tp.ib_stride = stride; tp.imap = imap;
     tp.rec32 = js.rec32.as<float4>(); tp.torg = js.torg.as<double>(); tp.nj = js.n;
     tp.tbox = js.tbox.as<float>();
@@ -2296,18 +2351,18 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
     tp.part = part; tp.cnt = cnt;
     tp.hslot = hslot; tp.caph = caph; tp.sslot = sslot; tp.caps = caps;
     tp.fnear = cfg.near;
-    tp.nseg_c = 1;
-    tp.seg_tiles = 0;
+    tp.nseg_c = nsegc;
+    tp.seg_tiles = nsegc > 1 ? sege / cfg.threads : 0;
     KTimer &t1 = ktimer("regf_items");
     if (!capturing) t1.start(s);
     launch_items(cfg, m_flag != 0, grid, tp, items, nitems, queue, s);
     NB6_HIP(hipGetLastError());
     if (!capturing) t1.stop(s);
-    FinParams fp;
+    FinParams fp{};
     fp.ib = d_i; fp.ni = nslots; fp.ib_stride = stride; fp.imap = imap;
     fp.rec64 = js.rec64.as<double>(); fp.perm = js.perm.as<int>(); fp.jbase = jbase; fp.nj = js.n;
     fp.nchunk = 1;
-    fp.nprow = 1;
+    fp.nprow = nsegc;
     fp.part = part; fp.cnt = cnt;
     fp.hslot = hslot; fp.caph = caph; fp.sslot = sslot; fp.caps = caps;
     fp.lmax = lmax; fp.nnbmax = nnbeff;
@@ -2319,6 +2374,7 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
     fp.dbg = g_dbg;
     fp.rowsel = rowsel; fp.nrows = ctr ? ctr + 2 : nullptr;
     fp.padoff = padoff; fp.nseg = nc;
+    fp.row_items = 0;                    // items appended (up to item_cap)
     for (int q = 0; q < NERR; q++) fp.kf[q] = cfg.kref_ens[q];
     fp.fix_rows = fix_slots; fp.nfix = nfix;
     const dim3 fgrid((nslots + 3) / 4);
@@ -2330,14 +2386,14 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
     if (refine) {
         // FP64 pass of the flagged rows: items bucketed by cluster, each cluster's records
         // through LDS once per 256 of its rows
-        ChunkFixParams cp;
+        ChunkFixParams cp{};
         cp.ib = d_i; cp.ib_stride = stride; cp.imap = imap;
-        cp.rec64 = js.rec64.as<double>(); cp.nj = js.n; cp.clen = 0; cp.nchunk = nc;
-        cp.ni = nslots; cp.nsub = nsub; cp.nseg_c = 1;
+        cp.rec64 = js.rec64.as<double>(); cp.nj = js.n; cp.clen = sege; cp.nchunk = (int)nbk;
+        cp.ni = nslots; cp.nsub = nsub; cp.nseg_c = nsegc;
         cp.part = part; cp.cnt = cnt; cp.hslot = hslot; cp.caph = caph;
         cp.sslot = sslot; cp.caps = caps;
         cp.items = fitems; cp.nitems = ctr + 1; cp.item_cap = fp.item_cap;
-        cp.row_items = 1;
+        cp.row_items = nsegc;            // a row's items: at most one per segment
         cp.seg_items = seg_items; cp.seg_cnt = ctr + 6;
         cp.cand = cand; cp.part64 = part64;
         cp.rowsel = rowsel; cp.nrows = ctr + 2;
@@ -2351,10 +2407,11 @@ int regf_items_launch(const JSet &js, int nslots, const int *imap, int stride, c
             hipLaunchKernelGGL(regf_segfix_kernel<false>, dim3(CHUNKFIX_GROUPS), dim3(256), 0, s, cp);
         }
         NB6_HIP(hipGetLastError());
-        hipLaunchKernelGGL(regf_segapply_kernel, dim3((nslots + 3) / 4), dim3(256), 0, s, cp);
+        hipLaunchKernelGGL(regf_segapply_kernel, dim3(std::min((nslots + 3) / 4, 16384)), dim3(256),
+                           0, s, cp);
         NB6_HIP(hipGetLastError());
         // rows whose items did not fit and rows with NaN sums: whole rows
-        FixRowParams xp;
+        FixRowParams xp{};
         xp.ib = d_i; xp.ib_stride = stride; xp.imap = imap;
         xp.slots = fix_slots; xp.nfix = nfix;
         xp.padoff = padoff; xp.nc = nc;

@@ -224,7 +224,9 @@ def test_c4_256_clusters_every_particle():
         out = do.to_host(np.float64, (7, ntot))
         lst = dl.to_host(np.int32, (ntot, lmax))
         L.nb6_ens_close()
-        # FP64 kernel, one cluster at a time
+        # FP64 kernel, one cluster at a time, through the single-call path (opened here:
+        # it must not depend on another test having left it open)
+        g = lib.Gpunb(int(sizes.max()))
         L.nb6_set_regf_precision(2)
         try:
             for c in range(nc):
@@ -234,6 +236,7 @@ def test_c4_256_clusters_every_particle():
                                          dl.ptr + 4 * lmax * a, None) == 0
         finally:
             L.nb6_set_regf_precision(0)
+            g.close()
         L.nb6_sync()
         ref = do.to_host(np.float64, (7, ntot))
         rlst = dl.to_host(np.int32, (ntot, lmax))
