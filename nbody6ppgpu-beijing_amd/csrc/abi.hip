@@ -121,6 +121,22 @@ void *DBuf::get(size_t need) {
     }
     return p;
 }
+void *DBuf::try_get(size_t need) {
+    if (need == 0) need = 1;
+    if (need > bytes) {
+        if (p) NB6_HIP(hipFree(p));
+        p = nullptr;
+        bytes = 0;
+        const size_t nb = need + need / 8;
+        if (hipMalloc(&p, nb) != hipSuccess) {
+            (void)hipGetLastError();   // clear the sticky error of the failed allocation
+            p = nullptr;
+            return nullptr;
+        }
+        bytes = nb;
+    }
+    return p;
+}
 void DBuf::release() {
     if (p) (void)hipFree(p);
     p = nullptr;

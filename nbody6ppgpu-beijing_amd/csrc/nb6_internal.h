@@ -41,6 +41,7 @@ struct DBuf {
     void *p = nullptr;
     size_t bytes = 0;
     void *get(size_t need);
+    void *try_get(size_t need);   // nullptr (and no buffer) where the device has no room
     void release();
     template <class T> T *as() const { return static_cast<T *>(p); }
 };

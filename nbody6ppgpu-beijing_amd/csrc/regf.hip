@@ -2037,19 +2037,28 @@ int regf_launch_impl(const JSet &js, int ni, int ni_stride, const double *d_i, i
             return E_ARG;
         }
     } else {
-        part = static_cast<double *>(g_work.part.get(rows * P * NPART * sizeof(double)));
-        cnt = static_cast<int *>(g_work.cnt.get(rows * 2 * sizeof(int)));
-        hslot = static_cast<int *>(g_work.hslot.get(rows * caph * sizeof(int)));
-        sslot = static_cast<int *>(g_work.sslot.get(rows * caps * sizeof(int)));
+        // the work arrays grow with the call (14 GB at ni = 131,072 against 2^20 j): a
+        // device that cannot hold them gets an error, not an abort
+        part = static_cast<double *>(g_work.part.try_get(rows * P * NPART * sizeof(double)));
+        cnt = static_cast<int *>(g_work.cnt.try_get(rows * 2 * sizeof(int)));
+        hslot = static_cast<int *>(g_work.hslot.try_get(rows * caph * sizeof(int)));
+        sslot = static_cast<int *>(g_work.sslot.try_get(rows * caps * sizeof(int)));
         if (refine) {
-            fix_rows = static_cast<int *>(g_work.fixrows.get((size_t)ni * sizeof(int)));
-            ctr = static_cast<int *>(g_work.nfix.get((6 + (size_t)nseg) * sizeof(int)));
-            fixpart = static_cast<double *>(g_work.fixpart.get(fix_cap * 7 * sizeof(double)));
-            seg_items = static_cast<int *>(g_work.segitems.get(rows * P * sizeof(int)));
-            items = static_cast<int2 *>(g_work.items.get(item_cap * sizeof(int2)));
-            cand = static_cast<double *>(g_work.delta.get(item_cap * 7 * sizeof(double)));
-            part64 = static_cast<double *>(g_work.part64.get(item_cap * nsub * 7 * sizeof(double)));
-            rowsel = static_cast<int4 *>(g_work.rowsel.get((size_t)nbatch_i * sizeof(int4)));
+            fix_rows = static_cast<int *>(g_work.fixrows.try_get((size_t)ni * sizeof(int)));
+            ctr = static_cast<int *>(g_work.nfix.try_get((6 + (size_t)nseg) * sizeof(int)));
+            fixpart = static_cast<double *>(g_work.fixpart.try_get(fix_cap * 7 * sizeof(double)));
+            seg_items = static_cast<int *>(g_work.segitems.try_get(rows * P * sizeof(int)));
+            items = static_cast<int2 *>(g_work.items.try_get(item_cap * sizeof(int2)));
+            cand = static_cast<double *>(g_work.delta.try_get(item_cap * 7 * sizeof(double)));
+            part64 = static_cast<double *>(g_work.part64.try_get(item_cap * nsub * 7 * sizeof(double)));
+            rowsel = static_cast<int4 *>(g_work.rowsel.try_get((size_t)nbatch_i * sizeof(int4)));
+        }
+        if (!part || !cnt || !hslot || !sslot ||
+            (refine && (!fix_rows || !ctr || !fixpart || !seg_items || !items || !cand ||
+                        !part64 || !rowsel))) {
+            set_error(E_STATE, "gpunb_regf: device memory for the work arrays is not available "
+                               "(lower nb6_set_regf_slot_budget)");
+            return E_STATE;
         }
     }
     int *seg_cnt = ctr ? ctr + 6 : nullptr;   // items per chunk of the current batch

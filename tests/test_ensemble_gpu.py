@@ -106,3 +106,72 @@ def test_ensemble_autotune_keeps_results():
     assert rows_as_sets(lst) == rows_as_sets(base[1])
     ea = vec_rel_err(out[0:3].T, base[0][0:3].T)
     assert ea.max() < 1e-6
+
+
+def _same_rows(a, b):
+    """Counts equal and, where non-negative, the neighbours they cover (entries past
+    a row's count are not written)."""
+    if not np.array_equal(a[:, 0], b[:, 0]):
+        return False
+    c = np.maximum(a[:, 0], 0)
+    mask = np.arange(a.shape[1] - 1)[None, :] < c[:, None]
+    return bool(np.array_equal(np.where(mask, a[:, 1:], 0), np.where(mask, b[:, 1:], 0)))
+
+
+def test_ensemble_resend_same_offsets_equals_a_fresh_send():
+    """A second send with the same offsets keeps the cluster layout and work items
+    (an integration re-sends every step): the graph-replayed step on the moved
+    particles gives the same bits as a freshly opened ensemble sent those positions;
+    a send with different offsets rebuilds the layout."""
+    sizes = [1500, 300, 5000, 64, 2600]
+    ms, xs, vs, h2s, dts, off = make_ensemble(sizes, seed=91)
+    lmax, nnbmax = 200, 150
+    # the same clusters a step later (positions drifted along their velocities)
+    xs2 = [x + 0.01 * v for x, v in zip(xs, vs)]
+    L = lib.load()
+    ntot = int(off[-1])
+
+    def arrays(xl):
+        m, x, v = np.concatenate(ms), np.concatenate(xl), np.concatenate(vs)
+        h2, dtr = np.concatenate(h2s), np.concatenate(dts)
+        dj = lib.DeviceArray.from_host(np.ascontiguousarray(np.concatenate([x.T, v.T, m[None]])))
+        di = lib.DeviceArray.from_host(np.ascontiguousarray(np.concatenate([x.T, v.T, h2[None], dtr[None]])))
+        return dj, di
+
+    dj1, di1 = arrays(xs)
+    dj2, di2 = arrays(xs2)
+    do = lib.DeviceArray(7 * ntot * 8)
+    dl = lib.DeviceArray(ntot * lmax * 4)
+    offp = off.ctypes.data_as(lib._ip)
+    try:
+        # fresh ensemble, second positions
+        assert L.nb6_ens_open(len(sizes), ntot) == 0
+        assert L.nb6_ens_send_device(len(sizes), offp, dj2.ptr, ntot, None) == 0
+        assert L.nb6_ens_regf_device(di2.ptr, ntot, lmax, nnbmax, 0, do.ptr, dl.ptr, 1, None) == 0
+        L.nb6_sync()
+        want = (do.to_host(np.float64, (7, ntot)), dl.to_host(np.int32, (ntot, lmax)))
+        L.nb6_ens_close()
+        # first positions, then the second with the same offsets on the kept layout
+        assert L.nb6_ens_open(len(sizes), ntot) == 0
+        assert L.nb6_ens_send_device(len(sizes), offp, dj1.ptr, ntot, None) == 0
+        assert L.nb6_ens_regf_device(di1.ptr, ntot, lmax, nnbmax, 0, do.ptr, dl.ptr, 1, None) == 0
+        assert L.nb6_ens_send_device(len(sizes), offp, dj2.ptr, ntot, None) == 0
+        assert L.nb6_ens_regf_device(di2.ptr, ntot, lmax, nnbmax, 0, do.ptr, dl.ptr, 1, None) == 0
+        L.nb6_sync()
+        got = (do.to_host(np.float64, (7, ntot)), dl.to_host(np.int32, (ntot, lmax)))
+        assert np.array_equal(got[0], want[0])
+        assert _same_rows(got[1], want[1])
+        # different offsets (two clusters merged): the layout is rebuilt, results stay
+        # those of a direct computation on each cluster
+        off3 = np.array([0, sizes[0] + sizes[1], ntot - sizes[4] - sizes[3], ntot - sizes[4], ntot],
+                        dtype=np.int32)
+        assert L.nb6_ens_send_device(4, off3.ctypes.data_as(lib._ip), dj2.ptr, ntot, None) == 0
+        assert L.nb6_ens_regf_device(di2.ptr, ntot, lmax, nnbmax, 0, do.ptr, dl.ptr, 1, None) == 0
+        L.nb6_sync()
+        lst3 = dl.to_host(np.int32, (ntot, lmax))
+        a, b = int(off3[3]), int(off3[4])    # the last cluster is unchanged
+        assert _same_rows(lst3[a:b], want[1][a:b])
+    finally:
+        L.nb6_ens_close()
+        for d in (dj1, di1, dj2, di2, do, dl):
+            d.free()
