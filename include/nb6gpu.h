@@ -182,14 +182,15 @@ int nb6_set_regf_slot_budget(int mib);
 int nb6_last_regf_batches(void);
 
 /* FP64 pass of the regular force (default on; no counterpart in the reference).
- * The FP32 kernel also sums, per row and j-chunk, error scales of its FP32 partial
- * sums and of the FP32 rounding of its separations; the finalize flags every row
- * whose bound could put its acc or jerk more than 1e-6 (relative) from FP64, and the
- * chunks of that row whose bound exceeds 1/nchunk of the allowed error are recomputed
- * in FP64 (same neighbour test, so lists are unchanged); rows whose FP32 sums are
- * incomplete (NaN) are recomputed over all j.  nb6_last_regf_refined() returns the
- * rows of the last call's FP64 pass, nb6_last_regf_refined_chunks() its (row, chunk)
- * items; both synchronise the device. */
+ * The FP32 kernel also sums, per row and j-segment, error scales of its FP32
+ * partial sums and of the FP32 rounding of its separations; the finalize flags every
+ * row whose bound could put its acc or jerk more than 1e-6 (relative) from FP64 and
+ * selects the segments with the largest shares of that bound until the rest is within
+ * it; those segments are recomputed in FP64 (same neighbour test, so lists are
+ * unchanged); rows whose FP32 sums are incomplete (NaN), or that need more segments
+ * than they may store, are recomputed over all j.  nb6_last_regf_refined() returns
+ * the rows of the last call's FP64 pass, nb6_last_regf_refined_chunks() its (row,
+ * segment) items; both synchronise the device. */
 int nb6_set_regf_refine(int on);
 int nb6_last_regf_refined(void);
 int nb6_last_regf_refined_chunks(void);
