@@ -388,6 +388,27 @@ def test_regf_ragged_j_and_resend(gpunb):
         _compare(got, ref, f"ragged nj={nj}")
 
 
+def test_kernel_timers_only_when_asked_and_same_results(gpunb):
+    """Kernel event timers are off by default (nb6_kernel_ms reports -1 for a call
+    made without them) and, switched on, time the call without changing its bits."""
+    m, x, v, idx, h2, dtr = c2_inputs(n=16384, ni=600)
+    L = lib.load()
+    g = gpunb(m.size)
+    g.send(m, x, v)
+    L.nb6_set_kernel_timing(0)
+    off = g.regf(h2, dtr, x[idx], v[idx], 300, 250)
+    assert L.nb6_kernel_ms(b"regf_tile") == -1.0
+    L.nb6_set_kernel_timing(1)
+    try:
+        on = g.regf(h2, dtr, x[idx], v[idx], 300, 250)
+        t = L.nb6_kernel_ms(b"regf_tile")
+    finally:
+        L.nb6_set_kernel_timing(0)
+    assert 0.0 < t < 1000.0
+    for a, b in zip(off, on):
+        assert np.array_equal(a, b)
+
+
 def test_regf_device_entry_point_matches_host_abi(gpunb):
     """nb6_regf_device (inputs resident in HBM) gives the same bits as gpunb_regf_."""
     m, x, v, idx, h2, dtr = c2_inputs(n=16384, ni=1000)
