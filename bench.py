@@ -240,15 +240,16 @@ def bench_ensemble(steps, warmup):
                           "weights_batched_and_looped": list(ENS_REFINE)}}
 
 
-def bench_learned_rs(train_samples, eval_steps, no_cpu):
-    """Learned neighbour radius: King W0=7, N=262144, NNBOPT=100, LMAX=600.
+def bench_learned_rs(train_samples, eval_steps, no_cpu, lmax=600):
+    """Learned neighbour radius: King W0=7, N=262144, NNBOPT=100, LMAX=600 (or `lmax`;
+    NNBMAX = LMAX - 50).
     Pairs (features, radius giving NNBOPT neighbours) come from regular block steps
     of one realisation under the reference rule; the network trains on the GPU;
     then a second realisation runs the same number of block steps under the rule
     and under the network (rule as fallback), counting the util_gpu.F overflow
     re-runs of gpunb_regf."""
     from nb6.rsdrive import RSNet, Sim
-    n, nnbopt, lmax = 262144, 100, 600
+    n, nnbopt = 262144, 100
     t = time.time()
     m, x, v = ics.king(n, SEED + 50, 7.0, masses=ics.kroupa_masses(n, SEED + 51))
     rec = []
@@ -304,7 +305,7 @@ def bench_learned_rs(train_samples, eval_steps, no_cpu):
               "miss30_ratio_later_steps": (split(r)["later_miss30_frac"] /
                                            max(split(q)["later_miss30_frac"], 1e-12))}
     return {"metric": "neighbour-list overflow retries: reference RS rule / learned RS "
-                      "(King W0=7 N=262144, NNBOPT=100)",
+                      f"(King W0=7 N=262144, NNBOPT=100, LMAX={lmax})",
             "value": ratio, "unit": "x fewer gpunb_regf re-runs", "n_gpus": 1,
             "start_up_vs_steady_state": steady,
             "higher_is_better": True, "dtype": "bf16 weights / fp32 accumulate",
@@ -962,6 +963,9 @@ def main():
                          "learned-rs: overflow retries of the RS network vs the reference rule")
     ap.add_argument("--rs-train-samples", type=int, default=10_000_000)
     ap.add_argument("--rs-eval-steps", type=int, default=300)
+    ap.add_argument("--rs-lmax", type=int, default=600,
+                    help="learned-rs: list length (NNBMAX = LMAX - 50); 200 makes the rule "
+                         "overflow after start-up too")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one plain copy kernel of MIB MiB (byte-counter calibration)")
     args = ap.parse_args()
@@ -1001,7 +1005,7 @@ def main():
         return 0
     if args.workload == "learned-rs":
         print(json.dumps(bench_learned_rs(args.rs_train_samples, args.rs_eval_steps,
-                                          args.no_cpu)), flush=True)
+                                          args.no_cpu, args.rs_lmax)), flush=True)
         return 0
 
     m, x, v, idx, h2, dtr = make_workload()

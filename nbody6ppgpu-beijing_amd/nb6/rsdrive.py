@@ -12,7 +12,8 @@ Follows the reference's regular cycle:
     (util_gpu.F:65-100);
   * the RS for the particle's next regular step comes from the reference rule
     (regcor_gpu.F:95-238, on the device) or from the RS network (rsnet.hip) with
-    the rule as fallback.
+    the rule as fallback.  The network aims at the count the rule aims at: NNBOPT,
+    or 0.9 NBAVE where that is smaller (short lists, see `net_aim`).
 
 The trajectory is integrated with individual power-of-two regular steps and a
 fourth-order Hermite predictor-corrector on the full force (a gpunb_regf call
@@ -72,6 +73,7 @@ class Sim:
         self.nnbopt, self.lmax = nnbopt, lmax
         self.nnbmax = min(n // 2, lmax - 50)                  # input.F:132
         self.znbmin = max(0.01 * self.nnbmax, 1.0)            # input.F:133
+        self.aim = net_aim(nnbopt, self.nnbmax, self.znbmin)
         self.smax, self.dtmin, self.eta, self.maxthr = smax, dtmin, eta, maxthr
         self.g = lib.Gpunb(n)
         self.d = Dev(n, lmax)
@@ -186,7 +188,7 @@ class Sim:
         self.rs_used_prev[:] = self.rs
         self.has_prev[:] = True
         if policy == "net":
-            self.rs = net.predict(feats, self.rs, self.rs, np.zeros(self.n, np.int32))
+            self.rs = net.predict(feats, self.rs * self.aim, self.rs, np.zeros(self.n, np.int32))
             self.used_net[:] = True
         self.stats["calls"] = 0
 
@@ -312,7 +314,7 @@ class Sim:
         if policy == "net":
             use_rule = (self.used_net[act] & (dev > 0.3)).astype(np.int32)
             self.stats["fallback"] += int(use_rule.sum())
-            rs_next = net.predict(feats, rs, rs_rule, use_rule)
+            rs_next = net.predict(feats, rs * self.aim, rs_rule, use_rule)
             self.used_net[act] = use_rule == 0
         else:
             rs_next = rs_rule
@@ -388,6 +390,18 @@ class Sim:
         xp, vp = self.predict(self.t)
         zkin, pot, _, _ = lib.energy(xp, vp, self.m, np.zeros((self.n, 1), np.int32), self.n, 0, 0.0)
         return zkin - pot
+
+
+def net_aim(nnbopt, nnbmax, znbmin):
+    """Radius factor that moves the network's aim from NNBOPT to the rule's own target.
+
+    The rule caps the count it aims at to 0.9 NBAVE, NBAVE = min(NNBOPT + 150,
+    max(NNBMAX - 50, ZNBMIN)) (regcor_gpu.F:144-145): below NNBOPT when the list is
+    short (NNBMAX < NNBOPT / 0.9 + 50), e.g. 90 at NNBOPT = 100, LMAX = 200.  The
+    network predicts the radius holding NNBOPT neighbours; at a locally uniform
+    density the radius holding k is that times (k / NNBOPT)^(1/3)."""
+    nbave = min(nnbopt + 150, max(nnbmax - 50, int(znbmin)))
+    return (min(float(nnbopt), 0.9 * nbave) / nnbopt) ** (1.0 / 3.0)
 
 
 def _overflow(rs, nnb, nnbopt, nnbmax):

@@ -170,7 +170,10 @@ def test_fortran_caller_through_the_reference_interface(tmp_path):
     nnb = r["lists"][np.searchsorted(d["ireg"], d["iaddr"]), 0]
     assert irr["Ncall"] == 1 and irr["NI"] == int(nnb.sum())   # pairs per call
     assert abs(irr["NB"] - nnb.mean()) <= 1e-5 * nnb.mean()
-    assert irr["Gflops"] > 0
+    # wall time of the call (irr.avx.cpp:551,561-562), whether or not kernel timers are on;
+    # usec = 1e6 grav / Ncall, both printed with six decimals
+    assert irr["grav"] > 0 and irr["Gflops"] > 0
+    assert abs(irr["usec"] - 1e6 * irr["grav"] / irr["Ncall"]) <= 1.0 + 1e-3 * irr["usec"]
 
 
 def _profile_lines(err):
@@ -185,5 +188,6 @@ def _profile_lines(err):
     m = re.search(r"\[R\.0 GPU Irr\.F \] Ncall: (\d+) <NI>: (\d+) <NB>: (\S+) grav: (\S+) s, "
                   r"(\S+) Gflops, (\S+) usec", err)
     assert m, "no irr_simd_profile_ line"
-    out["irr"] = dict(Ncall=int(m[1]), NI=int(m[2]), NB=float(m[3]), Gflops=float(m[5]))
+    out["irr"] = dict(Ncall=int(m[1]), NI=int(m[2]), NB=float(m[3]), grav=float(m[4]),
+                      Gflops=float(m[5]), usec=float(m[6]))
     return out

@@ -152,3 +152,19 @@ def test_gradient_is_a_mean(n):
     l1, g1 = R.loss_and_grad(p, x, t)
     l2, g2 = R.loss_and_grad(p, np.concatenate([x, x]), np.concatenate([t, t]))
     assert np.isclose(l1, l2) and np.allclose(g1, g2)
+
+
+@pytest.mark.parametrize("nnbmax", [150, 550])
+def test_network_aim_is_the_rules_fixed_point(nnbmax):
+    """The RS network aims at NNBOPT aim^3 neighbours (nb6.rsdrive.net_aim): the count
+    at which the rule leaves RS unchanged inside the core; fewer neighbours grow RS,
+    more shrink it.  Short lists (NNBMAX = 150): 0.9 NBAVE = 90 < NNBOPT; long: NNBOPT."""
+    from nb6.rsdrive import net_aim
+    znbmin = max(0.01 * nnbmax, 1.0)
+    k = int(round(100 * net_aim(100, nnbmax, znbmin) ** 3))
+    assert k == (90 if nnbmax == 150 else 100)
+    kw = dict(nnbmax=nnbmax, znbmin=znbmin)
+    x, v = [0.1, 0.0, 0.0], [0.0, 0.3, 0.0]
+    assert one(0.05, k, k, x, v, 0.01, **kw) == 0.05
+    assert one(0.05, k - 20, k - 20, x, v, 0.01, **kw) > 0.05
+    assert one(0.05, k + 20, k + 20, x, v, 0.01, **kw) < 0.05
