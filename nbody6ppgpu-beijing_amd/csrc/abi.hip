@@ -3,6 +3,7 @@
 // nb6_* extension entry points.  See include/nb6gpu.h for the contract of each
 // symbol and the reference lines it replaces.
 #include "nb6_internal.h"
+#include "host_threads.h"
 #include "../../include/nb6gpu.h"
 
 #include <sys/time.h>
@@ -210,19 +211,6 @@ static bool check_finite(const double *a, size_t n) {
     return true;
 }
 
-// spread a host loop over a few threads (list assembly and packing)
-template <class F>
-static void par_for(int n, F f) {
-    const int nt = n > 4096 ? std::min(8, (int)std::thread::hardware_concurrency()) : 1;
-    if (nt <= 1) { f(0, n); return; }
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; t++) {
-        const int a = (int)((long long)n * t / nt), b = (int)((long long)n * (t + 1) / nt);
-        th.emplace_back([=] { f(a, b); });
-    }
-    for (auto &x : th) x.join();
-}
-
 const JSet *gpunb_jset() { return g_nb.open ? &g_nb.js : nullptr; }
 
 }  // namespace nb6
@@ -295,55 +283,11 @@ void gpunb_send_(int *nj, double mj[], double xj[][3], double vj[][3]) {
     // NaN check, in one pass spread over a few threads (one range each)
     double *h = static_cast<double *>(g.hj.get((size_t)8 * std::max(n, 1) * sizeof(double)));
     const int nt = n > 65536 ? std::max(1, std::min(8, (int)std::thread::hardware_concurrency())) : 1;
-    std::vector<double> part((size_t)nt * 7);
-    std::vector<char> nan(nt, 0);
-    auto pack = [&](int t) {
-        const int a = (int)((long long)n * t / nt), b = (int)((long long)n * (t + 1) / nt);
-        double bx[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
-        double mm = 0.0;
-        bool bad = false;
-        for (int j = a; j < b; j++) {
-            double *r = h + (size_t)8 * j;
-            r[0] = xj[j][0]; r[1] = xj[j][1]; r[2] = xj[j][2]; r[3] = mj[j];
-            r[4] = vj[j][0]; r[5] = vj[j][1]; r[6] = vj[j][2]; r[7] = 0.0;
-            for (int k = 0; k < 7; k++) bad = bad || !(r[k] == r[k]);
-            for (int k = 0; k < 3; k++) {
-                bx[k] = r[k] < bx[k] ? r[k] : bx[k];
-                bx[3 + k] = r[k] > bx[3 + k] ? r[k] : bx[3 + k];
-            }
-            mm = mj[j] > mm ? mj[j] : mm;
-        }
-        for (int k = 0; k < 6; k++) part[(size_t)t * 7 + k] = bx[k];
-        part[(size_t)t * 7 + 6] = mm;
-        nan[t] = bad;
-    };
-    if (nt == 1) {
-        pack(0);
-    } else {
-        // a thread that cannot be created (std::system_error, e.g. a process limit)
-        // leaves its range, and every later one, to the calling thread; every thread
-        // that was created is joined, so no joinable std::thread is ever destroyed
-        std::vector<std::thread> th;
-        int t = 0;
-        try {
-            for (; t < nt; t++) th.emplace_back(pack, t);
-        } catch (...) {
-        }
-        for (int r = t; r < nt; r++) pack(r);
-        for (auto &x : th) x.join();
-    }
-    double box[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+    double box[6];
     double mmax = 0.0;
-    for (int t = 0; t < nt; t++) {
-        if (nan[t]) {
-            set_error(E_ARG, "gpunb_send: NaN in j-particle data");
-            return;
-        }
-        for (int k = 0; k < 3; k++) {
-            box[k] = std::min(box[k], part[(size_t)t * 7 + k]);
-            box[3 + k] = std::max(box[3 + k], part[(size_t)t * 7 + 3 + k]);
-        }
-        mmax = std::max(mmax, part[(size_t)t * 7 + 6]);
+    if (!pack_j_records(n, nt, mj, xj, vj, h, box, &mmax)) {
+        set_error(E_ARG, "gpunb_send: NaN in j-particle data");
+        return;
     }
     g.time_send -= tstart;   // the pack counts as send time
     g.isend++;
