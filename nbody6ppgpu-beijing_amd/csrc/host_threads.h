@@ -11,10 +11,10 @@ namespace nb6 {
 
 // Run f(t, a, b) for t in [0, nt) over the contiguous ranges [a, b) that split
 // [0, n), one std::thread each; nt <= 1 runs f(0, 0, n) on the calling thread.  A
-// thread that cannot be created (std::system_error, e.g. a process or thread
-// limit) leaves its range, and every later one, to the calling thread; every
-// created thread is joined before this returns, so no joinable std::thread is
-// destroyed and every range has run.
+// thread that cannot be created (std::system_error under a process or thread
+// limit, or std::bad_alloc) leaves its range, and every later one, to the calling
+// thread; every created thread is joined before this returns, so no joinable
+// std::thread is destroyed and every range has run.
 template <class F>
 inline void run_ranges(int nt, int n, F f) {
     if (nt <= 1) {
@@ -25,9 +25,9 @@ inline void run_ranges(int nt, int n, F f) {
         f(t, (int)((long long)n * t / nt), (int)((long long)n * (t + 1) / nt));
     };
     std::vector<std::thread> th;
-    th.reserve(nt);
     int t = 0;
     try {
+        th.reserve(nt);
         for (; t < nt; t++) th.emplace_back(range, t);
     } catch (...) {
     }
