@@ -208,16 +208,19 @@ def test_features_closed_form():
     assert np.isclose(f[1, 1], 0.0)
 
 
-def test_learned_radius_beats_the_rule_on_a_short_run():
+@pytest.mark.parametrize("lmax", [300, 150])
+def test_learned_radius_beats_the_rule_on_a_short_run(lmax):
     """King W0 = 7, N = 16384, NNBOPT = 50: pairs from regular block steps under
     the rule train the network; then, from the same start, the network (rule as
     fallback) keeps counts closer to NNBOPT than the rule does, and does not
-    overflow more."""
+    overflow more.  LMAX = 150 is a short list (NNBMAX = 100): both policies aim at
+    0.9 NBAVE = 45 neighbours there."""
     n = 16384
     m, x, v = ics.king(n, 31, 7.0, masses=ics.kroupa_masses(n, 32))
     rec = []
-    s = Sim(m, x, v, nnbopt=50, lmax=300)
+    s = Sim(m, x, v, nnbopt=50, lmax=lmax)
     try:
+        assert np.isclose(s.aim ** 3 * 50, 50 if lmax == 300 else 45)
         s.start()
         s.run(120, "rule", record=rec)
     finally:
@@ -231,7 +234,7 @@ def test_learned_radius_beats_the_rule_on_a_short_run():
         print("train", len(tgt), "pairs, loss", [round(h, 4) for h in hist], "target var", float(tgt.var()))
         stats = {}
         for pol in ("rule", "net"):
-            s = Sim(m, x, v, nnbopt=50, lmax=300)
+            s = Sim(m, x, v, nnbopt=50, lmax=lmax)
             try:
                 s.start(pol, net if pol == "net" else None)
                 st = s.run(120, pol, net=net if pol == "net" else None)
@@ -245,6 +248,8 @@ def test_learned_radius_beats_the_rule_on_a_short_run():
                                        "overflow_by_step")},
               "mean |ln(c/NNBOPT)|", st["absdev"] / st["samples"])
     r, q = stats["rule"], stats["net"]
-    assert q["absdev"] / q["samples"] < 0.7 * r["absdev"] / r["samples"]
+    # on the short list ln(50/45) = 0.105 of each policy's mean is the shared aim itself
+    # (measured: 0.118 against 0.164 there, 40 overflow re-runs of the rule against none)
+    assert q["absdev"] / q["samples"] < (0.7 if lmax == 300 else 1.0) * r["absdev"] / r["samples"]
     assert q["miss30"] / q["samples"] < r["miss30"] / r["samples"]
     assert q["overflow_events"] <= r["overflow_events"]
