@@ -290,7 +290,10 @@ void nb6_jp_regf_(int *ni, int addr[], double h2[], double dtr[], double acc[][3
  *     overflow rows skipped; the same effect as IRR_SIMD_SET_LIST with the list the
  *     caller would have built.
  *   nb6_irr_firr_device(ti, ni, d_addr, d_out, stream): irr_simd_firr_vec_ with
- *     device addresses and d_out double[ni][8] = acc[3], jrk[3], nnbid (1-based), nnb. */
+ *     device addresses and d_out double[ni][8] = acc[3], jrk[3], nnbid (1-based), nnb.
+ *     Asynchronous on the stream, so it is not counted in irr_simd_profile_'s line
+ *     (that line covers irr_simd_firr_vec_ calls, whose pairs and time are known on
+ *     return). */
 /* Launch shape of the irregular-force gather: 16 or 64 lanes per active particle,
  * and whether blocks of >= 2048 active particles are processed in Morton order of
  * their positions (results unchanged).  Defaults: 64, 1. */
